@@ -1,0 +1,620 @@
+// ns3d_api.cpp -- the C ABI of include/ns3d_gpu.h over the HIP kernels.
+//
+// A context holds one periodic box resident in HBM, its own HIP stream and
+// the scratch it needs.  Every entry point returns a status; no exception
+// leaves this file.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/ns3d_gpu.h"
+#include "ns3d_internal.hpp"
+
+using namespace ns3d;
+
+struct ns3d_ctx {
+    ns3d_params P{};
+    ns3d_grid G{};
+    DevParams dp{};
+    Fields f{};
+    FaceArrays faces{};          // allocated on first ns3d_du_inviscous
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *slab = nullptr;      // one allocation for all cell arrays
+    double *face_slab = nullptr;
+    unsigned long long *d_flag = nullptr;   // [0] non-physical key, [1] CFL bits
+    double *d_partial = nullptr;            // per-block reduction partials
+    double **d_ptrs = nullptr;              // 10 device-side pointer slots
+    int nblocks = 0;
+    long long ncell = 0;
+    int seg_len[3] = {64, 64, 64};
+    unsigned long long bad_key = ~0ull;
+    long long steps_done = 0;
+    std::string err;
+    // per-kernel timing with HIP events on this context's stream
+    struct Pending { int kind; hipEvent_t a, b; };
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<Pending> pending;
+    std::vector<std::string> kind_name;
+    std::vector<double> kind_ms;
+    std::vector<long long> kind_n;
+};
+
+namespace {
+
+constexpr int kCellArrays = 6 + 5 + 5 + 12 + 12;
+
+int hip_fail(ns3d_ctx *c, hipError_t e, const char *where)
+{
+    if (c) c->err = std::string(where) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? NS3D_ERR_OOM : NS3D_ERR_HIP;
+}
+
+#define HIPCALL(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+hipEvent_t take_event(ns3d_ctx *c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int kind_id(ns3d_ctx *c, const char *name)
+{
+    for (size_t k = 0; k < c->kind_name.size(); ++k)
+        if (c->kind_name[k] == name) return (int)k;
+    c->kind_name.push_back(name);
+    c->kind_ms.push_back(0.0);
+    c->kind_n.push_back(0);
+    return (int)c->kind_name.size() - 1;
+}
+
+// Resolve recorded event pairs into per-kernel totals.
+int prof_collect(ns3d_ctx *c)
+{
+    for (auto &q : c->pending) {
+        HIPCALL(c, hipEventSynchronize(q.b));
+        float ms = 0.0f;
+        HIPCALL(c, hipEventElapsedTime(&ms, q.a, q.b));
+        c->kind_ms[q.kind] += ms;
+        c->kind_n[q.kind] += 1;
+        c->ev_pool.push_back(q.a);
+        c->ev_pool.push_back(q.b);
+    }
+    c->pending.clear();
+    return NS3D_OK;
+}
+
+// Launch through `fn`; when profiling, bracket it with events.
+template <class Fn>
+int timed(ns3d_ctx *c, const char *name, Fn &&fn)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        a = take_event(c);
+        b = take_event(c);
+        if (!a || !b) return hip_fail(c, hipErrorOutOfMemory, "hipEventCreate");
+        HIPCALL(c, hipEventRecord(a, c->stream));
+    }
+    HIPCALL(c, fn());
+    if (c->prof) {
+        HIPCALL(c, hipEventRecord(b, c->stream));
+        c->pending.push_back({kind_id(c, name), a, b});
+        if (c->pending.size() > 4096) return prof_collect(c);
+    }
+    return NS3D_OK;
+}
+
+#define TIMED(ctx, name, expr)                                           \
+    do {                                                                 \
+        int st_ = timed(ctx, name, [&]() { return (expr); });            \
+        if (st_) return st_;                                             \
+    } while (0)
+
+// Column segments per sweep: enough independent marching threads to keep the
+// 256 CUs busy (about 4 waves per SIMD), but segments no shorter than 16.
+int pick_seg(int nd, long long ncol)
+{
+    const long long want = 256LL * 4 * 4 * 64;
+    long long nseg = (want + ncol - 1) / ncol;
+    long long maxseg = nd / 16 > 0 ? nd / 16 : 1;
+    if (nseg > maxseg) nseg = maxseg;
+    if (nseg < 1) nseg = 1;
+    return (int)((nd + nseg - 1) / nseg);
+}
+
+int check_supported(const ns3d_params &p, const ns3d_grid &g)
+{
+    for (int f = 0; f < 6; ++f)
+        if (p.bc[f] != NS3D_BC_PERIODIC) return NS3D_ERR_UNSUPPORTED;
+    if (p.recon == NS3D_RECON_WCNS) return NS3D_ERR_CONFIG;
+    if (p.precision != NS3D_PREC_FP64) return NS3D_ERR_UNSUPPORTED;
+    if (p.fvs_type != NS3D_FVS_STEGER_WARMING && p.fvs_type != NS3D_FVS_LAX_FRIEDRICHS)
+        return NS3D_ERR_CONFIG;
+    // periodic images of the ghost layers must exist inside the interior
+    if (g.nx < g.ng || g.ny < g.ng || g.nz < g.ng) return NS3D_ERR_GRID_TOO_SMALL;
+    return NS3D_OK;
+}
+
+int stream_sync(ns3d_ctx *c)
+{
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int fill_ghosts(ns3d_ctx *c)
+{
+    TIMED(c, "fill_ghosts", launch_fill_ghosts(c->stream, c->dp, c->f));
+    return NS3D_OK;
+}
+
+int ensure_faces(ns3d_ctx *c)
+{
+    if (c->face_slab) return NS3D_OK;
+    const long long sx = c->dp.sx, sy = c->dp.sy, sz = c->dp.sz;
+    const long long nf[3] = {(sx - 1) * sy * sz, sx * (sy - 1) * sz, sx * sy * (sz - 1)};
+    const long long tot = 5 * (nf[0] + nf[1] + nf[2]);
+    HIPCALL(c, hipMalloc((void **)&c->face_slab, (size_t)tot * sizeof(double)));
+    HIPCALL(c, hipMemsetAsync(c->face_slab, 0, (size_t)tot * sizeof(double), c->stream));
+    double *q = c->face_slab;
+    for (int d = 0; d < 3; ++d)
+        for (int m = 0; m < 5; ++m) { c->faces.f[5 * d + m] = q; q += nf[d]; }
+    return NS3D_OK;
+}
+
+// inviscid accumulator A = (ax + ay) + az with the fused sweeps, or through
+// the face arrays for the characteristic-wise path.
+int inviscid(ns3d_ctx *c)
+{
+    if (c->P.char_recon) {
+        int st = ensure_faces(c);
+        if (st) return st;
+        static const char *fname[3] = {"faces_char_x", "faces_char_y", "faces_char_z"};
+        for (int d = 0; d < 3; ++d) TIMED(c, fname[d], launch_faces(c->stream, c->dp, c->f, c->faces, d, 1));
+        TIMED(c, "face_diff", launch_face_diff(c->stream, c->dp, c->f, c->faces));
+        return NS3D_OK;
+    }
+    static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
+    for (int d = 0; d < 3; ++d)
+        TIMED(c, sname[d], launch_sweep(c->stream, c->dp, c->f, d, d > 0, c->seg_len[d]));
+    return NS3D_OK;
+}
+
+// one RHS evaluation finished by k_vdiv in the given mode
+int rhs_pass(ns3d_ctx *c, int mode, double dt)
+{
+    int st = inviscid(c);
+    if (st) return st;
+    TIMED(c, "grad", launch_grad(c->stream, c->dp, c->f));
+    TIMED(c, "vflux", launch_vflux(c->stream, c->dp, c->f));
+    const unsigned long long key = (unsigned long long)c->steps_done << 40;
+    TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage", launch_vdiv(c->stream, c->dp, c->f, mode, dt, c->d_flag, key));
+    return NS3D_OK;
+}
+
+// one SSP-RK3 step, range_kutta.cpp:102-147 (U0 recorded by stage 1)
+int rk3_enqueue(ns3d_ctx *c, double dt)
+{
+    for (int stage = 1; stage <= 3; ++stage) {
+        int st = rhs_pass(c, stage, dt);
+        if (st) return st;
+        st = fill_ghosts(c);
+        if (st) return st;
+    }
+    c->steps_done++;
+    return NS3D_OK;
+}
+
+int check_flag(ns3d_ctx *c)
+{
+    unsigned long long h = ~0ull;
+    HIPCALL(c, hipMemcpyAsync(&h, c->d_flag, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int st = stream_sync(c);
+    if (st) return st;
+    if (h != ~0ull) {
+        c->bad_key = h;
+        return NS3D_ERR_NONPHYSICAL;
+    }
+    return NS3D_OK;
+}
+
+int copy_in(ns3d_ctx *c, double *dst, const double *src)
+{
+    HIPCALL(c, hipMemcpyAsync(dst, src, (size_t)c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return NS3D_OK;
+}
+
+int copy_out(ns3d_ctx *c, double *dst, const double *src)
+{
+    if (!dst) return NS3D_OK;
+    HIPCALL(c, hipMemcpyAsync(dst, src, (size_t)c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return NS3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ns3d_status_string(int s)
+{
+    switch (s) {
+    case NS3D_OK: return "ok";
+    case NS3D_ERR_INVALID_ARG: return "invalid argument";
+    case NS3D_ERR_CONFIG: return "unsupported scheme configuration";
+    case NS3D_ERR_GRID_TOO_SMALL: return "grid smaller than the stencil";
+    case NS3D_ERR_NONPHYSICAL: return "non-physical value (rho, p or T <= 0 or NaN)";
+    case NS3D_ERR_HIP: return "HIP runtime error";
+    case NS3D_ERR_OOM: return "device out of memory";
+    case NS3D_ERR_UNSUPPORTED: return "option not available on the GPU path";
+    case NS3D_ERR_IO: return "cannot open file";
+    default: return "unknown status";
+    }
+}
+
+const char *ns3d_last_error(const ns3d_ctx *c) { return c ? c->err.c_str() : ""; }
+
+int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
+{
+    if (!pin || !gin || !out) return NS3D_ERR_INVALID_ARG;
+    *out = nullptr;
+    ns3d_params p = *pin;
+    ns3d_grid g = *gin;
+    int st = ns3d_params_finalize(&p, &g);
+    if (st) return st;
+    st = check_supported(p, g);
+    if (st) return st;
+    const long long sx = g.nx + 2LL * g.ng, sy = g.ny + 2LL * g.ng, sz = g.nz + 2LL * g.ng;
+    const long long ncell = sx * sy * sz;
+    // 32-bit cell indices in the kernels: face arrays and cell arrays < 2^31
+    if (ncell >= (1LL << 31)) return NS3D_ERR_INVALID_ARG;
+
+    ns3d_ctx *c = new (std::nothrow) ns3d_ctx();
+    if (!c) return NS3D_ERR_OOM;
+    c->P = p;
+    c->G = g;
+    c->ncell = ncell;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { int r = hip_fail(c, e, "stream"); delete c; return r; }
+
+    DevParams &d = c->dp;
+    d.gamma = p.gamma; d.Rgas = p.Rgas; d.Cp = p.Cp; d.Pr = p.Pr; d.mu = p.mu;
+    d.sutherland_s = p.sutherland_s; d.mu_power = p.mu_power; d.sw_eps = p.sw_eps;
+    d.mdcd_disp = p.mdcd_disp; d.mdcd_diss = p.mdcd_diss; d.ud7l_alpha = p.ud7l_alpha;
+    d.dx = g.dx; d.dy = g.dy; d.dz = g.dz;
+    d.idx = 1.0 / g.dx; d.idy = 1.0 / g.dy; d.idz = 1.0 / g.dz;   // range_kutta.cpp:14-16
+    d.fvs = p.fvs_type; d.recon = p.recon; d.vis = p.vis_scheme; d.mu_law = p.mu_law;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ng = g.ng;
+    d.sx = (int)sx; d.sy = (int)sy; d.sz = (int)sz;
+    d.periodic[0] = d.periodic[1] = d.periodic[2] = 1;
+
+    e = hipMalloc((void **)&c->slab, (size_t)ncell * kCellArrays * sizeof(double));
+    if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc fields"); ns3d_destroy(c); return r; }
+    e = hipMemsetAsync(c->slab, 0, (size_t)ncell * kCellArrays * sizeof(double), c->stream);
+    double *q = c->slab;
+    for (int m = 0; m < 6; ++m) { c->f.W[m] = q; q += ncell; }
+    for (int m = 0; m < 5; ++m) { c->f.U0[m] = q; q += ncell; }
+    for (int m = 0; m < 5; ++m) { c->f.A[m] = q; q += ncell; }
+    for (int m = 0; m < 12; ++m) { c->f.G[m] = q; q += ncell; }
+    for (int m = 0; m < 12; ++m) { c->f.VF[m] = q; q += ncell; }
+    c->nblocks = ke_blocks(d);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_flag, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_partial, (size_t)c->nblocks * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_ptrs, 10 * sizeof(double *));
+    if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc scratch"); ns3d_destroy(c); return r; }
+    const long long ncol[3] = {(long long)g.ny * g.nz, (long long)g.nx * g.nz, (long long)g.nx * g.ny};
+    const int nd[3] = {g.nx, g.ny, g.nz};
+    for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = pick_seg(nd[dd], ncol[dd]);
+    if (const char *env = std::getenv("NS3D_SWEEP_SEG")) {
+        int v = std::atoi(env);
+        if (v > 0) for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = v;
+    }
+    e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 4);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { int r = hip_fail(c, e, "init"); ns3d_destroy(c); return r; }
+    *out = c;
+    return NS3D_OK;
+}
+
+int ns3d_destroy(ns3d_ctx *c)
+{
+    if (!c) return NS3D_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->slab) (void)hipFree(c->slab);
+    if (c->face_slab) (void)hipFree(c->face_slab);
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_partial) (void)hipFree(c->d_partial);
+    if (c->d_ptrs) (void)hipFree(c->d_ptrs);
+    for (auto &q : c->pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NS3D_OK;
+}
+
+long long ns3d_cells(const ns3d_ctx *c) { return c ? c->ncell : 0; }
+
+int ns3d_sizes(const ns3d_ctx *c, int *sx, int *sy, int *sz, int *ng)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    if (sx) *sx = c->dp.sx;
+    if (sy) *sy = c->dp.sy;
+    if (sz) *sz = c->dp.sz;
+    if (ng) *ng = c->dp.ng;
+    return NS3D_OK;
+}
+
+int ns3d_upload_primitive(ns3d_ctx *c, const double *rho, const double *u, const double *v,
+                          const double *w, const double *p)
+{
+    if (!c || !rho || !u || !v || !w || !p) return NS3D_ERR_INVALID_ARG;
+    const double *src[5] = {rho, u, v, w, p};
+    for (int m = 0; m < 5; ++m) {
+        int st = copy_in(c, c->f.W[m], src[m]);
+        if (st) return st;
+    }
+    // main.cpp:70-71: apply_boundary, then primitiveToConserved (clamp, T)
+    int st = fill_ghosts(c);
+    if (st) return st;
+    HIPCALL(c, launch_prim_setup(c->stream, c->dp, c->f));
+    return stream_sync(c);
+}
+
+int ns3d_upload_conserved(ns3d_ctx *c, const double *rho, const double *ru, const double *rv,
+                          const double *rw, const double *E)
+{
+    if (!c || !rho || !ru || !rv || !rw || !E) return NS3D_ERR_INVALID_ARG;
+    const double *src[5] = {rho, ru, rv, rw, E};
+    // stage the conserved input in the gradient arrays
+    for (int m = 0; m < 5; ++m) {
+        int st = copy_in(c, c->f.G[m], src[m]);
+        if (st) return st;
+    }
+    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 5 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, launch_c2p_all(c->stream, c->dp, c->f, c->d_ptrs));
+    int st = fill_ghosts(c);
+    if (st) return st;
+    return stream_sync(c);
+}
+
+int ns3d_download_primitive(ns3d_ctx *c, double *rho, double *u, double *v, double *w, double *p, double *T)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    double *dst[6] = {rho, u, v, w, p, T};
+    for (int m = 0; m < 6; ++m) {
+        int st = copy_out(c, dst[m], c->f.W[m]);
+        if (st) return st;
+    }
+    return stream_sync(c);
+}
+
+int ns3d_download_conserved(ns3d_ctx *c, double *rho, double *ru, double *rv, double *rw, double *E)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    // P2C into the gradient arrays (scratch), then copy out
+    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 5 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, launch_p2c_all(c->stream, c->dp, c->f, c->d_ptrs));
+    double *dst[5] = {rho, ru, rv, rw, E};
+    for (int m = 0; m < 5; ++m) {
+        int st = copy_out(c, dst[m], c->f.G[m]);
+        if (st) return st;
+    }
+    return stream_sync(c);
+}
+
+int ns3d_download_rhs(ns3d_ctx *c, double *r0, double *r1, double *r2, double *r3, double *r4)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    double *dst[5] = {r0, r1, r2, r3, r4};
+    for (int m = 0; m < 5; ++m) {
+        int st = copy_out(c, dst[m], c->f.A[m]);
+        if (st) return st;
+    }
+    return stream_sync(c);
+}
+
+int ns3d_apply_boundary(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    int st = fill_ghosts(c);
+    if (st) return st;
+    return stream_sync(c);
+}
+
+int ns3d_du_inviscous(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    int st = ensure_faces(c);
+    if (st) return st;
+    for (int d = 0; d < 3; ++d)
+        HIPCALL(c, launch_faces(c->stream, c->dp, c->f, c->faces, d, c->P.char_recon));
+    return stream_sync(c);
+}
+
+int ns3d_download_face_flux(ns3d_ctx *c, int dir, double *f0, double *f1, double *f2, double *f3, double *f4)
+{
+    if (!c || dir < 0 || dir > 2 || !c->face_slab) return NS3D_ERR_INVALID_ARG;
+    const long long sx = c->dp.sx, sy = c->dp.sy, sz = c->dp.sz;
+    const long long nf = dir == 0 ? (sx - 1) * sy * sz : dir == 1 ? sx * (sy - 1) * sz : sx * sy * (sz - 1);
+    double *dst[5] = {f0, f1, f2, f3, f4};
+    for (int m = 0; m < 5; ++m) {
+        if (!dst[m]) continue;
+        HIPCALL(c, hipMemcpyAsync(dst[m], c->faces.f[5 * dir + m], (size_t)nf * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+    }
+    return stream_sync(c);
+}
+
+int ns3d_compute_gradients(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    HIPCALL(c, launch_grad(c->stream, c->dp, c->f));
+    return stream_sync(c);
+}
+
+int ns3d_download_gradients(ns3d_ctx *c, double *const grads[12])
+{
+    if (!c || !grads) return NS3D_ERR_INVALID_ARG;
+    for (int m = 0; m < 12; ++m) {
+        int st = copy_out(c, grads[m], c->f.G[m]);
+        if (st) return st;
+    }
+    return stream_sync(c);
+}
+
+int ns3d_compute_viscous_flux(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    HIPCALL(c, launch_vflux(c->stream, c->dp, c->f));
+    return stream_sync(c);
+}
+
+int ns3d_download_viscous_flux(ns3d_ctx *c, double *const flux[12])
+{
+    if (!c || !flux) return NS3D_ERR_INVALID_ARG;
+    for (int m = 0; m < 12; ++m) {
+        int st = copy_out(c, flux[m], c->f.VF[m]);
+        if (st) return st;
+    }
+    return stream_sync(c);
+}
+
+int ns3d_compute_rhs(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    int st = rhs_pass(c, 0, 0.0);
+    if (st) return st;
+    return stream_sync(c);
+}
+
+int ns3d_rk3_step(ns3d_ctx *c, double dt)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    int st = rk3_enqueue(c, dt);
+    if (st) return st;
+    return check_flag(c);
+}
+
+int ns3d_advance(ns3d_ctx *c, int nsteps, double dt)
+{
+    if (!c || nsteps < 0) return NS3D_ERR_INVALID_ARG;
+    for (int s = 0; s < nsteps; ++s) {
+        int st = rk3_enqueue(c, dt);
+        if (st) return st;
+    }
+    return check_flag(c);
+}
+
+int ns3d_nonphysical_cell(const ns3d_ctx *c, int *i, int *j, int *k)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->bad_key == ~0ull) {
+        if (i) *i = -1;
+        if (j) *j = -1;
+        if (k) *k = -1;
+        return NS3D_OK;
+    }
+    long long id = (long long)(c->bad_key & ((1ull << 40) - 1));
+    const long long sx = c->dp.sx, sy = c->dp.sy;
+    if (i) *i = (int)(id % sx);
+    if (j) *j = (int)((id / sx) % sy);
+    if (k) *k = (int)(id / (sx * sy));
+    return NS3D_OK;
+}
+
+int ns3d_compute_timestep_local(ns3d_ctx *c, double *dt)
+{
+    if (!c || !dt) return NS3D_ERR_INVALID_ARG;
+    double init = 1e9;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, sizeof(bits));
+    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 1, bits, 1));
+    TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
+    unsigned long long h = 0;
+    HIPCALL(c, hipMemcpyAsync(&h, c->d_flag + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int st = stream_sync(c);
+    if (st) return st;
+    double m;
+    std::memcpy(&m, &h, sizeof(m));
+    *dt = c->P.cfl * m;   // timestep.cpp:32
+    return NS3D_OK;
+}
+
+int ns3d_total_energy(ns3d_ctx *c, double *ek)
+{
+    if (!c || !ek) return NS3D_ERR_INVALID_ARG;
+    HIPCALL(c, launch_ke(c->stream, c->dp, c->f, c->d_partial));
+    std::vector<double> h((size_t)c->nblocks);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->d_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    int st = stream_sync(c);
+    if (st) return st;
+    // compensated sum of the block partials, in block order
+    double s = 0.0, comp = 0.0;
+    for (double x : h) {
+        double y = x - comp;
+        double t = s + y;
+        comp = (t - s) - y;
+        s = t;
+    }
+    *ek = s;
+    return NS3D_OK;
+}
+
+int ns3d_sync(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    return stream_sync(c);
+}
+
+void *ns3d_stream(ns3d_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int ns3d_profile_enable(ns3d_ctx *c, int on)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    int st = prof_collect(c);
+    if (st) return st;
+    c->kind_name.clear();
+    c->kind_ms.clear();
+    c->kind_n.clear();
+    c->prof = on != 0;
+    return NS3D_OK;
+}
+
+int ns3d_profile_count(ns3d_ctx *c)
+{
+    if (!c) return -NS3D_ERR_INVALID_ARG;
+    int st = prof_collect(c);
+    if (st) return -st;
+    return (int)c->kind_name.size();
+}
+
+int ns3d_profile_entry(ns3d_ctx *c, int idx, char *name, int name_len, double *total_ms, long long *launches)
+{
+    if (!c || idx < 0 || idx >= (int)c->kind_name.size()) return NS3D_ERR_INVALID_ARG;
+    int st = prof_collect(c);
+    if (st) return st;
+    if (name && name_len > 0) {
+        std::strncpy(name, c->kind_name[idx].c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (total_ms) *total_ms = c->kind_ms[idx];
+    if (launches) *launches = c->kind_n[idx];
+    return NS3D_OK;
+}
+
+}  // extern "C"

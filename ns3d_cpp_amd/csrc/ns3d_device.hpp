@@ -1,0 +1,295 @@
+// ns3d_device.hpp -- device functions of the FP64 RHS path (gfx950).
+//
+// Every expression keeps the reference's evaluation order; the library is
+// built with -ffp-contract=off so the FP64 path forms no FMA, matching the
+// x86-64 baseline build of the reference (reference Makefile:6).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ns3d {
+
+// Scalars every kernel needs, passed by value.
+struct DevParams {
+    double gamma, Rgas, Cp, Pr, mu, sutherland_s, mu_power, sw_eps;
+    double mdcd_disp, mdcd_diss, ud7l_alpha;
+    double dx, dy, dz, idx, idy, idz;
+    int fvs, recon, vis, mu_law;
+    int nx, ny, nz, ng, sx, sy, sz;
+    int periodic[3];
+};
+
+enum { FVS_SW = 0, FVS_LF = 1 };
+enum { R_WENO5 = 0, R_LINEAR = 2, R_MDCD = 3, R_MDCD_HYBRID = 4, R_UD7 = 5, R_UD7L = 6 };
+enum { VIS_C8 = 0, VIS_C6 = 1, VIS_C4 = 2 };
+
+// stencil width per reconstruction (initialization.cpp:329-357)
+template <int RECON> struct Stencil;
+template <> struct Stencil<R_WENO5> { static constexpr int S = 6; };
+template <> struct Stencil<R_LINEAR> { static constexpr int S = 2; };
+template <> struct Stencil<R_MDCD> { static constexpr int S = 6; };
+template <> struct Stencil<R_MDCD_HYBRID> { static constexpr int S = 6; };
+template <> struct Stencil<R_UD7> { static constexpr int S = 8; };
+template <> struct Stencil<R_UD7L> { static constexpr int S = 8; };
+
+// ----------------------------------------------------------- conversions
+
+// E and c of primitiveToConserved (field_structures.h:445-449) from a
+// primitive state whose density is already clamped.
+__device__ __forceinline__ void p2c_energy_sound(double gamma, double rr, double u, double v,
+                                                 double w, double p, double &E, double &c)
+{
+    double kinetic = 0.5 * (u * u + v * v + w * w);
+    double e_internal = p / ((gamma - 1.0) * rr);
+    E = rr * (e_internal + kinetic);
+    c = sqrt(gamma * p / rr);
+}
+
+// SolverParams::get_mu (field_structures.h:94-105) + selectable laws.
+__device__ __forceinline__ double get_mu(const DevParams &P, double T)
+{
+    if (P.mu_law == 1) return P.mu;
+    if (P.mu_law == 2) return P.mu * pow(T, P.mu_power);
+    double S_ref = P.sutherland_s;
+    return P.mu * pow(T, 1.5) * (1.0 + S_ref) / (T + S_ref);
+}
+
+// --------------------------------------------------------------- splits
+
+__device__ __forceinline__ void lam_split(double lam, double eps, double &lp, double &lm)
+{
+    if (eps == 0.0) {
+        lp = 0.5 * (lam + fabs(lam));
+        lm = 0.5 * (lam - fabs(lam));
+    } else {  // README.md:179
+        double r = sqrt(lam * lam + eps * eps);
+        lp = 0.5 * (lam + r);
+        lm = 0.5 * (lam - r);
+    }
+}
+
+// split_StagerWarming / split_LaxFriedrichs (InviscousFlux_new.cpp:439-530)
+// for the axis (N1,N2,N3); E and c come from p2c_energy_sound.
+template <int FVS, int DIR>
+__device__ __forceinline__ void split_state(const DevParams &P, double rho, double u, double v,
+                                            double w, double p, double Fp[5], double Fm[5])
+{
+    constexpr double n1 = DIR == 0 ? 1.0 : 0.0;
+    constexpr double n2 = DIR == 1 ? 1.0 : 0.0;
+    constexpr double n3 = DIR == 2 ? 1.0 : 0.0;
+    const double gamma = P.gamma;
+    double E, c;
+    p2c_energy_sound(gamma, rho, u, v, w, p, E, c);
+    if constexpr (FVS == FVS_SW) {
+        double un = u * n1 + v * n2 + w * n3;
+        double lambda1 = un - c;
+        double lambda2 = un;
+        double lambda3 = un + c;
+        double l1p, l1m, l2p, l2m, l3p, l3m;
+        lam_split(lambda1, P.sw_eps, l1p, l1m);
+        lam_split(lambda2, P.sw_eps, l2p, l2m);
+        lam_split(lambda3, P.sw_eps, l3p, l3m);
+        double S1_p = l1p + l3p;
+        double S1_m = l1m + l3m;
+        double S2_p = l3p - l1p;
+        double S2_m = l3m - l1m;
+        double H = (E + p) / rho;
+        double ga1 = rho / (2.0 * gamma);
+        double ga2 = 2.0 * (gamma - 1.0);
+        double V = 0.5 * (u * u + v * v + w * w);
+        Fp[0] = ga1 * (S1_p + ga2 * l2p);
+        Fp[1] = ga1 * (S1_p * u + ga2 * l2p * u + S2_p * n1 * c);
+        Fp[2] = ga1 * (S1_p * v + ga2 * l2p * v + S2_p * n2 * c);
+        Fp[3] = ga1 * (S1_p * w + ga2 * l2p * w + S2_p * n3 * c);
+        Fp[4] = ga1 * (S1_p * H + ga2 * l2p * V + S2_p * un * c);
+        Fm[0] = ga1 * (S1_m + ga2 * l2m);
+        Fm[1] = ga1 * (S1_m * u + ga2 * l2m * u + S2_m * n1 * c);
+        Fm[2] = ga1 * (S1_m * v + ga2 * l2m * v + S2_m * n2 * c);
+        Fm[3] = ga1 * (S1_m * w + ga2 * l2m * w + S2_m * n3 * c);
+        Fm[4] = ga1 * (S1_m * H + ga2 * l2m * V + S2_m * un * c);
+    } else {
+        double un = u * n1 + v * n2 + w * n3;
+        double a_max = fabs(un) + c;
+        double F1 = rho * un;
+        double F2 = rho * u * un + p * n1;
+        double F3 = rho * v * un + p * n2;
+        double F4 = rho * w * un + p * n3;
+        double F5 = (E + p) * un;
+        Fp[0] = 0.5 * (F1 + a_max * rho);
+        Fp[1] = 0.5 * (F2 + a_max * rho * u);
+        Fp[2] = 0.5 * (F3 + a_max * rho * v);
+        Fp[3] = 0.5 * (F4 + a_max * rho * w);
+        Fp[4] = 0.5 * (F5 + a_max * E);
+        Fm[0] = 0.5 * (F1 - a_max * rho);
+        Fm[1] = 0.5 * (F2 - a_max * rho * u);
+        Fm[2] = 0.5 * (F3 - a_max * rho * v);
+        Fm[3] = 0.5 * (F4 - a_max * rho * w);
+        Fm[4] = 0.5 * (F5 - a_max * E);
+    }
+}
+
+// ------------------------------------------------------- reconstructions
+// s[] is the stencil already in the orientation reconstruct_select passes
+// (reversed for the minus side, reconstruction.cpp:29-72).
+
+__device__ __forceinline__ double recon_weno5(const double *s)
+{   // reconstruction.cpp:93-124
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    double eps = 1e-6;
+    double beta0 = 13.0 / 12.0 * (f0 - 2 * f1 + f2) * (f0 - 2 * f1 + f2)
+                 + 0.25 * (f0 - 4 * f1 + 3 * f2) * (f0 - 4 * f1 + 3 * f2);
+    double beta1 = 13.0 / 12.0 * (f1 - 2 * f2 + f3) * (f1 - 2 * f2 + f3)
+                 + 0.25 * (f1 - f3) * (f1 - f3);
+    double beta2 = 13.0 / 12.0 * (f2 - 2 * f3 + f4) * (f2 - 2 * f3 + f4)
+                 + 0.25 * (3 * f2 - 4 * f3 + f4) * (3 * f2 - 4 * f3 + f4);
+    double alpha0 = 0.1 / ((eps + beta0) * (eps + beta0));
+    double alpha1 = 0.6 / ((eps + beta1) * (eps + beta1));
+    double alpha2 = 0.3 / ((eps + beta2) * (eps + beta2));
+    double asum = alpha0 + alpha1 + alpha2;
+    double w0 = alpha0 / asum;
+    double w1 = alpha1 / asum;
+    double w2 = alpha2 / asum;
+    double p0 = (2 * f0 - 7 * f1 + 11 * f2) / 6.0;
+    double p1 = (-f1 + 5 * f2 + 2 * f3) / 6.0;
+    double p2 = (2 * f2 + 5 * f3 - f4) / 6.0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+__device__ __forceinline__ double recon_mdcd(const double *s, double disp, double diss)
+{   // reconstruction.cpp:126-135
+    return (1.0 / 2.0 * disp + 1.0 / 2.0 * diss) * s[0] +
+           (-3.0 / 2.0 * disp - 5.0 / 2.0 * diss - 1.0 / 12.0) * s[1] +
+           (7.0 / 12.0 + disp + 5.0 * diss) * s[2] +
+           (7.0 / 12.0 + disp - 5.0 * diss) * s[3] +
+           (-3.0 / 2.0 * disp + 5.0 / 2.0 * diss - 1.0 / 12.0) * s[4] +
+           (1.0 / 2.0 * disp - 1.0 / 2.0 * diss) * s[5];
+}
+
+__device__ __forceinline__ double recon_mdcd_hybrid(const double *s, double disp, double diss)
+{   // reconstruction.cpp:151-225 (std::pow(x,2) is x*x after g++ folding)
+    double eps_small = 1e-40;
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4], f5 = s[5];
+    double eps = 0.9 * 0.4 / (1.0 - 0.9 * 0.4) * 1e-4;
+    double a1 = fabs(f2 - f1) + fabs(f2 - 2 * f1 + f0);
+    double b1 = fabs(f2 - f3) + fabs(f2 - 2 * f3 + f4);
+    double a2 = fabs(f3 - f2) + fabs(f3 - 2 * f2 + f1);
+    double b2 = fabs(f3 - f4) + fabs(f3 - 2 * f4 + f5);
+    double sai1 = (2 * a1 * b1 + eps) / (a1 * a1 + b1 * b1 + eps);
+    double sai2 = (2 * a2 * b2 + eps) / (a2 * a2 + b2 * b2 + eps);
+    double sai = (sai2 < sai1) ? sai2 : sai1;
+    if (sai > 0.4) return recon_mdcd(s, disp, diss);
+    double q0 = (2 * f0 - 7 * f1 + 11 * f2) / 6.0;
+    double q1 = (-f1 + 5 * f2 + 2 * f3) / 6.0;
+    double q2 = (2 * f2 + 5 * f3 - f4) / 6.0;
+    double q3 = (11 * f3 - 7 * f4 + 2 * f5) / 6.0;
+    double d0 = 1.5 * (disp + diss);
+    double d1 = 0.5 - 1.5 * (disp - 3 * diss);
+    double d2 = 0.5 - 1.5 * (disp + 3 * diss);
+    double d3 = 1.5 * (disp - diss);
+    double t0 = f0 - 2 * f1 + f2, t0b = f0 - 4 * f1 + 3 * f2;
+    double t1 = f1 - 2 * f2 + f3, t1b = f1 - f3;
+    double t2 = f2 - 2 * f3 + f4, t2b = 3 * f2 - 4 * f3 + f4;
+    double beta0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (t0b * t0b);
+    double beta1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (t1b * t1b);
+    double beta2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (t2b * t2b);
+    double beta3 =
+        (271799 * (f0 * f0)
+         + f0 * (-2380800 * f1 + 4086352 * f2 - 3462252 * f3 + 1458762 * f4 - 245620 * f5)
+         + f1 * (5653317 * f1 - 20427884 * f2 + 17905032 * f3 - 7727988 * f4 + 1325006 * f5)
+         + f2 * (19510972 * f2 - 35817664 * f3 + 15929912 * f4 - 2792660 * f5)
+         + f3 * (17195652 * f3 - 15880404 * f4 + 2863984 * f5)
+         + f4 * (3824847 * f4 - 1429976 * f5)
+         + 139633 * (f5 * f5)) / 120960.0;
+    double tau6 = fabs(beta3 - (beta0 + 4 * beta1 + beta2) / 6.0);
+    double g0 = 20 + tau6 / (beta0 + eps_small);
+    double g1 = 20 + tau6 / (beta1 + eps_small);
+    double g2 = 20 + tau6 / (beta2 + eps_small);
+    double g3 = 20 + tau6 / (beta3 + eps_small);
+    double aa0 = d0 * (g0 * g0);
+    double aa1 = d1 * (g1 * g1);
+    double aa2 = d2 * (g2 * g2);
+    double aa3 = d3 * (g3 * g3);
+    return (aa0 * q0 + aa1 * q1 + aa2 * q2 + aa3 * q3) / (aa0 + aa1 + aa2 + aa3);
+}
+
+__device__ __forceinline__ double recon_ud7(const double *s)
+{   // reconstruction.cpp:227-232
+    return (-1.0 / 140.0) * s[0] + (5.0 / 84.0) * s[1] - (101.0 / 420.0) * s[2]
+           + (319.0 / 420.0) * s[3] + (107.0 / 210.0) * s[4]
+           - (19.0 / 210.0) * s[5] + (1.0 / 105.0) * s[6];
+}
+
+__device__ __forceinline__ double recon_ud7l(const double *s, double alpha)
+{   // reconstruction.cpp:234-249
+    double a0 = -3.0 / 420.0 * alpha - 3.0 / 840.0 * (1.0 - alpha);
+    double a1 = 25.0 / 420.0 * alpha + 29.0 / 840.0 * (1.0 - alpha);
+    double a2 = -101.0 / 420.0 * alpha - 139.0 / 840.0 * (1.0 - alpha);
+    double a3 = 319.0 / 420.0 * alpha + 533.0 / 840.0 * (1.0 - alpha);
+    double a4 = 214.0 / 420.0 * alpha + 533.0 / 840.0 * (1.0 - alpha);
+    double a5 = -38.0 / 420.0 * alpha - 139.0 / 840.0 * (1.0 - alpha);
+    double a6 = 4.0 / 420.0 * alpha + 29.0 / 840.0 * (1.0 - alpha);
+    double a7 = (-3.0 / 840.0) * (1.0 - alpha);
+    return a0 * s[0] + a1 * s[1] + a2 * s[2] + a3 * s[3]
+           + a4 * s[4] + a5 * s[5] + a6 * s[6] + a7 * s[7];
+}
+
+template <int RECON>
+__device__ __forceinline__ double recon(const DevParams &P, const double *s)
+{
+    if constexpr (RECON == R_WENO5) return recon_weno5(s);
+    else if constexpr (RECON == R_LINEAR) return s[0];
+    else if constexpr (RECON == R_MDCD) return recon_mdcd(s, P.mdcd_disp, P.mdcd_diss);
+    else if constexpr (RECON == R_MDCD_HYBRID) return recon_mdcd_hybrid(s, P.mdcd_disp, P.mdcd_diss);
+    else if constexpr (RECON == R_UD7) return recon_ud7(s);
+    else return recon_ud7l(s, P.ud7l_alpha);
+}
+
+// ------------------------------------------------- central differences
+// ViscousFlux.cpp:19-40 on values f(-r..r) read at stride `st` from `c`.
+
+template <int VIS>
+__device__ __forceinline__ double diff_central(const double *__restrict__ c, long st, double h)
+{
+    if constexpr (VIS == VIS_C8) {
+        double c1 = 0.8;
+        double c2 = -0.2;
+        double c3 = 0.0380952380952381;
+        double c4 = -0.00357142857142857;
+        return (c1 * (c[st] - c[-st]) +
+                c2 * (c[2 * st] - c[-2 * st]) +
+                c3 * (c[3 * st] - c[-3 * st]) +
+                c4 * (c[4 * st] - c[-4 * st])) / h;
+    } else if constexpr (VIS == VIS_C6) {
+        return (c[3 * st] - 9.0 * c[2 * st] + 45.0 * c[st] - 45.0 * c[-st] + 9.0 * c[-2 * st] - c[-3 * st]) /
+               (60.0 * h);
+    } else {
+        return (-c[2 * st] + 8.0 * c[st] - 8.0 * c[-st] + c[-2 * st]) / (12.0 * h);
+    }
+}
+
+template <int VIS> struct VisReach;
+template <> struct VisReach<VIS_C8> { static constexpr int R = 4; };
+template <> struct VisReach<VIS_C6> { static constexpr int R = 3; };
+template <> struct VisReach<VIS_C4> { static constexpr int R = 2; };
+
+// Same operator on values gathered into v[-R..R] (v points at the centre).
+template <int VIS>
+__device__ __forceinline__ double diff_window(const double *v, double h)
+{
+    if constexpr (VIS == VIS_C8) {
+        double c1 = 0.8;
+        double c2 = -0.2;
+        double c3 = 0.0380952380952381;
+        double c4 = -0.00357142857142857;
+        return (c1 * (v[1] - v[-1]) +
+                c2 * (v[2] - v[-2]) +
+                c3 * (v[3] - v[-3]) +
+                c4 * (v[4] - v[-4])) / h;
+    } else if constexpr (VIS == VIS_C6) {
+        return (v[3] - 9.0 * v[2] + 45.0 * v[1] - 45.0 * v[-1] + 9.0 * v[-2] - v[-3]) / (60.0 * h);
+    } else {
+        return (-v[2] + 8.0 * v[1] - 8.0 * v[-1] + v[-2]) / (12.0 * h);
+    }
+}
+
+}  // namespace ns3d

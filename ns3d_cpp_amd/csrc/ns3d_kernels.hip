@@ -1,0 +1,723 @@
+// ns3d_kernels.hip -- HIP kernels of the FP64 RHS + RK3 path for gfx950 and
+// the launch helpers the C ABI (ns3d_api.cpp) calls.
+//
+// Device state per context (SoA, idx3 layout with ng ghosts, FP64):
+//   W[6]  rho u v w p T    primitive state; conserved U = P2C(W) on the fly
+//   U0[5]                 conserved state at the start of the RK3 step
+//   A[5]                  inviscid accumulator S = (ax + ay) + az, then RHS
+//   G[12], VF[12]         velocity/temperature gradients, viscous fluxes
+#include "ns3d_device.hpp"
+#include "ns3d_internal.hpp"
+
+namespace ns3d {
+
+__device__ __forceinline__ int wrapi(int q, int ng, int n)
+{
+    return q < ng ? q + n : (q >= ng + n ? q - n : q);
+}
+
+// ------------------------------------------------------------- utilities
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// primitiveToConserved's clamp and T over every cell (field_structures.h:436-448)
+__global__ void k_prim_setup(DevParams P, Fields F, int ncell)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    double rr = F.W[0][id];
+    if (rr <= 0.0) { rr = 1e-12; F.W[0][id] = rr; }
+    F.W[5][id] = F.W[4][id] / (rr * P.Rgas);
+}
+
+// conservedToPrimitive over every cell (field_structures.h:405-427); the
+// conserved input sits in src[0..4].
+__global__ void k_c2p_all(DevParams P, Fields F, const double *const *src, int ncell)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    const double gamma = P.gamma;
+    double rr = src[0][id];
+    if (rr <= 0.0) rr = 1e-12;
+    double u = src[1][id] / rr;
+    double v = src[2][id] / rr;
+    double w = src[3][id] / rr;
+    double kinetic = 0.5 * (u * u + v * v + w * w);
+    double e_internal = src[4][id] / rr - kinetic;
+    double p = (gamma - 1.0) * rr * e_internal;
+    F.W[0][id] = rr; F.W[1][id] = u; F.W[2][id] = v; F.W[3][id] = w; F.W[4][id] = p;
+    F.W[5][id] = p / (rr * P.Rgas);
+}
+
+// primitiveToConserved (field_structures.h:430-451) into dst[0..4]
+__global__ void k_p2c_all(DevParams P, Fields F, double *const *dst, int ncell)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    const double gamma = P.gamma;
+    double rr = F.W[0][id];
+    if (rr <= 0.0) rr = 1e-12;
+    double u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
+    dst[0][id] = rr;
+    dst[1][id] = rr * u;
+    dst[2][id] = rr * v;
+    dst[3][id] = rr * w;
+    double kinetic = 0.5 * (u * u + v * v + w * w);
+    double e_internal = p / ((gamma - 1.0) * rr);
+    dst[4][id] = rr * (e_internal + kinetic);
+}
+
+// Periodic ghost fill of the six W arrays: the face regions of
+// exchange_halos_physical on one rank (field_structures.h:795-853), cross
+// directions restricted to interior cells.  blockIdx.y selects the axis.
+__global__ void k_fill_periodic(DevParams P, Fields F)
+{
+    const int axis = blockIdx.y;
+    if (!P.periodic[axis]) return;
+    const int ng = P.ng;
+    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const int na = axis == 0 ? P.ny : P.nx;   // fastest cross axis
+    const int nb = axis == 2 ? P.ny : P.nz;
+    const int total = 2 * ng * na * nb;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int g = t % (2 * ng);            // 0..ng-1 low side, ng..2ng-1 high side
+    int rest = t / (2 * ng);
+    int a = ng + rest % na;
+    int b = ng + rest / na;
+    int q, qs;
+    if (g < ng) { q = g; qs = g + nd; }
+    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
+    int dst, src;
+    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
+    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
+    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) F.W[m][dst] = F.W[m][src];
+}
+
+// --------------------------------------------------------- fused sweeps
+//
+// One thread marches one column of cells along the sweep axis DIR over a
+// segment [c0, c1), keeping the split fluxes of the S cells of the current
+// face stencil in registers: each cell is split once, each face flux is
+// reconstructed once (plus one face of warm-up per segment).  Lanes run
+// along the fastest cross axis, so for DIR = 1, 2 every load is coalesced.
+// The cell result (F(f) - F(f-1)) * inv_d is what range_kutta.cpp:55-97
+// forms per direction; ACC = 0 stores it, ACC = 1 adds it to the
+// accumulator, so after x, y, z the accumulator holds (ax + ay) + az.
+template <int DIR, int RECON, int FVS, int ACC>
+__global__ __launch_bounds__(64) void k_sweep(DevParams P, Fields F, int seg_len)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    const int na = DIR == 0 ? P.ny : P.nx;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const int b = blockIdx.y;
+    const int nd = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const int c0 = P.ng + blockIdx.z * seg_len;
+    const int c1 = min(c0 + seg_len, P.ng + nd);
+    if (c0 >= c1) return;
+    int base, st;
+    if (DIR == 0) { base = ((P.ng + b) * P.sy + (P.ng + a)) * P.sx; st = 1; }
+    else if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + a); st = P.sx; }
+    else { base = (P.ng + b) * P.sx + (P.ng + a); st = P.sx * P.sy; }
+    const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
+    const double *__restrict__ rho = F.W[0];
+    const double *__restrict__ uu = F.W[1];
+    const double *__restrict__ vv = F.W[2];
+    const double *__restrict__ ww = F.W[3];
+    const double *__restrict__ pp = F.W[4];
+
+    double fp[S][5], fm[S][5];
+#pragma unroll
+    for (int s = 1; s < S; ++s) {
+        int id = base + (c0 - 2 - MID + s) * st;
+        split_state<FVS, DIR>(P, rho[id], uu[id], vv[id], ww[id], pp[id], fp[s], fm[s]);
+    }
+    double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int f = c0 - 1; f < c1; ++f) {
+#pragma unroll
+        for (int s = 0; s < S - 1; ++s)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) { fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m]; }
+        {
+            int id = base + (f - MID + S - 1) * st;
+            split_state<FVS, DIR>(P, rho[id], uu[id], vv[id], ww[id], pp[id], fp[S - 1], fm[S - 1]);
+        }
+        double ff[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[s] = fm[S - 1 - s][m]; }
+            double plus_face = recon<RECON>(P, sp);
+            double minus_face = recon<RECON>(P, sm);
+            ff[m] = plus_face + minus_face;
+        }
+        if (f >= c0) {
+            int idc = base + f * st;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                double ad = (ff[m] - fprev[m]) * inv_d;
+                if (ACC == 0) F.A[m][idc] = ad;
+                else F.A[m][idc] = F.A[m][idc] + ad;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fprev[m] = ff[m];
+    }
+}
+
+// ------------------------------------------------ face-flux path (unfused)
+
+// build_eigen_matrices_new (InviscousFlux_new.cpp:532-625) and
+// computeRoeAveragedState_new (:628-659)
+__device__ void char_matrices(double gamma, const double ul[5], const double ur[5],
+                              double nx, double ny, double nz, double Lm[5][5], double Rm[5][5])
+{
+    double rho_L = ul[0], u_L = ul[1], v_L = ul[2], w_L = ul[3], p_L = ul[4];
+    double E_L = rho_L * (0.5 * (u_L * u_L + v_L * v_L + w_L * w_L) + p_L / ((gamma - 1.0) * rho_L));
+    double rho_R = ur[0], u_R = ur[1], v_R = ur[2], w_R = ur[3], p_R = ur[4];
+    double E_R = rho_R * (0.5 * (u_R * u_R + v_R * v_R + w_R * w_R) + p_R / ((gamma - 1.0) * rho_R));
+    double sqrt_rho_L = sqrt(rho_L);
+    double sqrt_rho_R = sqrt(rho_R);
+    double denom = 1.0 / (sqrt_rho_L + sqrt_rho_R);
+    double u0 = (sqrt_rho_L * u_L + sqrt_rho_R * u_R) * denom;
+    double v0 = (sqrt_rho_L * v_L + sqrt_rho_R * v_R) * denom;
+    double w0 = (sqrt_rho_L * w_L + sqrt_rho_R * w_R) * denom;
+    double H_L = (E_L + p_L) / rho_L;
+    double H_R = (E_R + p_R) / rho_R;
+    double h0 = (sqrt_rho_L * H_L + sqrt_rho_R * H_R) * denom;
+    double V = 0.5 * (u0 * u0 + v0 * v0 + w0 * w0);
+    double un = u0 * nx + v0 * ny + w0 * nz;
+    double c = sqrt((gamma - 1.0) * (h0 - V));
+    Rm[0][0] = 1.0; Rm[1][0] = u0 - c * nx; Rm[2][0] = v0 - c * ny; Rm[3][0] = w0 - c * nz; Rm[4][0] = h0 - un * c;
+    Rm[0][1] = 1.0; Rm[1][1] = u0; Rm[2][1] = v0; Rm[3][1] = w0; Rm[4][1] = V;
+    Rm[0][2] = 0.0; Rm[1][2] = nz; Rm[2][2] = nx; Rm[3][2] = ny; Rm[4][2] = u0 * nz + v0 * nx + w0 * ny;
+    Rm[0][3] = 0.0; Rm[1][3] = ny; Rm[2][3] = nz; Rm[3][3] = nx; Rm[4][3] = u0 * ny + v0 * nz + w0 * nx;
+    Rm[0][4] = 1.0; Rm[1][4] = u0 + c * nx; Rm[2][4] = v0 + c * ny; Rm[3][4] = w0 + c * nz; Rm[4][4] = h0 + un * c;
+    double chi = 0.5 / c;
+    double K = (gamma - 1.0) / (c * c);
+    double K1 = K / 2;
+    Lm[0][0] = K1 * V + chi * un; Lm[0][1] = -chi * nx - K1 * u0; Lm[0][2] = -chi * ny - K1 * v0;
+    Lm[0][3] = -chi * nz - K1 * w0; Lm[0][4] = K1;
+    Lm[1][0] = 1.0 - K * V; Lm[1][1] = K * u0; Lm[1][2] = K * v0; Lm[1][3] = K * w0; Lm[1][4] = -K;
+    Lm[2][0] = -(u0 * nz + v0 * nx + w0 * ny); Lm[2][1] = nz; Lm[2][2] = nx; Lm[2][3] = ny; Lm[2][4] = 0.0;
+    Lm[3][0] = -(u0 * ny + v0 * nz + w0 * nx); Lm[3][1] = ny; Lm[3][2] = nz; Lm[3][3] = nx; Lm[3][4] = 0.0;
+    Lm[4][0] = K1 * V - chi * un; Lm[4][1] = chi * nx - K1 * u0; Lm[4][2] = chi * ny - K1 * v0;
+    Lm[4][3] = chi * nz - K1 * w0; Lm[4][4] = K1;
+}
+
+// One thread per face: du_inviscous (InviscousFlux_new.cpp:4-163) or, with
+// CHAR, du_inviscous_character (:165-437).  Output in the reference's face
+// layouts idx_fx / idx_fy / idx_fz (field_structures.h:72-81).
+template <int DIR, int RECON, int FVS, int CHAR>
+__global__ __launch_bounds__(256) void k_faces(DevParams P, Fields F, FaceArrays out)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    const int ng = P.ng;
+    // fastest coordinate x-like, then y, then z; the face axis has n+1 faces
+    const int n0 = DIR == 0 ? P.nx + 1 : P.nx;
+    const int n1 = DIR == 1 ? P.ny + 1 : P.ny;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+    const int t2 = blockIdx.z;
+    if (t0 >= n0 || t1 >= n1) return;
+    int i = (DIR == 0 ? ng - 1 : ng) + t0;
+    int j = (DIR == 1 ? ng - 1 : ng) + t1;
+    int k = (DIR == 2 ? ng - 1 : ng) + t2;
+    const int st = DIR == 0 ? 1 : DIR == 1 ? P.sx : P.sx * P.sy;
+    const int idf = (k * P.sy + j) * P.sx + i;   // cell left of the face
+    double fp[S][5], fm[S][5];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        int id = idf + (s - MID) * st;
+        split_state<FVS, DIR>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], fp[s], fm[s]);
+    }
+    double wface[5];
+    if constexpr (CHAR) {
+        constexpr double nxv = DIR == 0 ? 1.0 : 0.0, nyv = DIR == 1 ? 1.0 : 0.0, nzv = DIR == 2 ? 1.0 : 0.0;
+        double ul[5], ur[5], Lm[5][5], Rm[5][5];
+        const int ir = idf + st;
+        ul[0] = F.W[0][idf]; ul[1] = F.W[1][idf]; ul[2] = F.W[2][idf]; ul[3] = F.W[3][idf]; ul[4] = F.W[4][idf];
+        ur[0] = F.W[0][ir]; ur[1] = F.W[1][ir]; ur[2] = F.W[2][ir]; ur[3] = F.W[3][ir]; ur[4] = F.W[4][ir];
+        char_matrices(P.gamma, ul, ur, nxv, nyv, nzv, Lm, Rm);
+        double wchar[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                double ap = 0.0, am = 0.0;
+#pragma unroll
+                for (int r = 0; r < 5; ++r) { ap += Lm[n][r] * fp[s][r]; am += Lm[n][r] * fm[s][r]; }
+                sp[s] = ap;
+                sm[S - 1 - s] = am;
+            }
+            double plus_face = recon<RECON>(P, sp);
+            double minus_face = recon<RECON>(P, sm);
+            wchar[n] = plus_face + minus_face;
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) sum += Rm[n][r] * wchar[r];
+            wface[n] = sum;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[s] = fm[S - 1 - s][m]; }
+            double plus_face = recon<RECON>(P, sp);
+            double minus_face = recon<RECON>(P, sm);
+            wface[m] = plus_face + minus_face;
+        }
+    }
+    int fid;
+    if (DIR == 0) fid = (k * P.sy + j) * (P.sx - 1) + i;
+    else if (DIR == 1) fid = (k * (P.sy - 1) + j) * P.sx + i;
+    else fid = (k * P.sy + j) * P.sx + i;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) out.f[DIR * 5 + m][fid] = wface[m];
+}
+
+// Flux difference of range_kutta.cpp:51-98 from the face arrays into the
+// accumulator: A = ((fx_r-fx_l)*idx + (fy_t-fy_b)*idy) + (fz_f-fz_b)*idz.
+__global__ void k_face_diff(DevParams P, Fields F, FaceArrays ff)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    const int fxr = (k * P.sy + j) * (P.sx - 1) + i;
+    const int fyt = (k * (P.sy - 1) + j) * P.sx + i;
+    const int fzf = (k * P.sy + j) * P.sx + i;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        double fx_r = ff.f[m][fxr], fx_l = ff.f[m][fxr - 1];
+        double fy_t = ff.f[5 + m][fyt], fy_b = ff.f[5 + m][fyt - P.sx];
+        double fz_f = ff.f[10 + m][fzf], fz_b = ff.f[10 + m][fzf - P.sx * P.sy];
+        F.A[m][id] = (fx_r - fx_l) * P.idx + (fy_t - fy_b) * P.idy + (fz_f - fz_b) * P.idz;
+    }
+}
+
+// ------------------------------------------------------------- viscous
+
+// compute_gradients (ViscousFlux.cpp:240-279), periodic axes: full order.
+template <int VIS>
+__global__ __launch_bounds__(256) void k_grad(DevParams P, Fields F)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    const int sxy = P.sx * P.sy;
+    const double *src[4] = {F.W[1], F.W[2], F.W[3], F.W[5]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double *c = src[q] + id;
+        F.G[3 * q + 0][id] = diff_central<VIS>(c, 1, P.dx);
+        F.G[3 * q + 1][id] = diff_central<VIS>(c, P.sx, P.dy);
+        F.G[3 * q + 2][id] = diff_central<VIS>(c, sxy, P.dz);
+    }
+}
+
+// compute_viscous_flux (ViscousFlux.cpp:282-339)
+__global__ __launch_bounds__(256) void k_vflux(DevParams P, Fields F)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    const double Pr = P.Pr, Cp = P.Cp;
+    double mu = get_mu(P, F.W[5][id]);
+    double du_dx = F.G[0][id], du_dy = F.G[1][id], du_dz = F.G[2][id];
+    double dv_dx = F.G[3][id], dv_dy = F.G[4][id], dv_dz = F.G[5][id];
+    double dw_dx = F.G[6][id], dw_dy = F.G[7][id], dw_dz = F.G[8][id];
+    double dT_dx = F.G[9][id], dT_dy = F.G[10][id], dT_dz = F.G[11][id];
+    double divU = du_dx + dv_dy + dw_dz;
+    double tau_xx = 2.0 * mu * du_dx - (2.0 / 3.0) * mu * divU;
+    double tau_yy = 2.0 * mu * dv_dy - (2.0 / 3.0) * mu * divU;
+    double tau_zz = 2.0 * mu * dw_dz - (2.0 / 3.0) * mu * divU;
+    double tau_xy = mu * (du_dy + dv_dx);
+    double tau_xz = mu * (du_dz + dw_dx);
+    double tau_yz = mu * (dv_dz + dw_dy);
+    double qx = mu * Cp / Pr * dT_dx;
+    double qy = mu * Cp / Pr * dT_dy;
+    double qz = mu * Cp / Pr * dT_dz;
+    double u = F.W[1][id], v = F.W[2][id], w = F.W[3][id];
+    F.VF[0][id] = tau_xx;
+    F.VF[1][id] = tau_xy;
+    F.VF[2][id] = tau_xz;
+    F.VF[3][id] = u * tau_xx + v * tau_xy + w * tau_xz + qx;
+    F.VF[4][id] = tau_xy;
+    F.VF[5][id] = tau_yy;
+    F.VF[6][id] = tau_yz;
+    F.VF[7][id] = u * tau_xy + v * tau_yy + w * tau_yz + qy;
+    F.VF[8][id] = tau_xz;
+    F.VF[9][id] = tau_yz;
+    F.VF[10][id] = tau_zz;
+    F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
+}
+
+// compute_vis_flux (ViscousFlux.cpp:341-386) + the RHS combine of
+// range_kutta.cpp:61-97, and for MODE 1..3 the RK3 stage update of
+// range_kutta.cpp:109-147 with the C2P of field_structures.h:405-427.
+// Viscous-flux ghosts are read as periodic images (index wrap), which is
+// what exchange_halos_viscous_flux stores there on one periodic rank.
+// MODE 0: A <- rhs.  MODE s = 1,2,3: W <- C2P(stage s update), U0 written at
+// stage 1; stage 3 flags non-physical cells (range_kutta.cpp:150-161).
+template <int VIS, int MODE>
+__global__ __launch_bounds__(256) void k_vdiv(DevParams P, Fields F, double dt,
+                                              unsigned long long *flag, unsigned long long step_key)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int ng = P.ng;
+    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = ng + blockIdx.z;
+    if (i >= ng + P.nx || j >= ng + P.ny) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    int ox[2 * R + 1], oy[2 * R + 1], oz[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; ++o) {
+        ox[o + R] = (k * P.sy + j) * P.sx + wrapi(i + o, ng, P.nx);
+        oy[o + R] = (k * P.sy + wrapi(j + o, ng, P.ny)) * P.sx + i;
+        oz[o + R] = (wrapi(k + o, ng, P.nz) * P.sy + j) * P.sx + i;
+    }
+    double r[5];
+    r[0] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double win[2 * R + 1];
+        double acc = 0.0;
+#pragma unroll
+        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[m][ox[o]];
+        acc += diff_window<VIS>(win + R, P.dx);
+#pragma unroll
+        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[4 + m][oy[o]];
+        acc += diff_window<VIS>(win + R, P.dy);
+#pragma unroll
+        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[8 + m][oz[o]];
+        acc += diff_window<VIS>(win + R, P.dz);
+        r[m + 1] = acc;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) r[m] += -(F.A[m][id]);
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) F.A[m][id] = r[m];
+    } else {
+        const double gamma = P.gamma;
+        double rr = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
+        double U[5];
+        U[0] = rr;
+        U[1] = rr * u;
+        U[2] = rr * v;
+        U[3] = rr * w;
+        {
+            double kinetic = 0.5 * (u * u + v * v + w * w);
+            double e_internal = p / ((gamma - 1.0) * rr);
+            U[4] = rr * (e_internal + kinetic);
+        }
+        double Un[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if constexpr (MODE == 1) {
+                F.U0[m][id] = U[m];
+                Un[m] = U[m] + dt * r[m];
+            } else if constexpr (MODE == 2) {
+                Un[m] = 0.75 * F.U0[m][id] + 0.25 * (U[m] + dt * r[m]);
+            } else {
+                Un[m] = (1.0 / 3.0) * F.U0[m][id] + (2.0 / 3.0) * (U[m] + dt * r[m]);
+            }
+        }
+        double rn = Un[0];
+        if (rn <= 0.0) rn = 1e-12;
+        double un = Un[1] / rn;
+        double vn = Un[2] / rn;
+        double wn = Un[3] / rn;
+        double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+        double e_internal = Un[4] / rn - kinetic;
+        double pn = (gamma - 1.0) * rn * e_internal;
+        double Tn = pn / (rn * P.Rgas);
+        F.W[0][id] = rn; F.W[1][id] = un; F.W[2][id] = vn; F.W[3][id] = wn; F.W[4][id] = pn;
+        F.W[5][id] = Tn;
+        if constexpr (MODE == 3) {
+            if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
+                atomicMin(flag, step_key | (unsigned long long)id);
+        }
+    }
+}
+
+// ----------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// compute_timestep (timestep.cpp:5-33): min over interior cells; the result
+// (a positive double) is folded with an unsigned 64-bit atomicMin on its bits.
+__global__ __launch_bounds__(256) void k_cfl(DevParams P, Fields F, unsigned long long *out)
+{
+    __shared__ double red[4];
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    double m = 1e9;
+    if (i < P.ng + P.nx && j < P.ng + P.ny) {
+        const int id = (k * P.sy + j) * P.sx + i;
+        double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
+        double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
+        double dt_x = P.dx / (fabs(u) + a);
+        double dt_y = P.dy / (fabs(v) + a);
+        double dt_z = P.dz / (fabs(w) + a);
+        m = fmin(m, fmin(dt_x, fmin(dt_y, dt_z)));
+    }
+    m = wave_min(m);
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    const int wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double b = red[0];
+        for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; ++q) b = fmin(b, red[q]);
+        atomicMin(out, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// compute_total_energy (postprocess.cpp:103-128): per-block partial sums of
+// 0.5*rho*|u|^2*dV, added on the host in block order.
+__global__ __launch_bounds__(256) void k_ke(DevParams P, Fields F, double *partial)
+{
+    __shared__ double red[4];
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    double s = 0.0;
+    if (i < P.ng + P.nx && j < P.ng + P.ny) {
+        const int id = (k * P.sy + j) * P.sx + i;
+        double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id];
+        double kinetic = 0.5 * rho * (u * u + v * v + w * w);
+        s = kinetic * (P.dx * P.dy * P.dz);
+    }
+    s = wave_sum(s);
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    const int wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    if (lane == 0) red[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double b = 0.0;
+        for (int q = 0; q < (int)(blockDim.x * blockDim.y) / 64; ++q) b += red[q];
+        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = b;
+    }
+}
+
+// ------------------------------------------------------ launch helpers
+
+static inline dim3 cell_grid(const DevParams &P)
+{
+    return dim3((P.nx + 63) / 64, (P.ny + 3) / 4, P.nz);
+}
+static const dim3 kCellBlock(64, 4, 1);
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_prim_setup(hipStream_t s, const DevParams &P, const Fields &F)
+{
+    int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(k_prim_setup, dim3((n + 255) / 256), dim3(256), 0, s, P, F, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_c2p_all(hipStream_t s, const DevParams &P, const Fields &F, const double *const *src)
+{
+    int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(k_c2p_all, dim3((n + 255) / 256), dim3(256), 0, s, P, F, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_p2c_all(hipStream_t s, const DevParams &P, const Fields &F, double *const *dst)
+{
+    int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(k_p2c_all, dim3((n + 255) / 256), dim3(256), 0, s, P, F, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F)
+{
+    int m = P.ny * P.nz;
+    if (P.nx * P.nz > m) m = P.nx * P.nz;
+    if (P.nx * P.ny > m) m = P.nx * P.ny;
+    int total = 2 * P.ng * m;
+    hipLaunchKernelGGL(k_fill_periodic, dim3((total + 255) / 256, 3), dim3(256), 0, s, P, F);
+    return hipGetLastError();
+}
+
+using SweepFn = void (*)(DevParams, Fields, int);
+
+template <int RECON, int FVS>
+static SweepFn pick_sweep(int dir, int acc)
+{
+    if (dir == 0) return acc ? (SweepFn)k_sweep<0, RECON, FVS, 1> : (SweepFn)k_sweep<0, RECON, FVS, 0>;
+    if (dir == 1) return acc ? (SweepFn)k_sweep<1, RECON, FVS, 1> : (SweepFn)k_sweep<1, RECON, FVS, 0>;
+    return acc ? (SweepFn)k_sweep<2, RECON, FVS, 1> : (SweepFn)k_sweep<2, RECON, FVS, 0>;
+}
+
+template <int FVS>
+static SweepFn pick_sweep_r(int recon, int dir, int acc)
+{
+    switch (recon) {
+    case R_WENO5: return pick_sweep<R_WENO5, FVS>(dir, acc);
+    case R_LINEAR: return pick_sweep<R_LINEAR, FVS>(dir, acc);
+    case R_MDCD: return pick_sweep<R_MDCD, FVS>(dir, acc);
+    case R_MDCD_HYBRID: return pick_sweep<R_MDCD_HYBRID, FVS>(dir, acc);
+    case R_UD7: return pick_sweep<R_UD7, FVS>(dir, acc);
+    case R_UD7L: return pick_sweep<R_UD7L, FVS>(dir, acc);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int acc, int seg_len)
+{
+    SweepFn fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW>(P.recon, dir, acc)
+                                 : pick_sweep_r<FVS_LF>(P.recon, dir, acc);
+    if (!fn) return hipErrorInvalidValue;
+    const int na = dir == 0 ? P.ny : P.nx;
+    const int nb = dir == 2 ? P.ny : P.nz;
+    const int nd = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
+    const int nseg = (nd + seg_len - 1) / seg_len;
+    hipLaunchKernelGGL(fn, dim3((na + 63) / 64, nb, nseg), dim3(64), 0, s, P, F, seg_len);
+    return hipGetLastError();
+}
+
+using FaceFn = void (*)(DevParams, Fields, FaceArrays);
+
+template <int RECON, int FVS, int CHAR>
+static FaceFn pick_faces(int dir)
+{
+    if (dir == 0) return k_faces<0, RECON, FVS, CHAR>;
+    if (dir == 1) return k_faces<1, RECON, FVS, CHAR>;
+    return k_faces<2, RECON, FVS, CHAR>;
+}
+
+template <int FVS, int CHAR>
+static FaceFn pick_faces_r(int recon, int dir)
+{
+    switch (recon) {
+    case R_WENO5: return pick_faces<R_WENO5, FVS, CHAR>(dir);
+    case R_LINEAR: return pick_faces<R_LINEAR, FVS, CHAR>(dir);
+    case R_MDCD: return pick_faces<R_MDCD, FVS, CHAR>(dir);
+    case R_MDCD_HYBRID: return pick_faces<R_MDCD_HYBRID, FVS, CHAR>(dir);
+    case R_UD7: return pick_faces<R_UD7, FVS, CHAR>(dir);
+    case R_UD7L: return pick_faces<R_UD7L, FVS, CHAR>(dir);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &out,
+                        int dir, int char_recon)
+{
+    FaceFn fn;
+    if (P.fvs == FVS_SW) fn = char_recon ? pick_faces_r<FVS_SW, 1>(P.recon, dir) : pick_faces_r<FVS_SW, 0>(P.recon, dir);
+    else fn = char_recon ? pick_faces_r<FVS_LF, 1>(P.recon, dir) : pick_faces_r<FVS_LF, 0>(P.recon, dir);
+    if (!fn) return hipErrorInvalidValue;
+    const int n0 = dir == 0 ? P.nx + 1 : P.nx;
+    const int n1 = dir == 1 ? P.ny + 1 : P.ny;
+    const int n2 = dir == 2 ? P.nz + 1 : P.nz;
+    hipLaunchKernelGGL(fn, dim3((n0 + 63) / 64, (n1 + 3) / 4, n2), kCellBlock, 0, s, P, F, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &ff)
+{
+    hipLaunchKernelGGL(k_face_diff, cell_grid(P), kCellBlock, 0, s, P, F, ff);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad(hipStream_t s, const DevParams &P, const Fields &F)
+{
+    switch (P.vis) {
+    case VIS_C8: hipLaunchKernelGGL(k_grad<VIS_C8>, cell_grid(P), kCellBlock, 0, s, P, F); break;
+    case VIS_C6: hipLaunchKernelGGL(k_grad<VIS_C6>, cell_grid(P), kCellBlock, 0, s, P, F); break;
+    case VIS_C4: hipLaunchKernelGGL(k_grad<VIS_C4>, cell_grid(P), kCellBlock, 0, s, P, F); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vflux(hipStream_t s, const DevParams &P, const Fields &F)
+{
+    hipLaunchKernelGGL(k_vflux, cell_grid(P), kCellBlock, 0, s, P, F);
+    return hipGetLastError();
+}
+
+template <int VIS>
+static void vdiv_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt,
+                      unsigned long long *flag, unsigned long long key)
+{
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_vdiv<VIS, 0>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
+    case 1: hipLaunchKernelGGL((k_vdiv<VIS, 1>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
+    case 2: hipLaunchKernelGGL((k_vdiv<VIS, 2>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
+    default: hipLaunchKernelGGL((k_vdiv<VIS, 3>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
+    }
+}
+
+hipError_t launch_vdiv(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt,
+                       unsigned long long *flag, unsigned long long key)
+{
+    switch (P.vis) {
+    case VIS_C8: vdiv_mode<VIS_C8>(s, P, F, mode, dt, flag, key); break;
+    case VIS_C6: vdiv_mode<VIS_C6>(s, P, F, mode, dt, flag, key); break;
+    case VIS_C4: vdiv_mode<VIS_C4>(s, P, F, mode, dt, flag, key); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsigned long long *out)
+{
+    hipLaunchKernelGGL(k_cfl, cell_grid(P), kCellBlock, 0, s, P, F, out);
+    return hipGetLastError();
+}
+
+int ke_blocks(const DevParams &P)
+{
+    dim3 g = cell_grid(P);
+    return (int)(g.x * g.y * g.z);
+}
+
+hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial)
+{
+    hipLaunchKernelGGL(k_ke, cell_grid(P), kCellBlock, 0, s, P, F, partial);
+    return hipGetLastError();
+}
+
+}  // namespace ns3d
