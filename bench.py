@@ -99,12 +99,15 @@ def main():
                           vis_scheme="C8th", sutherland_s=110.4 / 273.15)
     s = ns.Solver(p, g)
     s.upload_primitive(*st)
-    del st
     H = hip()
     ev0, ev1 = C.c_void_p(), C.c_void_p()
     H.hipEventCreate(C.byref(ev0))
     H.hipEventCreate(C.byref(ev1))
 
+    del st
+    # achievable streaming bandwidth on this device (writes gradient and
+    # viscous-flux scratch only, which every RHS recomputes)
+    copy_gbps = s.copy_probe(10)
     for _ in range(args.warmup):
         s.rk3_step(s.timestep())
     s.sync()
@@ -165,6 +168,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "step_model": {"bytes_per_cell_update": STEP_MODEL_BYTES_FP64,
                        "hbm_frac": round(STEP_MODEL_BYTES_FP64 * value / (HBM_PEAK_GBS * 1e9), 4)},
+        "copy_probe_GBps": round(copy_gbps, 1),
         "kernels": kernels,
         "wall_s_timed": round(wall, 4),
         "kinetic_energy_end": ek,

@@ -200,6 +200,12 @@ class Solver:
     def stream(self):
         return self._L.ns3d_stream(self.ctx)
 
+    def copy_probe(self, reps=10):
+        """Achieved GB/s of a plain streaming copy over 12 cell arrays."""
+        g = C.c_double()
+        self._check(self._L.ns3d_copy_probe(self.ctx, int(reps), C.byref(g)), "copy_probe")
+        return g.value
+
     def profile(self, on=True):
         """Start (or stop) per-kernel event timing; resets the totals."""
         self._check(self._L.ns3d_profile_enable(self.ctx, 1 if on else 0), "profile_enable")

@@ -83,6 +83,7 @@ SIGNATURES = {
     "ns3d_total_energy": (C.c_int, [C.c_void_p, DP]),
     "ns3d_sync": (C.c_int, [C.c_void_p]),
     "ns3d_stream": (C.c_void_p, [C.c_void_p]),
+    "ns3d_copy_probe": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "ns3d_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "ns3d_profile_count": (C.c_int, [C.c_void_p]),
     "ns3d_profile_entry": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_longlong)]),

@@ -583,6 +583,27 @@ int ns3d_sync(ns3d_ctx *c)
 
 void *ns3d_stream(ns3d_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int ns3d_copy_probe(ns3d_ctx *c, int reps, double *gbps)
+{
+    if (!c || reps <= 0 || !gbps) return NS3D_ERR_INVALID_ARG;
+    // copy the 12 gradient arrays into the 12 viscous-flux arrays (scratch)
+    const long n = (long)c->ncell * 12;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCALL(c, hipEventCreate(&a));
+    HIPCALL(c, hipEventCreate(&b));
+    HIPCALL(c, launch_copy(c->stream, c->f.G[0], c->f.VF[0], n));
+    HIPCALL(c, hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) TIMED(c, "copy", launch_copy(c->stream, c->f.G[0], c->f.VF[0], n));
+    HIPCALL(c, hipEventRecord(b, c->stream));
+    HIPCALL(c, hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCALL(c, hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *gbps = 2.0 * 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    return NS3D_OK;
+}
+
 int ns3d_profile_enable(ns3d_ctx *c, int on)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;

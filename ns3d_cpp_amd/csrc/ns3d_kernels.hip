@@ -478,17 +478,21 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// compute_timestep (timestep.cpp:5-33): min over interior cells; the result
-// (a positive double) is folded with an unsigned 64-bit atomicMin on its bits.
+// compute_timestep (timestep.cpp:5-33): min over interior cells.  A bounded
+// grid strides over the interior; each block folds its minimum (a positive
+// double) into *out with one unsigned 64-bit atomicMin on its bits.
 __global__ __launch_bounds__(256) void k_cfl(DevParams P, Fields F, unsigned long long *out)
 {
     __shared__ double red[4];
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
+    const int nxy = P.nx * P.ny;
+    const int total = nxy * P.nz;
     double m = 1e9;
-    if (i < P.ng + P.nx && j < P.ng + P.ny) {
-        const int id = (k * P.sy + j) * P.sx + i;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int k = t / nxy;
+        const int r = t - k * nxy;
+        const int j = r / P.nx;
+        const int i = r - j * P.nx;
+        const int id = ((P.ng + k) * P.sy + (P.ng + j)) * P.sx + (P.ng + i);
         double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
         double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
         double dt_x = P.dx / (fabs(u) + a);
@@ -497,15 +501,23 @@ __global__ __launch_bounds__(256) void k_cfl(DevParams P, Fields F, unsigned lon
         m = fmin(m, fmin(dt_x, fmin(dt_y, dt_z)));
     }
     m = wave_min(m);
-    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
-    const int wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = m;
     __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        double b = red[0];
-        for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; ++q) b = fmin(b, red[q]);
+    if (threadIdx.x == 0) {
+        double b = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
         atomicMin(out, (unsigned long long)__double_as_longlong(b));
     }
+}
+
+// Plain streaming copy dst <- src of n doubles (two per thread per
+// iteration, 16-byte accesses): the reference point for achievable HBM
+// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n2)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (long)gridDim.x * blockDim.x)
+        dst[t] = src[t];
 }
 
 // compute_total_energy (postprocess.cpp:103-128): per-block partial sums of
@@ -704,7 +716,20 @@ hipError_t launch_vdiv(hipStream_t s, const DevParams &P, const Fields &F, int m
 
 hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsigned long long *out)
 {
-    hipLaunchKernelGGL(k_cfl, cell_grid(P), kCellBlock, 0, s, P, F, out);
+    const long total = (long)P.nx * P.ny * P.nz;
+    long blocks = (total + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_cfl, dim3((unsigned)blocks), dim3(256), 0, s, P, F, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n)
+{
+    const long n2 = n / 2;
+    long blocks = (n2 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst), n2);
     return hipGetLastError();
 }
 

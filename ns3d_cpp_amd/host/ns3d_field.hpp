@@ -1,0 +1,67 @@
+// ns3d_field.hpp -- C++ host layer over the C ABI that keeps the reference's
+// call surface (ns3d_func.h) for one periodic subdomain resident on one GPU.
+//
+// The reference's free functions take a host Field3D& and mutate it in place;
+// here Field3D owns a device context and the same functions act on the
+// device-resident state.  Host mirrors of the arrays are filled on demand
+// (sync_to_host) for monitors and output, as the reference's monitors read
+// its host arrays (maincycle.cpp:62-102).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ns3d_gpu.h"
+
+namespace ns3d_host {
+
+// GridDesc (field_structures.h:32-37) and SolverParams (:85-166) are the C
+// structs of the ABI.
+using GridDesc = ns3d_grid;
+using SolverParams = ns3d_params;
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int st, const std::string &what) : std::runtime_error(what), status(st) {}
+};
+
+// Field3D (field_structures.h:176-472) with device-resident storage.
+struct Field3D {
+    ns3d_ctx *ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0, ng = 0, sx = 0, sy = 0, sz = 0;
+    // host mirrors (idx3 layout), valid after sync_to_host()
+    std::vector<double> rho, u, v, w, p, T;
+    std::vector<double> rhou, rhov, rhow, E;
+    double global_Etot = 0.0;
+
+    Field3D() = default;
+    Field3D(const Field3D &) = delete;
+    Field3D &operator=(const Field3D &) = delete;
+    ~Field3D();
+
+    // Field3D::allocate (field_structures.h:225-334)
+    void allocate(const SolverParams &P, const GridDesc &G);
+    inline int I(int i, int j, int k) const { return (k * sy + j) * sx + i; }
+    // upload rho,u,v,w,p (host mirrors) then apply_boundary + P2C (main.cpp:70-71)
+    void upload_primitive();
+    // download primitives and conserved into the host mirrors
+    void sync_to_host();
+};
+
+void check(int st, const char *what, const Field3D *F = nullptr);
+
+// ns3d_func.h call surface
+void apply_boundary(Field3D &F, GridDesc &G, const SolverParams &P);     // :56
+void du_inviscous(Field3D &F, const SolverParams &P);                      // :154
+void compute_gradients(Field3D &F, const GridDesc &G, const SolverParams &P);  // :157
+void compute_viscous_flux(Field3D &F, const SolverParams &P);              // :160
+void compute_rhs(Field3D &F, GridDesc &G, SolverParams &P);                // :99
+void runge_kutta_3(Field3D &F, GridDesc &G, SolverParams &P, double dt);   // :102
+double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P);  // timestep.cpp:5
+void compute_total_energy(Field3D &F, const GridDesc &G, const SolverParams &P); // postprocess.cpp:103
+// time_advance (maincycle.cpp:7-113): dt fixed or CFL, dt clipping to the
+// output time and TotalTime, monitor lines on rank 0's stdout.
+void time_advance(Field3D &F, GridDesc &G, SolverParams &P);
+
+}  // namespace ns3d_host
