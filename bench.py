@@ -1,4 +1,4 @@
-"""Throughput of the RK3 RHS path on one MI355X.
+"""Throughput of the RK3 RHS path on one MI355X: FP64 vs mixed precision.
 
 Workload (default): decaying compressible isotropic turbulence on a 256^3
 periodic box [0, 2pi)^3 -- seeded solenoidal random-phase velocity with
@@ -8,9 +8,12 @@ S/T_ref = 110.4/273.15; WENO5 + Steger-Warming + C8 central viscous terms;
 SSP-RK3 with the CFL (0.5) time step recomputed every step.
 
 A step = one RK3 step (3 RHS + 3 stage updates + the CFL reduction).
-Metric: cell-updates/s = N^3 / (seconds per step).  Inputs are resident in
-HBM before the timed region.  Per-kernel durations come from HIP events
-recorded around every launch on the solver's stream during the timed region.
+Metric: cell-updates/s = N^3 / (seconds per step), for the mixed-precision
+path (`value`) and the FP64 path (`fp64`).  Inputs are resident in HBM
+before the timed region.  Per-kernel durations come from HIP events recorded
+around every launch on the solver's stream inside the timed region.  The
+mixed path's error is measured against the FP64 path over --err-steps steps
+from the same initial state, with the FP64 path's CFL time step.
 """
 import argparse
 import ctypes as C
@@ -26,19 +29,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RK3 cell-updates/s @256^3 FP64 vs mixed-FP32; % HBM peak; adjoint/fwd cost"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per interior cell per launch of each FP64 kernel
-# (every array touched once per launch; halos reused on chip).
-ALG_BYTES_PER_CELL = {
-    "sweep_x": 80,        # read rho,u,v,w,p; write A(5)
-    "sweep_y": 120,       # + read A(5)
-    "sweep_z": 120,
-    "grad": 128,          # read u,v,w,T; write 12 gradients
-    "vflux": 224,         # read 12 gradients + T,u,v,w; write 12 fluxes
-    "vdiv_stage": 264,    # read 12 fluxes, A(5), W(5), U0(5); write W(6)
-    "cfl": 40,            # read rho,u,v,w,p
+# Algorithmic HBM bytes per interior cell per launch of each kernel (every
+# array touched once per launch; halos reused on chip).
+ALG_BYTES = {
+    "fp64": {
+        "sweep_x": 80,        # read rho,u,v,w,p; write A(5)
+        "sweep_y": 120,       # + read A(5)
+        "sweep_z": 120,
+        "gradflux": 128,      # read u,v,w,T; write 12 viscous fluxes
+        "vdiv_stage": 264,    # read 12 fluxes, A(5), W(5), U0(5); write W(6)
+        "cfl": 40,            # read rho,u,v,w,p
+    },
+    "mixed": {
+        "sweep_x": 40,        # read FP32 shadow (5); write FP32 A(5)
+        "sweep_y": 60,        # + read A(5)
+        "sweep_z": 60,
+        "gradflux": 64,       # read u,v,w,T-1 (FP32); write 12 FP32 fluxes
+        "vdiv_stage": 212,    # read 12 FP32 fluxes, A32, U64, U0; write U64, shadow(6)
+        "cfl": 20,            # read the FP32 shadow
+    },
 }
-# Per cell-update model of the full step (3 RHS + 3 stages + CFL read).
-STEP_MODEL_BYTES_FP64 = 2448
+# Per cell-update byte model of the full step used for "% of HBM peak"
+# (3 RHS + 3 stages + CFL; FP64 2448 B, mixed 1564 B).
+STEP_MODEL_BYTES = {"fp64": 2448, "mixed": 1564}
 
 
 def hip():
@@ -51,6 +64,99 @@ def hip():
     return lib
 
 
+def make(ns, n, Ma, Re, recon, precision):
+    return ns.make_params(n, Re=Re, Ma=Ma, Pr=0.72, cfl=0.5, recon=recon, fvs_type="StagerWarming",
+                          vis_scheme="C8th", sutherland_s=110.4 / 273.15, precision=precision)
+
+
+def traffic_from_profiles(prec, kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration) and
+    committed under profiles/; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{prec}_hit256.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
+    p, g = make(ns, n, Ma, Re, recon, prec)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    for _ in range(warmup):
+        s.rk3_step(s.timestep())
+    s.sync()
+    H.hipDeviceSynchronize()
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    H.hipEventCreate(C.byref(ev0))
+    H.hipEventCreate(C.byref(ev1))
+    s.profile(True)
+    t0 = time.perf_counter()
+    H.hipEventRecord(ev0, s.stream())
+    for _ in range(steps):
+        s.rk3_step(s.timestep())
+    H.hipEventRecord(ev1, s.stream())
+    H.hipEventSynchronize(ev1)
+    s.sync()
+    H.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms = C.c_float()
+    H.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    rep = s.profile_report()
+    s.profile(False)
+    s.close()
+    ms_step = ms.value / steps
+    value = n ** 3 / (ms_step * 1e-3)
+    alg = ALG_BYTES[prec]
+    kernels = {}
+    for k, (tot, cnt) in sorted(rep.items(), key=lambda kv: -kv[1][0]):
+        e = {"avg_ms": round(tot / cnt, 4), "launches": cnt, "share": round(tot / ms.value, 4)}
+        if k in alg:
+            e["GBps"] = round(alg[k] * n ** 3 / (tot / cnt * 1e-3) / 1e9, 1)
+        kernels[k] = e
+    dom = max(rep, key=lambda k: rep[k][0])
+    dom_avg = rep[dom][0] / rep[dom][1]
+    achieved = alg.get(dom, 0) * n ** 3 / (dom_avg * 1e-3) / 1e9
+    traffic = traffic_from_profiles(prec, dom)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+    step_bytes = 3 * sum(alg[k] for k in alg if k != "cfl") + alg["cfl"]
+    return {"value": value, "ms_per_step": ms_step, "wall_s_timed": round(wall, 4), "kernels": kernels,
+            "roofline": roof,
+            "kernel_bytes_per_cell_update": step_bytes,
+            "kernel_bytes_hbm_frac": round(step_bytes * value / (HBM_PEAK_GBS * 1e9), 4),
+            "step_model": {"bytes_per_cell_update": STEP_MODEL_BYTES[prec],
+                           "hbm_frac": round(STEP_MODEL_BYTES[prec] * value / (HBM_PEAK_GBS * 1e9), 4)}}
+
+
+def mixed_error(ns, n, st, Ma, Re, recon, nsteps):
+    """rel-L2 of each conserved component, mixed vs FP64, after nsteps."""
+    pd, gd = make(ns, n, Ma, Re, recon, "fp64")
+    pm, gm = make(ns, n, Ma, Re, recon, "mixed")
+    d, m = ns.Solver(pd, gd), ns.Solver(pm, gm)
+    d.upload_primitive(*st)
+    m.upload_primitive(*st)
+    t = 0.0
+    for _ in range(nsteps):
+        dt = d.timestep()
+        d.rk3_step(dt)
+        m.rk3_step(dt)
+        t += dt
+    ud, um = d.conserved(), m.conserved()
+    errs = []
+    for a, b in zip(um, ud):
+        a, b = d.interior(a), d.interior(b)
+        errs.append(float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b))))
+    ek = (d.total_energy(), m.total_energy())
+    d.close()
+    m.close()
+    return {"steps": nsteps, "time": t, "rel_l2_U": errs, "max_rel_l2": max(errs),
+            "kinetic_energy_fp64": ek[0], "kinetic_energy_mixed": ek[1]}
+
+
 def cpu_baseline(n, nsteps, Re, Mt, k0, seed):
     """The CPU FP64 oracle on the host's cores, on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -58,8 +164,7 @@ def cpu_baseline(n, nsteps, Re, Mt, k0, seed):
     from ns3d_cpp_amd import fields
     from oracle import Oracle, load
     st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=seed)
-    p, g = ns.make_params(n, Re=Re, Ma=Ma, Pr=0.72, cfl=0.5, recon="WENO5", fvs_type="StagerWarming",
-                          vis_scheme="C8th", sutherland_s=110.4 / 273.15)
+    p, g = make(ns, n, Ma, Re, "WENO5", "fp64")
     o = Oracle(p, g)
     o.set_primitive(*st)
     t0 = time.perf_counter()
@@ -80,6 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--recon", default="WENO5")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--err-steps", type=int, default=500)
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -95,85 +202,56 @@ def main():
 
     n, Re, Mt, k0 = args.n, 200.0, 0.3, 4.0
     st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=args.seed)
-    p, g = ns.make_params(n, Re=Re, Ma=Ma, Pr=0.72, cfl=0.5, recon=args.recon, fvs_type="StagerWarming",
-                          vis_scheme="C8th", sutherland_s=110.4 / 273.15)
-    s = ns.Solver(p, g)
-    s.upload_primitive(*st)
     H = hip()
-    ev0, ev1 = C.c_void_p(), C.c_void_p()
-    H.hipEventCreate(C.byref(ev0))
-    H.hipEventCreate(C.byref(ev1))
+    # achievable streaming bandwidth on this device
+    p, g = make(ns, n, Ma, Re, args.recon, "fp64")
+    probe = ns.Solver(p, g)
+    copy_gbps = probe.copy_probe(10)
+    probe.close()
 
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {prec: time_path(ns, H, n, st, Ma, Re, args.recon, prec, args.steps, args.warmup) for prec in precs}
+    err = None
+    if args.precision == "both" and args.err_steps > 0:
+        err = mixed_error(ns, n, st, Ma, Re, args.recon, args.err_steps)
     del st
-    # achievable streaming bandwidth on this device (writes gradient and
-    # viscous-flux scratch only, which every RHS recomputes)
-    copy_gbps = s.copy_probe(10)
-    for _ in range(args.warmup):
-        s.rk3_step(s.timestep())
-    s.sync()
-    H.hipDeviceSynchronize()
-
-    s.profile(True)
-    t0 = time.perf_counter()
-    H.hipEventRecord(ev0, s.stream())
-    for _ in range(args.steps):
-        s.rk3_step(s.timestep())
-    H.hipEventRecord(ev1, s.stream())
-    H.hipEventSynchronize(ev1)
-    s.sync()
-    H.hipDeviceSynchronize()
-    wall = time.perf_counter() - t0
-    ms = C.c_float()
-    H.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-    rep = s.profile_report()
-    s.profile(False)
-    ek = s.total_energy()
-
-    ms_per_step = ms.value / args.steps
-    value = n ** 3 / (ms_per_step * 1e-3)
-    # dominant kernel by total time
-    dom = max(rep, key=lambda k: rep[k][0])
-    dom_ms, dom_n = rep[dom]
-    dom_avg = dom_ms / dom_n
-    bytes_per_launch = ALG_BYTES_PER_CELL.get(dom, 0) * n ** 3
-    achieved = bytes_per_launch / (dom_avg * 1e-3) / 1e9
-    kernels = {k: {"avg_ms": round(v[0] / v[1], 4), "launches": v[1],
-                   "share": round(v[0] / ms.value, 4)} for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}
-    for k in kernels:
-        if k in ALG_BYTES_PER_CELL:
-            kernels[k]["GBps"] = round(ALG_BYTES_PER_CELL[k] * n ** 3 / (kernels[k]["avg_ms"] * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.cpu_steps, Re, Mt, k0, args.seed)
 
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": h["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"decaying isotropic turbulence {n}^3, k0=4, Mt=0.3, Re=200, "
                                f"{args.recon} + Steger-Warming + C8th, Sutherland 110.4/273.15, "
-                               "RK3 with CFL 0.5 dt every step, FP64",
-                   "n": n, "recon": args.recon, "fvs": "StagerWarming", "vis": "C8th", "precision": "fp64"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
-        "step_model": {"bytes_per_cell_update": STEP_MODEL_BYTES_FP64,
-                       "hbm_frac": round(STEP_MODEL_BYTES_FP64 * value / (HBM_PEAK_GBS * 1e9), 4)},
+                               f"RK3 with CFL 0.5 dt every step, {head} path",
+                   "n": n, "recon": args.recon, "fvs": "StagerWarming", "vis": "C8th", "precision": head},
+        "roofline": h["roofline"],
+        "step_model": h["step_model"],
+        "kernel_bytes_per_cell_update": h["kernel_bytes_per_cell_update"],
+        "kernel_bytes_hbm_frac": h["kernel_bytes_hbm_frac"],
         "copy_probe_GBps": round(copy_gbps, 1),
-        "kernels": kernels,
-        "wall_s_timed": round(wall, 4),
-        "kinetic_energy_end": ek,
-        "cpu_baseline": cpu,
+        "kernels": h["kernels"],
     }
+    if head == "mixed" and "fp64" in res:
+        line["fp64"] = res["fp64"]
+        line["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    if err is not None:
+        line["mixed_error_vs_fp64"] = err
+    line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

@@ -22,6 +22,9 @@ struct ns3d_ctx {
     ns3d_grid G{};
     DevParams dp{};
     Fields f{};
+    MFields m{};                 // mixed-precision arrays (precision == MIXED)
+    bool mixed = false;
+    float *fslab = nullptr;      // FP32 arrays of the mixed path
     FaceArrays faces{};          // allocated on first ns3d_du_inviscous
     int device = 0;
     hipStream_t stream = nullptr;
@@ -33,6 +36,9 @@ struct ns3d_ctx {
     int nblocks = 0;
     long long ncell = 0;
     int seg_len[3] = {64, 64, 64};
+    int seg_z = 32;                          // z segments of the z-marching kernels
+    int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
+    int sweep_minw = 2;                      // waves-per-EU floor of k_sweep
     unsigned long long bad_key = ~0ull;
     long long steps_done = 0;
     std::string err;
@@ -143,7 +149,9 @@ int check_supported(const ns3d_params &p, const ns3d_grid &g)
     for (int f = 0; f < 6; ++f)
         if (p.bc[f] != NS3D_BC_PERIODIC) return NS3D_ERR_UNSUPPORTED;
     if (p.recon == NS3D_RECON_WCNS) return NS3D_ERR_CONFIG;
-    if (p.precision != NS3D_PREC_FP64) return NS3D_ERR_UNSUPPORTED;
+    if (p.precision != NS3D_PREC_FP64 && p.precision != NS3D_PREC_MIXED) return NS3D_ERR_INVALID_ARG;
+    // the mixed path is component-wise with full-row x blocks
+    if (p.precision == NS3D_PREC_MIXED && p.char_recon) return NS3D_ERR_UNSUPPORTED;
     if (p.fvs_type != NS3D_FVS_STEGER_WARMING && p.fvs_type != NS3D_FVS_LAX_FRIEDRICHS)
         return NS3D_ERR_CONFIG;
     // periodic images of the ghost layers must exist inside the interior
@@ -159,7 +167,8 @@ int stream_sync(ns3d_ctx *c)
 
 int fill_ghosts(ns3d_ctx *c)
 {
-    TIMED(c, "fill_ghosts", launch_fill_ghosts(c->stream, c->dp, c->f));
+    if (c->mixed) TIMED(c, "fill_ghosts", launch_m_fill_ghosts(c->stream, c->dp, c->m));
+    else TIMED(c, "fill_ghosts", launch_fill_ghosts(c->stream, c->dp, c->f));
     return NS3D_OK;
 }
 
@@ -189,21 +198,35 @@ int inviscid(ns3d_ctx *c)
         TIMED(c, "face_diff", launch_face_diff(c->stream, c->dp, c->f, c->faces));
         return NS3D_OK;
     }
+    if (xrow_supported(c->dp)) TIMED(c, "sweep_x", launch_sweep_xrow(c->stream, c->dp, c->f));
+    else TIMED(c, "sweep_x", launch_sweep(c->stream, c->dp, c->f, 0, c->seg_len[0], c->sweep_minw));
+    TIMED(c, "sweep_y", launch_sweep(c->stream, c->dp, c->f, 1, c->seg_len[1], c->sweep_minw));
+    TIMED(c, "sweep_z", launch_sweep(c->stream, c->dp, c->f, 2, c->seg_len[2], c->sweep_minw));
+    return NS3D_OK;
+}
+
+// one RHS evaluation of the mixed-precision path
+int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
+{
     static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
-    for (int d = 0; d < 3; ++d)
-        TIMED(c, sname[d], launch_sweep(c->stream, c->dp, c->f, d, d > 0, c->seg_len[d]));
+    for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
+    TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z));
+    const unsigned long long key = (unsigned long long)c->steps_done << 40;
+    TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
+          launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, key));
     return NS3D_OK;
 }
 
 // one RHS evaluation finished by k_vdiv in the given mode
 int rhs_pass(ns3d_ctx *c, int mode, double dt)
 {
+    if (c->mixed) return rhs_pass_mixed(c, mode, dt);
     int st = inviscid(c);
     if (st) return st;
-    TIMED(c, "grad", launch_grad(c->stream, c->dp, c->f));
-    TIMED(c, "vflux", launch_vflux(c->stream, c->dp, c->f));
+    TIMED(c, "gradflux", launch_gradflux(c->stream, c->dp, c->f, c->seg_z, c->gradflux_minb));
     const unsigned long long key = (unsigned long long)c->steps_done << 40;
-    TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage", launch_vdiv(c->stream, c->dp, c->f, mode, dt, c->d_flag, key));
+    TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
+          launch_vdiv_z(c->stream, c->dp, c->f, mode, dt, c->seg_z, c->d_flag, key));
     return NS3D_OK;
 }
 
@@ -312,6 +335,25 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     for (int m = 0; m < 5; ++m) { c->f.A[m] = q; q += ncell; }
     for (int m = 0; m < 12; ++m) { c->f.G[m] = q; q += ncell; }
     for (int m = 0; m < 12; ++m) { c->f.VF[m] = q; q += ncell; }
+    d.rho_ref = 1.0;
+    d.p_ref = p.Rgas;   // p at rho = 1, T = 1: 1 / (gamma Ma^2)
+    c->mixed = p.precision == NS3D_PREC_MIXED;
+    if (c->mixed) {
+        if (!xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
+        const int nf = 6 + 5 + 12;
+        e = hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float));
+        if (e == hipSuccess) e = hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream);
+        if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc mixed"); ns3d_destroy(c); return r; }
+        float *fq = c->fslab;
+        for (int m = 0; m < 6; ++m) { c->m.S[m] = fq; fq += ncell; }
+        for (int m = 0; m < 5; ++m) { c->m.A[m] = fq; fq += ncell; }
+        for (int m = 0; m < 12; ++m) { c->m.VF[m] = fq; fq += ncell; }
+        for (int m = 0; m < 5; ++m) {
+            c->m.U[m] = c->f.W[m];      // conserved state lives in the W arrays
+            c->m.U0[m] = c->f.U0[m];
+            c->m.out[m] = c->f.G[m];    // RHS of ns3d_compute_rhs
+        }
+    }
     c->nblocks = ke_blocks(d);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_flag, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_partial, (size_t)c->nblocks * sizeof(double));
@@ -320,6 +362,22 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     const long long ncol[3] = {(long long)g.ny * g.nz, (long long)g.nx * g.nz, (long long)g.nx * g.ny};
     const int nd[3] = {g.nx, g.ny, g.nz};
     for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = pick_seg(nd[dd], ncol[dd]);
+    {
+        // z segments for the (i, j)-column kernels: ~8 waves per SIMD
+        const long long want = 256LL * 4 * 8 * 64;
+        const long long cols = (long long)g.nx * g.ny;
+        long long nseg = (want + cols - 1) / cols;
+        long long maxseg = g.nz / 16 > 0 ? g.nz / 16 : 1;
+        if (nseg > maxseg) nseg = maxseg;
+        if (nseg < 1) nseg = 1;
+        c->seg_z = (int)((g.nz + nseg - 1) / nseg);
+    }
+    if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
+    if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
+    if (const char *env = std::getenv("NS3D_SEG_Z")) {
+        int v = std::atoi(env);
+        if (v > 0) c->seg_z = v;
+    }
     if (const char *env = std::getenv("NS3D_SWEEP_SEG")) {
         int v = std::atoi(env);
         if (v > 0) for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = v;
@@ -337,6 +395,7 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->slab) (void)hipFree(c->slab);
     if (c->face_slab) (void)hipFree(c->face_slab);
+    if (c->fslab) (void)hipFree(c->fslab);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_ptrs) (void)hipFree(c->d_ptrs);
@@ -364,6 +423,17 @@ int ns3d_upload_primitive(ns3d_ctx *c, const double *rho, const double *u, const
 {
     if (!c || !rho || !u || !v || !w || !p) return NS3D_ERR_INVALID_ARG;
     const double *src[5] = {rho, u, v, w, p};
+    if (c->mixed) {
+        for (int m = 0; m < 5; ++m) {
+            int st = copy_in(c, c->f.G[m], src[m]);
+            if (st) return st;
+        }
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 5 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_from_prims(c->stream, c->dp, c->m, c->d_ptrs, 1));
+        int st = fill_ghosts(c);
+        if (st) return st;
+        return stream_sync(c);
+    }
     for (int m = 0; m < 5; ++m) {
         int st = copy_in(c, c->f.W[m], src[m]);
         if (st) return st;
@@ -380,6 +450,19 @@ int ns3d_upload_conserved(ns3d_ctx *c, const double *rho, const double *ru, cons
 {
     if (!c || !rho || !ru || !rv || !rw || !E) return NS3D_ERR_INVALID_ARG;
     const double *src[5] = {rho, ru, rv, rw, E};
+    if (c->mixed) {
+        // keep U exactly; the shadow comes from C2P(U)
+        for (int m = 0; m < 5; ++m) {
+            int st = copy_in(c, c->m.U[m], src[m]);
+            if (st) return st;
+        }
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_to_prims(c->stream, c->dp, c->m, c->d_ptrs));
+        HIPCALL(c, launch_m_from_prims(c->stream, c->dp, c->m, c->d_ptrs, 0));
+        int st = fill_ghosts(c);
+        if (st) return st;
+        return stream_sync(c);
+    }
     // stage the conserved input in the gradient arrays
     for (int m = 0; m < 5; ++m) {
         int st = copy_in(c, c->f.G[m], src[m]);
@@ -396,6 +479,15 @@ int ns3d_download_primitive(ns3d_ctx *c, double *rho, double *u, double *v, doub
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
     double *dst[6] = {rho, u, v, w, p, T};
+    if (c->mixed) {
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_to_prims(c->stream, c->dp, c->m, c->d_ptrs));
+        for (int m = 0; m < 6; ++m) {
+            int st = copy_out(c, dst[m], c->f.G[m]);
+            if (st) return st;
+        }
+        return stream_sync(c);
+    }
     for (int m = 0; m < 6; ++m) {
         int st = copy_out(c, dst[m], c->f.W[m]);
         if (st) return st;
@@ -406,6 +498,14 @@ int ns3d_download_primitive(ns3d_ctx *c, double *rho, double *u, double *v, doub
 int ns3d_download_conserved(ns3d_ctx *c, double *rho, double *ru, double *rv, double *rw, double *E)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) {
+        double *dm[5] = {rho, ru, rv, rw, E};
+        for (int m = 0; m < 5; ++m) {
+            int st = copy_out(c, dm[m], c->m.U[m]);
+            if (st) return st;
+        }
+        return stream_sync(c);
+    }
     // P2C into the gradient arrays (scratch), then copy out
     HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 5 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
     HIPCALL(c, launch_p2c_all(c->stream, c->dp, c->f, c->d_ptrs));
@@ -422,7 +522,7 @@ int ns3d_download_rhs(ns3d_ctx *c, double *r0, double *r1, double *r2, double *r
     if (!c) return NS3D_ERR_INVALID_ARG;
     double *dst[5] = {r0, r1, r2, r3, r4};
     for (int m = 0; m < 5; ++m) {
-        int st = copy_out(c, dst[m], c->f.A[m]);
+        int st = copy_out(c, dst[m], c->mixed ? c->m.out[m] : c->f.A[m]);
         if (st) return st;
     }
     return stream_sync(c);
@@ -439,6 +539,7 @@ int ns3d_apply_boundary(ns3d_ctx *c)
 int ns3d_du_inviscous(ns3d_ctx *c)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
     int st = ensure_faces(c);
     if (st) return st;
     for (int d = 0; d < 3; ++d)
@@ -463,6 +564,7 @@ int ns3d_download_face_flux(ns3d_ctx *c, int dir, double *f0, double *f1, double
 int ns3d_compute_gradients(ns3d_ctx *c)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
     HIPCALL(c, launch_grad(c->stream, c->dp, c->f));
     return stream_sync(c);
 }
@@ -480,6 +582,7 @@ int ns3d_download_gradients(ns3d_ctx *c, double *const grads[12])
 int ns3d_compute_viscous_flux(ns3d_ctx *c)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
     HIPCALL(c, launch_vflux(c->stream, c->dp, c->f));
     return stream_sync(c);
 }
@@ -544,7 +647,8 @@ int ns3d_compute_timestep_local(ns3d_ctx *c, double *dt)
     unsigned long long bits;
     std::memcpy(&bits, &init, sizeof(bits));
     HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 1, bits, 1));
-    TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
+    if (c->mixed) TIMED(c, "cfl", launch_m_cfl(c->stream, c->dp, c->m, c->d_flag + 1));
+    else TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
     unsigned long long h = 0;
     HIPCALL(c, hipMemcpyAsync(&h, c->d_flag + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     int st = stream_sync(c);
@@ -558,7 +662,16 @@ int ns3d_compute_timestep_local(ns3d_ctx *c, double *dt)
 int ns3d_total_energy(ns3d_ctx *c, double *ek)
 {
     if (!c || !ek) return NS3D_ERR_INVALID_ARG;
-    HIPCALL(c, launch_ke(c->stream, c->dp, c->f, c->d_partial));
+    if (c->mixed) {
+        // primitives into the FP64 scratch, then the FP64 reduction on them
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_to_prims(c->stream, c->dp, c->m, c->d_ptrs));
+        Fields tmp = c->f;
+        for (int m = 0; m < 6; ++m) tmp.W[m] = c->f.G[m];
+        HIPCALL(c, launch_ke(c->stream, c->dp, tmp, c->d_partial));
+    } else {
+        HIPCALL(c, launch_ke(c->stream, c->dp, c->f, c->d_partial));
+    }
     std::vector<double> h((size_t)c->nblocks);
     HIPCALL(c, hipMemcpyAsync(h.data(), c->d_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     int st = stream_sync(c);

@@ -17,6 +17,7 @@ struct DevParams {
     int fvs, recon, vis, mu_law;
     int nx, ny, nz, ng, sx, sy, sz;
     int periodic[3];
+    double rho_ref, p_ref;   // reference state of the mixed-precision shadow
 };
 
 enum { FVS_SW = 0, FVS_LF = 1 };

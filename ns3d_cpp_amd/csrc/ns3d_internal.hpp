@@ -16,6 +16,21 @@ struct Fields {
     double *VF[12];  // Fv(momx,momy,momz,E) Hv(...) Gv(...)
 };
 
+// Device arrays of the mixed-precision path.
+struct MFields {
+    double *U[5];    // conserved rho, rho u, rho v, rho w, E (FP64)
+    double *U0[5];   // conserved state at the start of the step (FP64)
+    double *out[5];  // RHS of ns3d_compute_rhs (FP64)
+    float *S[6];     // shadow: rho - rho_ref, u, v, w, p - p_ref, T - 1
+    float *A[5];     // inviscid accumulator
+    float *VF[12];   // viscous fluxes
+};
+
+__device__ __forceinline__ int wrapi_m(int q, int ng, int n)
+{
+    return q < ng ? q + n : (q >= ng + n ? q - n : q);
+}
+
 // Face-flux arrays of du_inviscous: f[5*dir + m] (idx_fx / idx_fy / idx_fz).
 struct FaceArrays {
     double *f[15];
@@ -26,7 +41,12 @@ hipError_t launch_prim_setup(hipStream_t s, const DevParams &P, const Fields &F)
 hipError_t launch_c2p_all(hipStream_t s, const DevParams &P, const Fields &F, const double *const *src);
 hipError_t launch_p2c_all(hipStream_t s, const DevParams &P, const Fields &F, double *const *dst);
 hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F);
-hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int acc, int seg_len);
+hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int seg_len, int minw);
+bool xrow_supported(const DevParams &P);
+hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F);
+hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb);
+hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                         unsigned long long *flag, unsigned long long key);
 hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &out,
                         int dir, int char_recon);
 hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &ff);
@@ -37,6 +57,15 @@ hipError_t launch_vdiv(hipStream_t s, const DevParams &P, const Fields &F, int m
 hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsigned long long *out);
 hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n);
 int ke_blocks(const DevParams &P);
+hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields &M, const double *const *src,
+                               int write_u);
+hipError_t launch_m_to_prims(hipStream_t s, const DevParams &P, const MFields &M, double *const *dst);
+hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields &M);
+hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out);
+hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len);
+hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                         unsigned long long *flag, unsigned long long key);
 hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 
 }  // namespace ns3d

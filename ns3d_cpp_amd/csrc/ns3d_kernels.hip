@@ -110,8 +110,8 @@ __global__ void k_fill_periodic(DevParams P, Fields F)
 // The cell result (F(f) - F(f-1)) * inv_d is what range_kutta.cpp:55-97
 // forms per direction; ACC = 0 stores it, ACC = 1 adds it to the
 // accumulator, so after x, y, z the accumulator holds (ax + ay) + az.
-template <int DIR, int RECON, int FVS, int ACC>
-__global__ __launch_bounds__(64) void k_sweep(DevParams P, Fields F, int seg_len)
+template <int DIR, int RECON, int FVS, int ACC, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int seg_len)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
@@ -141,14 +141,25 @@ __global__ __launch_bounds__(64) void k_sweep(DevParams P, Fields F, int seg_len
         split_state<FVS, DIR>(P, rho[id], uu[id], vv[id], ww[id], pp[id], fp[s], fm[s]);
     }
     double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // the primitives of the cell entering the window are loaded one face
+    // ahead, so their latency hides behind a face's reconstruction
+    double nx_[5];
+    {
+        int id = base + (c0 - 1 - MID + S - 1) * st;
+        nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
+    }
     for (int f = c0 - 1; f < c1; ++f) {
 #pragma unroll
         for (int s = 0; s < S - 1; ++s)
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m]; }
         {
-            int id = base + (f - MID + S - 1) * st;
-            split_state<FVS, DIR>(P, rho[id], uu[id], vv[id], ww[id], pp[id], fp[S - 1], fm[S - 1]);
+            double cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
+            if (f + 1 < c1) {
+                int id = base + (f + 1 - MID + S - 1) * st;
+                nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
+            }
+            split_state<FVS, DIR>(P, cur[0], cur[1], cur[2], cur[3], cur[4], fp[S - 1], fm[S - 1]);
         }
         double ff[5];
 #pragma unroll
@@ -171,6 +182,132 @@ __global__ __launch_bounds__(64) void k_sweep(DevParams P, Fields F, int seg_len
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) fprev[m] = ff[m];
+    }
+}
+
+// x sweep, one thread per cell, one block per full x-row (periodic x).
+// Threads split their cell (plus the S-1 halo cells of the row's stencil)
+// into LDS (SoA: conflict-free), then each thread reconstructs the right
+// face of its cell.  On a periodic axis the row's left boundary face is
+// bit-identical to its right boundary face (its stencil cells are the exact
+// periodic images), so the first cell takes it from the last thread and no
+// face is computed twice.  Loads and stores are coalesced along x.
+template <int RECON, int FVS>
+__global__ __launch_bounds__(1024) void k_sweep_xrow(DevParams P, Fields F)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    extern __shared__ double lds[];
+    const int TX = P.nx;
+    const int NSLOT = TX + S - 1;          // split slots: cells ng-MID .. ng+nx-2-MID+S
+    double *fpl = lds;                      // [5][NSLOT]
+    double *fml = lds + 5 * NSLOT;          // [5][NSLOT]
+    double *ffl = lds + 10 * NSLOT;         // [5][TX] face fluxes
+    const int t = threadIdx.x;
+    const int j = P.ng + blockIdx.x;
+    const int k = P.ng + blockIdx.y;
+    const int row = (k * P.sy + j) * P.sx;
+    const int i0 = P.ng - MID;              // cell of slot 0
+    for (int q = t; q < NSLOT; q += blockDim.x) {
+        const int id = row + i0 + q;
+        double Fp[5], Fm[5];
+        split_state<FVS, 0>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], Fp, Fm);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { fpl[m * NSLOT + q] = Fp[m]; fml[m * NSLOT + q] = Fm[m]; }
+    }
+    __syncthreads();
+    if (t < TX) {
+        // face ng+t (right face of cell ng+t): stencil cells ng+t-MID .. +S-1
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2) {
+                sp[s2] = fpl[m * NSLOT + t + s2];
+                sm[S - 1 - s2] = fml[m * NSLOT + t + s2];
+            }
+            double plus_face = recon<RECON>(P, sp);
+            double minus_face = recon<RECON>(P, sm);
+            ffl[m * TX + t] = plus_face + minus_face;
+        }
+    }
+    __syncthreads();
+    if (t < TX) {
+        const int tl = t == 0 ? TX - 1 : t - 1;
+        const int id = row + P.ng + t;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) F.A[m][id] = (ffl[m * TX + t] - ffl[m * TX + tl]) * P.idx;
+    }
+}
+
+// Gradients (ViscousFlux.cpp:240-279) fused with the viscous flux
+// (ViscousFlux.cpp:282-339): one thread per (i, j) column marching along z
+// with the R planes above and below held in registers, so the z stencil
+// reads each plane once; x and y neighbours come through the caches.
+// Writes only the 12 viscous-flux arrays.  Same arithmetic and order as
+// k_grad + k_vflux.
+template <int VIS, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, int seg_len)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int k0 = P.ng + blockIdx.z * seg_len;
+    const int k1 = min(k0 + seg_len, P.ng + P.nz);
+    const int sxy = P.sx * P.sy;
+    const int col = j * P.sx + i;
+    const double *src[4] = {F.W[1], F.W[2], F.W[3], F.W[5]};
+    double win[4][2 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
+    const double Pr = P.Pr, Cp = P.Cp;
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[q][o] = win[q][o + 1];
+            win[q][2 * R] = src[q][id + R * sxy];
+        }
+        double g[12];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *c = src[q] + id;
+            g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
+            g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
+            g[3 * q + 2] = diff_window<VIS>(&win[q][R], P.dz);
+        }
+        double mu = get_mu(P, win[3][R]);
+        double du_dx = g[0], du_dy = g[1], du_dz = g[2];
+        double dv_dx = g[3], dv_dy = g[4], dv_dz = g[5];
+        double dw_dx = g[6], dw_dy = g[7], dw_dz = g[8];
+        double dT_dx = g[9], dT_dy = g[10], dT_dz = g[11];
+        double divU = du_dx + dv_dy + dw_dz;
+        double tau_xx = 2.0 * mu * du_dx - (2.0 / 3.0) * mu * divU;
+        double tau_yy = 2.0 * mu * dv_dy - (2.0 / 3.0) * mu * divU;
+        double tau_zz = 2.0 * mu * dw_dz - (2.0 / 3.0) * mu * divU;
+        double tau_xy = mu * (du_dy + dv_dx);
+        double tau_xz = mu * (du_dz + dw_dx);
+        double tau_yz = mu * (dv_dz + dw_dy);
+        double qx = mu * Cp / Pr * dT_dx;
+        double qy = mu * Cp / Pr * dT_dy;
+        double qz = mu * Cp / Pr * dT_dz;
+        double u = win[0][R], v = win[1][R], w = win[2][R];
+        F.VF[0][id] = tau_xx;
+        F.VF[1][id] = tau_xy;
+        F.VF[2][id] = tau_xz;
+        F.VF[3][id] = u * tau_xx + v * tau_xy + w * tau_xz + qx;
+        F.VF[4][id] = tau_xy;
+        F.VF[5][id] = tau_yy;
+        F.VF[6][id] = tau_yz;
+        F.VF[7][id] = u * tau_xy + v * tau_yy + w * tau_yz + qy;
+        F.VF[8][id] = tau_xz;
+        F.VF[9][id] = tau_yz;
+        F.VF[10][id] = tau_zz;
+        F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
     }
 }
 
@@ -463,6 +600,105 @@ __global__ __launch_bounds__(256) void k_vdiv(DevParams P, Fields F, double dt,
     }
 }
 
+// k_vdiv with the z derivative of Gv taken from a register window of the
+// R planes above and below (one thread per (i, j) column marching along z;
+// periodic wrap in z as in k_vdiv).  Same arithmetic and order as k_vdiv.
+template <int VIS, int MODE>
+__global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt, int seg_len,
+                                                unsigned long long *flag, unsigned long long step_key)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int ng = P.ng;
+    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= ng + P.nx || j >= ng + P.ny) return;
+    const int k0 = ng + blockIdx.z * seg_len;
+    const int k1 = min(k0 + seg_len, ng + P.nz);
+    const int sxy = P.sx * P.sy;
+    const int col = j * P.sx + i;
+    int oxc[2 * R + 1], oyc[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; ++o) {
+        oxc[o + R] = j * P.sx + wrapi(i + o, ng, P.nx);
+        oyc[o + R] = wrapi(j + o, ng, P.ny) * P.sx + i;
+    }
+    double win[4][2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o)
+            win[m][o] = F.VF[8 + m][wrapi(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+        const int kn = wrapi(k + R, ng, P.nz) * sxy + col;
+        double r[5];
+        r[0] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
+            win[m][2 * R] = F.VF[8 + m][kn];
+            double wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+            for (int o = 0; o < 2 * R + 1; ++o) {
+                wx[o] = F.VF[m][k * sxy + oxc[o]];
+                wy[o] = F.VF[4 + m][k * sxy + oyc[o]];
+            }
+            double acc = 0.0;
+            acc += diff_window<VIS>(wx + R, P.dx);
+            acc += diff_window<VIS>(wy + R, P.dy);
+            acc += diff_window<VIS>(&win[m][R], P.dz);
+            r[m + 1] = acc;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) r[m] += -(F.A[m][id]);
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) F.A[m][id] = r[m];
+        } else {
+            const double gamma = P.gamma;
+            double rr = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
+            double U[5];
+            U[0] = rr;
+            U[1] = rr * u;
+            U[2] = rr * v;
+            U[3] = rr * w;
+            {
+                double kinetic = 0.5 * (u * u + v * v + w * w);
+                double e_internal = p / ((gamma - 1.0) * rr);
+                U[4] = rr * (e_internal + kinetic);
+            }
+            double Un[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                if constexpr (MODE == 1) {
+                    F.U0[m][id] = U[m];
+                    Un[m] = U[m] + dt * r[m];
+                } else if constexpr (MODE == 2) {
+                    Un[m] = 0.75 * F.U0[m][id] + 0.25 * (U[m] + dt * r[m]);
+                } else {
+                    Un[m] = (1.0 / 3.0) * F.U0[m][id] + (2.0 / 3.0) * (U[m] + dt * r[m]);
+                }
+            }
+            double rn = Un[0];
+            if (rn <= 0.0) rn = 1e-12;
+            double un = Un[1] / rn;
+            double vn = Un[2] / rn;
+            double wn = Un[3] / rn;
+            double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+            double e_internal = Un[4] / rn - kinetic;
+            double pn = (gamma - 1.0) * rn * e_internal;
+            double Tn = pn / (rn * P.Rgas);
+            F.W[0][id] = rn; F.W[1][id] = un; F.W[2][id] = vn; F.W[3][id] = wn; F.W[4][id] = pn;
+            F.W[5][id] = Tn;
+            if constexpr (MODE == 3) {
+                if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
+                    atomicMin(flag, step_key | (unsigned long long)id);
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------- reductions
 
 __device__ __forceinline__ double wave_min(double v)
@@ -594,38 +830,126 @@ hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F
 
 using SweepFn = void (*)(DevParams, Fields, int);
 
-template <int RECON, int FVS>
-static SweepFn pick_sweep(int dir, int acc)
+// x stores (ACC = 0), y and z accumulate (ACC = 1); MINW is the waves-per-SIMD
+// floor the register allocation must allow (2: no spills, 3: a few spills).
+template <int RECON, int FVS, int MINW>
+static SweepFn pick_sweep(int dir)
 {
-    if (dir == 0) return acc ? (SweepFn)k_sweep<0, RECON, FVS, 1> : (SweepFn)k_sweep<0, RECON, FVS, 0>;
-    if (dir == 1) return acc ? (SweepFn)k_sweep<1, RECON, FVS, 1> : (SweepFn)k_sweep<1, RECON, FVS, 0>;
-    return acc ? (SweepFn)k_sweep<2, RECON, FVS, 1> : (SweepFn)k_sweep<2, RECON, FVS, 0>;
+    if (dir == 0) return (SweepFn)k_sweep<0, RECON, FVS, 0, MINW>;
+    if (dir == 1) return (SweepFn)k_sweep<1, RECON, FVS, 1, MINW>;
+    return (SweepFn)k_sweep<2, RECON, FVS, 1, MINW>;
 }
 
-template <int FVS>
-static SweepFn pick_sweep_r(int recon, int dir, int acc)
+template <int FVS, int MINW>
+static SweepFn pick_sweep_r(int recon, int dir)
 {
     switch (recon) {
-    case R_WENO5: return pick_sweep<R_WENO5, FVS>(dir, acc);
-    case R_LINEAR: return pick_sweep<R_LINEAR, FVS>(dir, acc);
-    case R_MDCD: return pick_sweep<R_MDCD, FVS>(dir, acc);
-    case R_MDCD_HYBRID: return pick_sweep<R_MDCD_HYBRID, FVS>(dir, acc);
-    case R_UD7: return pick_sweep<R_UD7, FVS>(dir, acc);
-    case R_UD7L: return pick_sweep<R_UD7L, FVS>(dir, acc);
+    case R_WENO5: return pick_sweep<R_WENO5, FVS, MINW>(dir);
+    case R_LINEAR: return pick_sweep<R_LINEAR, FVS, MINW>(dir);
+    case R_MDCD: return pick_sweep<R_MDCD, FVS, MINW>(dir);
+    case R_MDCD_HYBRID: return pick_sweep<R_MDCD_HYBRID, FVS, MINW>(dir);
+    case R_UD7: return pick_sweep<R_UD7, FVS, MINW>(dir);
+    case R_UD7L: return pick_sweep<R_UD7L, FVS, MINW>(dir);
     default: return nullptr;
     }
 }
 
-hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int acc, int seg_len)
+hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int seg_len, int minw)
 {
-    SweepFn fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW>(P.recon, dir, acc)
-                                 : pick_sweep_r<FVS_LF>(P.recon, dir, acc);
+    SweepFn fn;
+    if (minw == 3) fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW, 3>(P.recon, dir) : pick_sweep_r<FVS_LF, 3>(P.recon, dir);
+    else fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW, 2>(P.recon, dir) : pick_sweep_r<FVS_LF, 2>(P.recon, dir);
     if (!fn) return hipErrorInvalidValue;
     const int na = dir == 0 ? P.ny : P.nx;
     const int nb = dir == 2 ? P.ny : P.nz;
     const int nd = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
     const int nseg = (nd + seg_len - 1) / seg_len;
     hipLaunchKernelGGL(fn, dim3((na + 63) / 64, nb, nseg), dim3(64), 0, s, P, F, seg_len);
+    return hipGetLastError();
+}
+
+using XrowFn = void (*)(DevParams, Fields);
+
+template <int FVS>
+static XrowFn pick_xrow(int recon)
+{
+    switch (recon) {
+    case R_WENO5: return k_sweep_xrow<R_WENO5, FVS>;
+    case R_LINEAR: return k_sweep_xrow<R_LINEAR, FVS>;
+    case R_MDCD: return k_sweep_xrow<R_MDCD, FVS>;
+    case R_MDCD_HYBRID: return k_sweep_xrow<R_MDCD_HYBRID, FVS>;
+    case R_UD7: return k_sweep_xrow<R_UD7, FVS>;
+    case R_UD7L: return k_sweep_xrow<R_UD7L, FVS>;
+    default: return nullptr;
+    }
+}
+
+static size_t xrow_lds_bytes(const DevParams &P)
+{
+    const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
+                  : P.recon == R_LINEAR ? 2 : 8;
+    return (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double);
+}
+
+// full-row blocks need a periodic x axis and a row that fits in 64 KiB of LDS
+bool xrow_supported(const DevParams &P)
+{
+    return P.periodic[0] && P.nx <= 1024 && xrow_lds_bytes(P) <= 65536;
+}
+
+hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F)
+{
+    XrowFn fn = P.fvs == FVS_SW ? pick_xrow<FVS_SW>(P.recon) : pick_xrow<FVS_LF>(P.recon);
+    if (!fn || !xrow_supported(P)) return hipErrorInvalidValue;
+    const size_t lds = xrow_lds_bytes(P);
+    const int threads = ((P.nx + 63) / 64) * 64;
+    hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb)
+{
+    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg_len - 1) / seg_len);
+    if (minb == 2) {
+        switch (P.vis) {
+        case VIS_C8: hipLaunchKernelGGL((k_gradflux<VIS_C8, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        case VIS_C6: hipLaunchKernelGGL((k_gradflux<VIS_C6, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        case VIS_C4: hipLaunchKernelGGL((k_gradflux<VIS_C4, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (P.vis) {
+        case VIS_C8: hipLaunchKernelGGL((k_gradflux<VIS_C8, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        case VIS_C6: hipLaunchKernelGGL((k_gradflux<VIS_C6, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        case VIS_C4: hipLaunchKernelGGL((k_gradflux<VIS_C4, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+template <int VIS>
+static void vdivz_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                       unsigned long long *flag, unsigned long long key)
+{
+    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_vdiv_z<VIS, 0>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 1: hipLaunchKernelGGL((k_vdiv_z<VIS, 1>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 2: hipLaunchKernelGGL((k_vdiv_z<VIS, 2>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    default: hipLaunchKernelGGL((k_vdiv_z<VIS, 3>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    }
+}
+
+hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                         unsigned long long *flag, unsigned long long key)
+{
+    switch (P.vis) {
+    case VIS_C8: vdivz_mode<VIS_C8>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C6: vdivz_mode<VIS_C6>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C4: vdivz_mode<VIS_C4>(s, P, F, mode, dt, seg, flag, key); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

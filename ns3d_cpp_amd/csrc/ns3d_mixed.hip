@@ -1,0 +1,638 @@
+// ns3d_mixed.hip -- mixed-precision RHS + RK3 path (NS3D_PREC_MIXED).
+//
+// State and arithmetic split:
+//   FP64  conserved state U and U0, the FVS split, the face fluxes and their
+//         differences, the RK3 stage update, and p/T recovery from E minus
+//         the kinetic energy (C2P).
+//   FP32  a shadow of the primitives stored as deviations from the
+//         nondimensional reference state (rho-1, u, v, w, p-p_inf, T-1;
+//         p_inf = 1/(gamma Ma^2)), the reconstructions (weights and smoothness
+//         indicators included), the gradients, viscous fluxes and their
+//         divergence, and the inviscid accumulator.
+// The FP32 reconstructions act on split-flux residuals F(s) - F(ref), with
+// the reference taken at the face's upwind cell and added back in FP64:
+// every reconstruction here is translation invariant (weights depend on
+// differences only, linear stencils sum to 1), so the low-Mach pressure
+// part of the fluxes never passes through FP32.
+#include "ns3d_device.hpp"
+#include "ns3d_internal.hpp"
+
+namespace ns3d {
+
+// ------------------------------------------------ FP32 reconstructions
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float recon32_weno5(const float *s)
+{
+    float f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    const float eps = 1e-6f;
+    float d0 = f0 - 2.0f * f1 + f2, e0 = f0 - 4.0f * f1 + 3.0f * f2;
+    float d1 = f1 - 2.0f * f2 + f3, e1 = f1 - f3;
+    float d2 = f2 - 2.0f * f3 + f4, e2 = 3.0f * f2 - 4.0f * f3 + f4;
+    float b0 = (13.0f / 12.0f) * d0 * d0 + 0.25f * e0 * e0;
+    float b1 = (13.0f / 12.0f) * d1 * d1 + 0.25f * e1 * e1;
+    float b2 = (13.0f / 12.0f) * d2 * d2 + 0.25f * e2 * e2;
+    float t0 = eps + b0, t1 = eps + b1, t2 = eps + b2;
+    float a0 = 0.1f * rcpf(t0 * t0);
+    float a1 = 0.6f * rcpf(t1 * t1);
+    float a2 = 0.3f * rcpf(t2 * t2);
+    float inv = rcpf(a0 + a1 + a2) * (1.0f / 6.0f);
+    float p0 = 2.0f * f0 - 7.0f * f1 + 11.0f * f2;
+    float p1 = -f1 + 5.0f * f2 + 2.0f * f3;
+    float p2 = 2.0f * f2 + 5.0f * f3 - f4;
+    return (a0 * p0 + a1 * p1 + a2 * p2) * inv;
+}
+
+__device__ __forceinline__ float recon32_mdcd(const float *s, float disp, float diss)
+{
+    return (0.5f * disp + 0.5f * diss) * s[0] +
+           (-1.5f * disp - 2.5f * diss - 1.0f / 12.0f) * s[1] +
+           (7.0f / 12.0f + disp + 5.0f * diss) * s[2] +
+           (7.0f / 12.0f + disp - 5.0f * diss) * s[3] +
+           (-1.5f * disp + 2.5f * diss - 1.0f / 12.0f) * s[4] +
+           (0.5f * disp - 0.5f * diss) * s[5];
+}
+
+__device__ __forceinline__ float recon32_ud7(const float *s)
+{
+    return (-1.0f / 140.0f) * s[0] + (5.0f / 84.0f) * s[1] - (101.0f / 420.0f) * s[2]
+           + (319.0f / 420.0f) * s[3] + (107.0f / 210.0f) * s[4]
+           - (19.0f / 210.0f) * s[5] + (1.0f / 105.0f) * s[6];
+}
+
+__device__ __forceinline__ float recon32_ud7l(const float *s, float alpha)
+{
+    float b = 1.0f - alpha;
+    float a0 = (-3.0f / 420.0f) * alpha - (3.0f / 840.0f) * b;
+    float a1 = (25.0f / 420.0f) * alpha + (29.0f / 840.0f) * b;
+    float a2 = (-101.0f / 420.0f) * alpha - (139.0f / 840.0f) * b;
+    float a3 = (319.0f / 420.0f) * alpha + (533.0f / 840.0f) * b;
+    float a4 = (214.0f / 420.0f) * alpha + (533.0f / 840.0f) * b;
+    float a5 = (-38.0f / 420.0f) * alpha - (139.0f / 840.0f) * b;
+    float a6 = (4.0f / 420.0f) * alpha + (29.0f / 840.0f) * b;
+    float a7 = (-3.0f / 840.0f) * b;
+    return a0 * s[0] + a1 * s[1] + a2 * s[2] + a3 * s[3] + a4 * s[4] + a5 * s[5] + a6 * s[6] + a7 * s[7];
+}
+
+template <int RECON>
+__device__ __forceinline__ float recon32(const DevParams &P, const float *s)
+{
+    if constexpr (RECON == R_WENO5) return recon32_weno5(s);
+    else if constexpr (RECON == R_LINEAR) return s[0];
+    else if constexpr (RECON == R_MDCD) return recon32_mdcd(s, (float)P.mdcd_disp, (float)P.mdcd_diss);
+    else if constexpr (RECON == R_MDCD_HYBRID) return 0.0f;   // never used: see face_mixed
+    else if constexpr (RECON == R_UD7) return recon32_ud7(s);
+    else return recon32_ud7l(s, (float)P.ud7l_alpha);
+}
+
+// Face flux of component m from FP64 split values of the S stencil cells:
+// residuals against the upwind cell of each side go through FP32.
+template <int RECON, int S>
+__device__ __forceinline__ double face_mixed(const DevParams &P, const double *fp, const double *fm)
+{
+    constexpr int MID = (S - 1) / 2;
+    if constexpr (RECON == R_MDCD_HYBRID) {
+        // its 6-point indicator beta3 cancels terms ~2e7 times larger than
+        // its value: input rounding to FP32 alone would leave O(1) relative
+        // error in it, so this option is reconstructed in FP64 throughout
+        double dm[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) dm[S - 1 - s] = fm[s];
+        return recon_mdcd_hybrid(fp, P.mdcd_disp, P.mdcd_diss) + recon_mdcd_hybrid(dm, P.mdcd_disp, P.mdcd_diss);
+    }
+    const double rp = fp[MID], rm = fm[MID + 1 < S ? MID + 1 : S - 1];
+    float sp[S], sm[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        sp[s] = (float)(fp[s] - rp);
+        sm[S - 1 - s] = (float)(fm[s] - rm);
+    }
+    return ((double)recon32<RECON>(P, sp) + rp) + ((double)recon32<RECON>(P, sm) + rm);
+}
+
+__device__ __forceinline__ void shadow_prims(const DevParams &P, const MFields &M, int id, double &rho, double &u,
+                                             double &v, double &w, double &p)
+{
+    rho = P.rho_ref + (double)M.S[0][id];
+    u = (double)M.S[1][id];
+    v = (double)M.S[2][id];
+    w = (double)M.S[3][id];
+    p = P.p_ref + (double)M.S[4][id];
+}
+
+// ---------------------------------------------------------------- sweeps
+
+// x sweep, one block per full periodic x-row (as k_sweep_xrow).
+template <int RECON, int FVS>
+__global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    extern __shared__ double lds[];
+    const int TX = P.nx;
+    const int NSLOT = TX + S - 1;
+    double *fpl = lds;
+    double *fml = lds + 5 * NSLOT;
+    double *ffl = lds + 10 * NSLOT;
+    const int t = threadIdx.x;
+    const int j = P.ng + blockIdx.x;
+    const int k = P.ng + blockIdx.y;
+    const int row = (k * P.sy + j) * P.sx;
+    const int i0 = P.ng - MID;
+    for (int q = t; q < NSLOT; q += blockDim.x) {
+        double rho, u, v, w, p, Fp[5], Fm[5];
+        shadow_prims(P, M, row + i0 + q, rho, u, v, w, p);
+        split_state<FVS, 0>(P, rho, u, v, w, p, Fp, Fm);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { fpl[m * NSLOT + q] = Fp[m]; fml[m * NSLOT + q] = Fm[m]; }
+    }
+    __syncthreads();
+    if (t < TX) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2) { sp[s2] = fpl[m * NSLOT + t + s2]; sm[s2] = fml[m * NSLOT + t + s2]; }
+            ffl[m * TX + t] = face_mixed<RECON, S>(P, sp, sm);
+        }
+    }
+    __syncthreads();
+    if (t < TX) {
+        const int tl = t == 0 ? TX - 1 : t - 1;
+        const int id = row + P.ng + t;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) M.A[m][id] = (float)((ffl[m * TX + t] - ffl[m * TX + tl]) * P.idx);
+    }
+}
+
+// y / z sweeps: column marching as k_sweep, FP64 split window, FP32
+// reconstruction of residuals; accumulates into the FP32 accumulator.
+template <int DIR, int RECON, int FVS>
+__global__ __launch_bounds__(64, 2) void km_sweep(DevParams P, MFields M, int seg_len)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.nx) return;
+    const int b = blockIdx.y;
+    const int nd = DIR == 1 ? P.ny : P.nz;
+    const int c0 = P.ng + blockIdx.z * seg_len;
+    const int c1 = min(c0 + seg_len, P.ng + nd);
+    if (c0 >= c1) return;
+    int base, st;
+    if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + a); st = P.sx; }
+    else { base = (P.ng + b) * P.sx + (P.ng + a); st = P.sx * P.sy; }
+    const double inv_d = DIR == 1 ? P.idy : P.idz;
+    double fp[S][5], fm[S][5];
+#pragma unroll
+    for (int s = 1; s < S; ++s) {
+        double rho, u, v, w, p;
+        shadow_prims(P, M, base + (c0 - 2 - MID + s) * st, rho, u, v, w, p);
+        split_state<FVS, DIR>(P, rho, u, v, w, p, fp[s], fm[s]);
+    }
+    float nx_[5];
+    {
+        const int id = base + (c0 - 1 - MID + S - 1) * st;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+    }
+    double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int f = c0 - 1; f < c1; ++f) {
+#pragma unroll
+        for (int s = 0; s < S - 1; ++s)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) { fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m]; }
+        {
+            float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
+            if (f + 1 < c1) {
+                const int id = base + (f + 1 - MID + S - 1) * st;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+            }
+            split_state<FVS, DIR>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
+                                  P.p_ref + (double)cur[4], fp[S - 1], fm[S - 1]);
+        }
+        double ff[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sp[S], sm[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[s] = fm[s][m]; }
+            ff[m] = face_mixed<RECON, S>(P, sp, sm);
+        }
+        if (f >= c0) {
+            const int idc = base + f * st;
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                M.A[m][idc] = (float)((double)M.A[m][idc] + (ff[m] - fprev[m]) * inv_d);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fprev[m] = ff[m];
+    }
+}
+
+// --------------------------------------------------------------- viscous
+
+__device__ __forceinline__ float get_mu32(const DevParams &P, float T)
+{
+    if (P.mu_law == 1) return (float)P.mu;
+    if (P.mu_law == 2) return (float)P.mu * powf(T, (float)P.mu_power);
+    const float S_ref = (float)P.sutherland_s;
+    return (float)P.mu * (T * sqrtf(T)) * (1.0f + S_ref) / (T + S_ref);
+}
+
+template <int VIS>
+__device__ __forceinline__ float diff32(const float *v, float h_inv)
+{
+    if constexpr (VIS == VIS_C8) {
+        return (0.8f * (v[1] - v[-1]) - 0.2f * (v[2] - v[-2]) + (4.0f / 105.0f) * (v[3] - v[-3]) -
+                (1.0f / 280.0f) * (v[4] - v[-4])) * h_inv;
+    } else if constexpr (VIS == VIS_C6) {
+        return (v[3] - 9.0f * v[2] + 45.0f * v[1] - 45.0f * v[-1] + 9.0f * v[-2] - v[-3]) * (h_inv / 60.0f);
+    } else {
+        return (-v[2] + 8.0f * v[1] - 8.0f * v[-1] + v[-2]) * (h_inv / 12.0f);
+    }
+}
+
+// gradients + viscous flux in FP32 from the shadow (u, v, w, T-1),
+// z-marching register window as k_gradflux.
+template <int VIS>
+__global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int k0 = P.ng + blockIdx.z * seg_len;
+    const int k1 = min(k0 + seg_len, P.ng + P.nz);
+    const int sxy = P.sx * P.sy;
+    const int col = j * P.sx + i;
+    const float *src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
+    const float idx = (float)P.idx, idy = (float)P.idy, idz = (float)P.idz;
+    float win[4][2 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
+    const float cq = (float)(P.Cp / P.Pr);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+        float g[12];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[q][o] = win[q][o + 1];
+            win[q][2 * R] = src[q][id + R * sxy];
+            float wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+            for (int o = -R; o <= R; ++o) {
+                wx[o + R] = src[q][id + o];
+                wy[o + R] = src[q][id + o * P.sx];
+            }
+            g[3 * q + 0] = diff32<VIS>(wx + R, idx);
+            g[3 * q + 1] = diff32<VIS>(wy + R, idy);
+            g[3 * q + 2] = diff32<VIS>(&win[q][R], idz);
+        }
+        const float mu = get_mu32(P, 1.0f + win[3][R]);
+        const float divU = g[0] + g[4] + g[8];
+        const float two3 = 2.0f / 3.0f;
+        const float txx = 2.0f * mu * g[0] - two3 * mu * divU;
+        const float tyy = 2.0f * mu * g[4] - two3 * mu * divU;
+        const float tzz = 2.0f * mu * g[8] - two3 * mu * divU;
+        const float txy = mu * (g[1] + g[3]);
+        const float txz = mu * (g[2] + g[6]);
+        const float tyz = mu * (g[5] + g[7]);
+        const float mk = mu * cq;
+        const float u = win[0][R], v = win[1][R], w = win[2][R];
+        M.VF[0][id] = txx;
+        M.VF[1][id] = txy;
+        M.VF[2][id] = txz;
+        M.VF[3][id] = u * txx + v * txy + w * txz + mk * g[9];
+        M.VF[4][id] = txy;
+        M.VF[5][id] = tyy;
+        M.VF[6][id] = tyz;
+        M.VF[7][id] = u * txy + v * tyy + w * tyz + mk * g[10];
+        M.VF[8][id] = txz;
+        M.VF[9][id] = tyz;
+        M.VF[10][id] = tzz;
+        M.VF[11][id] = u * txz + v * tyz + w * tzz + mk * g[11];
+    }
+}
+
+// divergence of the FP32 viscous fluxes (periodic wrap), the RHS in FP64,
+// and the stage update + C2P in FP64 (MODE 1..3) or the RHS into `rhs_out`
+// (MODE 0).
+template <int VIS, int MODE>
+__global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
+                                                 unsigned long long *flag, unsigned long long step_key)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int ng = P.ng;
+    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= ng + P.nx || j >= ng + P.ny) return;
+    const int k0 = ng + blockIdx.z * seg_len;
+    const int k1 = min(k0 + seg_len, ng + P.nz);
+    const int sxy = P.sx * P.sy;
+    const int col = j * P.sx + i;
+    int oxc[2 * R + 1], oyc[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; ++o) {
+        oxc[o + R] = j * P.sx + wrapi_m(i + o, ng, P.nx);
+        oyc[o + R] = wrapi_m(j + o, ng, P.ny) * P.sx + i;
+    }
+    const float idx = (float)P.idx, idy = (float)P.idy, idz = (float)P.idz;
+    float win[4][2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o)
+            win[m][o] = M.VF[8 + m][wrapi_m(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+        const int kn = wrapi_m(k + R, ng, P.nz) * sxy + col;
+        double r[5];
+        r[0] = -(double)M.A[0][id];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
+            win[m][2 * R] = M.VF[8 + m][kn];
+            float wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+            for (int o = 0; o < 2 * R + 1; ++o) {
+                wx[o] = M.VF[m][k * sxy + oxc[o]];
+                wy[o] = M.VF[4 + m][k * sxy + oyc[o]];
+            }
+            const float vd = diff32<VIS>(wx + R, idx) + diff32<VIS>(wy + R, idy) + diff32<VIS>(&win[m][R], idz);
+            r[m + 1] = (double)vd - (double)M.A[m + 1][id];
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) M.out[m][id] = r[m];
+        } else {
+            const double gamma = P.gamma;
+            double Un[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double U = M.U[m][id];
+                if constexpr (MODE == 1) {
+                    M.U0[m][id] = U;
+                    Un[m] = U + dt * r[m];
+                } else if constexpr (MODE == 2) {
+                    Un[m] = 0.75 * M.U0[m][id] + 0.25 * (U + dt * r[m]);
+                } else {
+                    Un[m] = (1.0 / 3.0) * M.U0[m][id] + (2.0 / 3.0) * (U + dt * r[m]);
+                }
+            }
+            double rn = Un[0];
+            if (rn <= 0.0) rn = 1e-12;
+            const double un = Un[1] / rn, vn = Un[2] / rn, wn = Un[3] / rn;
+            const double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+            const double pn = (gamma - 1.0) * rn * (Un[4] / rn - kinetic);
+            const double Tn = pn / (rn * P.Rgas);
+            M.U[0][id] = rn;
+#pragma unroll
+            for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
+            M.S[0][id] = (float)(rn - P.rho_ref);
+            M.S[1][id] = (float)un;
+            M.S[2][id] = (float)vn;
+            M.S[3][id] = (float)wn;
+            M.S[4][id] = (float)(pn - P.p_ref);
+            M.S[5][id] = (float)(Tn - 1.0);
+            if constexpr (MODE == 3) {
+                if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
+                    atomicMin(flag, step_key | (unsigned long long)id);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ state conversion
+
+// primitives (src[0..4], FP64, every cell) -> conserved U (FP64, P2C of
+// field_structures.h:430-451) and the FP32 deviation shadow
+__global__ void km_from_prims(DevParams P, MFields M, const double *const *src, int ncell, int write_u)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    const double gamma = P.gamma;
+    double rr = src[0][id];
+    if (rr <= 0.0) rr = 1e-12;
+    const double u = src[1][id], v = src[2][id], w = src[3][id], p = src[4][id];
+    if (write_u) {
+        M.U[0][id] = rr;
+        M.U[1][id] = rr * u;
+        M.U[2][id] = rr * v;
+        M.U[3][id] = rr * w;
+        const double kinetic = 0.5 * (u * u + v * v + w * w);
+        M.U[4][id] = rr * (p / ((gamma - 1.0) * rr) + kinetic);
+    }
+    const double T = p / (rr * P.Rgas);
+    M.S[0][id] = (float)(rr - P.rho_ref);
+    M.S[1][id] = (float)u;
+    M.S[2][id] = (float)v;
+    M.S[3][id] = (float)w;
+    M.S[4][id] = (float)(p - P.p_ref);
+    M.S[5][id] = (float)(T - 1.0);
+}
+
+// conserved U (FP64) -> primitives rho,u,v,w,p,T into dst[0..5] (C2P)
+__global__ void km_to_prims(DevParams P, MFields M, double *const *dst, int ncell)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    double rr = M.U[0][id];
+    if (rr <= 0.0) rr = 1e-12;
+    const double u = M.U[1][id] / rr, v = M.U[2][id] / rr, w = M.U[3][id] / rr;
+    const double kinetic = 0.5 * (u * u + v * v + w * w);
+    const double p = (P.gamma - 1.0) * rr * (M.U[4][id] / rr - kinetic);
+    dst[0][id] = rr; dst[1][id] = u; dst[2][id] = v; dst[3][id] = w; dst[4][id] = p;
+    dst[5][id] = p / (rr * P.Rgas);
+}
+
+// periodic ghost fill of the six FP32 shadow arrays (faces only)
+__global__ void km_fill_periodic(DevParams P, MFields M)
+{
+    const int axis = blockIdx.y;
+    if (!P.periodic[axis]) return;
+    const int ng = P.ng;
+    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const int na = axis == 0 ? P.ny : P.nx;
+    const int nb = axis == 2 ? P.ny : P.nz;
+    const int total = 2 * ng * na * nb;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int g = t % (2 * ng);
+    const int rest = t / (2 * ng);
+    const int a = ng + rest % na;
+    const int b = ng + rest / na;
+    int q, qs;
+    if (g < ng) { q = g; qs = g + nd; }
+    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
+    int dst, src;
+    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
+    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
+    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) M.S[m][dst] = M.S[m][src];
+}
+
+// CFL minimum (timestep.cpp:5-33) from the shadow
+__global__ __launch_bounds__(256) void km_cfl(DevParams P, MFields M, unsigned long long *out)
+{
+    __shared__ double red[4];
+    const int nxy = P.nx * P.ny;
+    const int total = nxy * P.nz;
+    double m = 1e9;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int k = t / nxy;
+        const int r = t - k * nxy;
+        const int j = r / P.nx;
+        const int i = r - j * P.nx;
+        const int id = ((P.ng + k) * P.sy + (P.ng + j)) * P.sx + (P.ng + i);
+        double rho, u, v, w, p;
+        shadow_prims(P, M, id, rho, u, v, w, p);
+        const double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
+        m = fmin(m, fmin(P.dx / (fabs(u) + a), fmin(P.dy / (fabs(v) + a), P.dz / (fabs(w) + a))));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        atomicMin(out, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// ------------------------------------------------------------- launches
+
+hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields &M, const double *const *src,
+                               int write_u)
+{
+    const int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(km_from_prims, dim3((n + 255) / 256), dim3(256), 0, s, P, M, src, n, write_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_to_prims(hipStream_t s, const DevParams &P, const MFields &M, double *const *dst)
+{
+    const int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(km_to_prims, dim3((n + 255) / 256), dim3(256), 0, s, P, M, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields &M)
+{
+    int m = P.ny * P.nz;
+    if (P.nx * P.nz > m) m = P.nx * P.nz;
+    if (P.nx * P.ny > m) m = P.nx * P.ny;
+    const int total = 2 * P.ng * m;
+    hipLaunchKernelGGL(km_fill_periodic, dim3((total + 255) / 256, 3), dim3(256), 0, s, P, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out)
+{
+    const long total = (long)P.nx * P.ny * P.nz;
+    long blocks = (total + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(km_cfl, dim3((unsigned)blocks), dim3(256), 0, s, P, M, out);
+    return hipGetLastError();
+}
+
+using MXrowFn = void (*)(DevParams, MFields);
+using MSweepFn = void (*)(DevParams, MFields, int);
+
+template <int FVS>
+static MXrowFn m_pick_xrow(int recon)
+{
+    switch (recon) {
+    case R_WENO5: return km_sweep_xrow<R_WENO5, FVS>;
+    case R_LINEAR: return km_sweep_xrow<R_LINEAR, FVS>;
+    case R_MDCD: return km_sweep_xrow<R_MDCD, FVS>;
+    case R_MDCD_HYBRID: return km_sweep_xrow<R_MDCD_HYBRID, FVS>;
+    case R_UD7: return km_sweep_xrow<R_UD7, FVS>;
+    case R_UD7L: return km_sweep_xrow<R_UD7L, FVS>;
+    default: return nullptr;
+    }
+}
+
+template <int DIR, int FVS>
+static MSweepFn m_pick_sweep(int recon)
+{
+    switch (recon) {
+    case R_WENO5: return km_sweep<DIR, R_WENO5, FVS>;
+    case R_LINEAR: return km_sweep<DIR, R_LINEAR, FVS>;
+    case R_MDCD: return km_sweep<DIR, R_MDCD, FVS>;
+    case R_MDCD_HYBRID: return km_sweep<DIR, R_MDCD_HYBRID, FVS>;
+    case R_UD7: return km_sweep<DIR, R_UD7, FVS>;
+    case R_UD7L: return km_sweep<DIR, R_UD7L, FVS>;
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len)
+{
+    if (dir == 0) {
+        MXrowFn fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW>(P.recon) : m_pick_xrow<FVS_LF>(P.recon);
+        if (!fn || !xrow_supported(P)) return hipErrorInvalidValue;
+        const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
+                      : P.recon == R_LINEAR ? 2 : 8;
+        const size_t lds = (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double);
+        const int threads = ((P.nx + 63) / 64) * 64;
+        hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, M);
+        return hipGetLastError();
+    }
+    MSweepFn fn;
+    if (dir == 1) fn = P.fvs == FVS_SW ? m_pick_sweep<1, FVS_SW>(P.recon) : m_pick_sweep<1, FVS_LF>(P.recon);
+    else fn = P.fvs == FVS_SW ? m_pick_sweep<2, FVS_SW>(P.recon) : m_pick_sweep<2, FVS_LF>(P.recon);
+    if (!fn) return hipErrorInvalidValue;
+    const int nb = dir == 2 ? P.ny : P.nz;
+    const int nd = dir == 1 ? P.ny : P.nz;
+    hipLaunchKernelGGL(fn, dim3((P.nx + 63) / 64, nb, (nd + seg_len - 1) / seg_len), dim3(64), 0, s, P, M, seg_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len)
+{
+    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg_len - 1) / seg_len);
+    dim3 b(64, 4, 1);
+    switch (P.vis) {
+    case VIS_C8: hipLaunchKernelGGL(km_gradflux<VIS_C8>, g, b, 0, s, P, M, seg_len); break;
+    case VIS_C6: hipLaunchKernelGGL(km_gradflux<VIS_C6>, g, b, 0, s, P, M, seg_len); break;
+    case VIS_C4: hipLaunchKernelGGL(km_gradflux<VIS_C4>, g, b, 0, s, P, M, seg_len); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int VIS>
+static void m_vdiv_mode(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                        unsigned long long *flag, unsigned long long key)
+{
+    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
+    dim3 b(64, 4, 1);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_z<VIS, 2>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    default: hipLaunchKernelGGL((km_vdiv_z<VIS, 3>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    }
+}
+
+hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                         unsigned long long *flag, unsigned long long key)
+{
+    switch (P.vis) {
+    case VIS_C8: m_vdiv_mode<VIS_C8>(s, P, M, mode, dt, seg, flag, key); break;
+    case VIS_C6: m_vdiv_mode<VIS_C6>(s, P, M, mode, dt, seg, flag, key); break;
+    case VIS_C4: m_vdiv_mode<VIS_C4>(s, P, M, mode, dt, seg, flag, key); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ns3d

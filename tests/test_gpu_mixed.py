@@ -1,0 +1,120 @@
+"""Mixed-precision path (FP32 reconstruction / gradients over an FP64 state)
+against the FP64 path on the GPU and against the oracle.
+
+Tolerances follow from the design: the FP32 shadow stores deviations from
+the reference state (rho-1, p-p_inf, T-1), so the state entering each RHS
+carries ~1e-7 relative rounding; per-step RHS errors stay near 1e-6 of the
+RHS scale and the accumulated error after tens of steps is ~1e-6 of U.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ns3d_cpp_amd as ns  # noqa: E402
+from ns3d_cpp_amd import fields  # noqa: E402
+from oracle import Oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2)) / (np.sqrt(np.sum(b * b)) + 1e-300))
+
+
+def solvers(n, state, **cfg):
+    out = []
+    for prec in ("fp64", "mixed"):
+        p, g = ns.make_params(n, precision=prec, **cfg)
+        s = ns.Solver(p, g)
+        s.upload_primitive(*state)
+        out.append(s)
+    return out
+
+
+CFG = dict(Re=200.0, Pr=0.72, recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th",
+           sutherland_s=110.4 / 273.15, cfl=0.5)
+
+
+def test_mixed_upload_download(gpu):
+    n = 16
+    st, Ma = fields.hit(n, k0=2, Mt=0.3, seed=5)
+    d, m = solvers(n, st, Ma=Ma, **CFG)
+    pd, pm = d.primitive(), m.primitive()
+    for k in ("rho", "u", "v", "w", "p"):
+        assert rel_l2(m.interior(pm[k]), d.interior(pd[k])) < 1e-14, k
+    for a, b in zip(m.conserved(), d.conserved()):
+        assert rel_l2(m.interior(a), d.interior(b)) < 1e-14
+
+
+def test_mixed_free_stream_is_exact(gpu):
+    n = 16
+    st = [np.full((n, n, n), v) for v in (1.1, 0.3, -0.2, 0.25, 30.0)]
+    p, g = ns.make_params(n, precision="mixed", Ma=0.17, **CFG)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    for r in s.compute_rhs():
+        assert np.all(s.interior(r) == 0.0)
+
+
+@pytest.mark.parametrize("recon,fvs", [("WENO5", "StagerWarming"), ("MDCD", "LaxFriedrichs"),
+                                       ("UPWIND_7TH", "StagerWarming"), ("UD7L", "LaxFriedrichs"),
+                                       ("MDCD_HYBRID", "StagerWarming"), ("LINEAR", "LaxFriedrichs")])
+def test_mixed_rhs_vs_oracle(gpu, recon, fvs):
+    """One RHS of a 32^3 isotropic-turbulence state: per-component relative
+    max error against the FP64 oracle <= 2e-5 of the component's scale."""
+    n = 32
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=9)
+    cfg = dict(CFG, recon=recon, fvs_type=fvs)
+    p, g = ns.make_params(n, precision="mixed", Ma=Ma, **cfg)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    pr, gr = ns.make_params(n, Ma=Ma, **cfg)
+    o = Oracle(pr, gr)
+    o.set_primitive(*st)
+    for m, (a, b) in enumerate(zip(s.compute_rhs(), o.compute_rhs())):
+        a, b = s.interior(a), o.interior(b)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err < 2e-5, (m, err)
+
+
+def test_mixed_vs_fp64_trajectory(gpu):
+    """64^3 isotropic turbulence, 50 RK3 steps with the FP64 path's CFL dt:
+    rel-L2 of every conserved component <= 1e-5."""
+    n = 64
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=21)
+    d, m = solvers(n, st, Ma=Ma, **CFG)
+    for _ in range(50):
+        dt = d.timestep()
+        d.rk3_step(dt)
+        m.rk3_step(dt)
+    errs = [rel_l2(m.interior(a), d.interior(b)) for a, b in zip(m.conserved(), d.conserved())]
+    assert max(errs) < 1e-5, errs
+
+
+def test_mixed_conservation_full_size(gpu):
+    """256^3: the FP64 face-flux differences telescope; the FP32 accumulator
+    adds rounding only, so sum(RHS) stays ~1e-6 of sum|RHS|."""
+    n = 256
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=42)
+    p, g = ns.make_params(n, precision="mixed", Ma=Ma, **CFG)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    for m, r in enumerate(s.compute_rhs()):
+        r = s.interior(r)
+        assert abs(r.sum()) <= 1e-6 * np.abs(r).sum(), m
+
+
+def test_mixed_rejects_fp64_only_entry_points(gpu):
+    n = 16
+    st, Ma = fields.hit(n, k0=2, Mt=0.3, seed=1)
+    p, g = ns.make_params(n, precision="mixed", Ma=Ma, **CFG)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    with pytest.raises(ns.NS3DError) as e:
+        s.du_inviscous()
+    assert e.value.status == ns._lib.NS3D_ERR_UNSUPPORTED
