@@ -33,6 +33,29 @@ template <> struct Stencil<R_MDCD_HYBRID> { static constexpr int S = 6; };
 template <> struct Stencil<R_UD7> { static constexpr int S = 8; };
 template <> struct Stencil<R_UD7L> { static constexpr int S = 8; };
 
+// --------------------------------------------------------- XCD mapping
+// Workgroups are dealt round-robin to the 8 XCDs of the device, each with
+// its own L2.  Remap the hardware workgroup id so that XCD x receives the
+// contiguous logical range [x*n/8, (x+1)*n/8): tiles that share halos then
+// share an L2.  Falls back to identity when n is not a multiple of 8.
+__device__ __forceinline__ int xcd_logical(int wg, int nwg)
+{
+    if (nwg & 7) return wg;
+    return (wg & 7) * (nwg >> 3) + (wg >> 3);
+}
+
+// logical (x, y, z) block coordinates of a 3D grid after the XCD remap
+__device__ __forceinline__ void xcd_block3(int &bx, int &by, int &bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nwg = gx * gy * gridDim.z;
+    const int wg = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int l = xcd_logical(wg, nwg);
+    bx = l % gx;
+    by = (l / gx) % gy;
+    bz = l / (gx * gy);
+}
+
 // ----------------------------------------------------------- conversions
 
 // E and c of primitiveToConserved (field_structures.h:445-449) from a

@@ -115,12 +115,16 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
+    // grid (nseg, lane tiles, b): after the XCD remap the segments of one
+    // column, whose halos overlap, run on the same XCD
+    int bs, bt, bb;
+    xcd_block3(bs, bt, bb);
     const int na = DIR == 0 ? P.ny : P.nx;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = bt * blockDim.x + threadIdx.x;
     if (a >= na) return;
-    const int b = blockIdx.y;
+    const int b = bb;
     const int nd = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
-    const int c0 = P.ng + blockIdx.z * seg_len;
+    const int c0 = P.ng + bs * seg_len;
     const int c1 = min(c0 + seg_len, P.ng + nd);
     if (c0 >= c1) return;
     int base, st;
@@ -250,10 +254,12 @@ template <int VIS, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, int seg_len)
 {
     constexpr int R = VisReach<VIS>::R;
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + by * blockDim.y + threadIdx.y;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
-    const int k0 = P.ng + blockIdx.z * seg_len;
+    const int k0 = P.ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, P.ng + P.nz);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
@@ -609,10 +615,12 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
 {
     constexpr int R = VisReach<VIS>::R;
     const int ng = P.ng;
-    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = ng + bx * blockDim.x + threadIdx.x;
+    const int j = ng + by * blockDim.y + threadIdx.y;
     if (i >= ng + P.nx || j >= ng + P.ny) return;
-    const int k0 = ng + blockIdx.z * seg_len;
+    const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
@@ -864,7 +872,7 @@ hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int 
     const int nb = dir == 2 ? P.ny : P.nz;
     const int nd = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
     const int nseg = (nd + seg_len - 1) / seg_len;
-    hipLaunchKernelGGL(fn, dim3((na + 63) / 64, nb, nseg), dim3(64), 0, s, P, F, seg_len);
+    hipLaunchKernelGGL(fn, dim3(nseg, (na + 63) / 64, nb), dim3(64), 0, s, P, F, seg_len);
     return hipGetLastError();
 }
 

@@ -173,11 +173,13 @@ __global__ __launch_bounds__(64, 2) void km_sweep(DevParams P, MFields M, int se
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    int bs, bt, bb;
+    xcd_block3(bs, bt, bb);   // grid (nseg, lane tiles, b), as km_sweep's launch
+    const int a = bt * blockDim.x + threadIdx.x;
     if (a >= P.nx) return;
-    const int b = blockIdx.y;
+    const int b = bb;
     const int nd = DIR == 1 ? P.ny : P.nz;
-    const int c0 = P.ng + blockIdx.z * seg_len;
+    const int c0 = P.ng + bs * seg_len;
     const int c1 = min(c0 + seg_len, P.ng + nd);
     if (c0 >= c1) return;
     int base, st;
@@ -261,10 +263,12 @@ template <int VIS>
 __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
 {
     constexpr int R = VisReach<VIS>::R;
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + by * blockDim.y + threadIdx.y;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
-    const int k0 = P.ng + blockIdx.z * seg_len;
+    const int k0 = P.ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, P.ng + P.nz);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
@@ -329,10 +333,12 @@ __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double 
 {
     constexpr int R = VisReach<VIS>::R;
     const int ng = P.ng;
-    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = ng + bx * blockDim.x + threadIdx.x;
+    const int j = ng + by * blockDim.y + threadIdx.y;
     if (i >= ng + P.nx || j >= ng + P.ny) return;
-    const int k0 = ng + blockIdx.z * seg_len;
+    const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
@@ -592,7 +598,7 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
     if (!fn) return hipErrorInvalidValue;
     const int nb = dir == 2 ? P.ny : P.nz;
     const int nd = dir == 1 ? P.ny : P.nz;
-    hipLaunchKernelGGL(fn, dim3((P.nx + 63) / 64, nb, (nd + seg_len - 1) / seg_len), dim3(64), 0, s, P, M, seg_len);
+    hipLaunchKernelGGL(fn, dim3((nd + seg_len - 1) / seg_len, (P.nx + 63) / 64, nb), dim3(64), 0, s, P, M, seg_len);
     return hipGetLastError();
 }
 
