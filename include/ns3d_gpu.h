@@ -187,6 +187,39 @@ int ns3d_profile_entry(ns3d_ctx *ctx, int idx, char *name, int name_len,
  * Overwrites the stored gradient / viscous-flux scratch. */
 int ns3d_copy_probe(ns3d_ctx *ctx, int reps, double *gbps);
 
+/* ---- Differentiation (no counterpart in the reference, which has no
+ * derivative code).  FP64 contexts on a periodic box with a component-wise
+ * WENO5 / linear / MDCD / UD7 / UD7L reconstruction and every grid extent
+ * >= 8; other settings return NS3D_ERR_UNSUPPORTED.  Host arrays are full
+ * sx*sy*sz layouts; only interior cells are read or written.  |x| in the
+ * plain SW/LF splits is differentiated as sign(x); sw_eps > 0 makes the
+ * split smooth. */
+
+/* Tangent-linear RHS at the current state: out = dR/dU * dU. */
+int ns3d_tlm_rhs(ns3d_ctx *ctx, const double *const dU[5], double *const out[5]);
+/* Adjoint RHS at the current state: out = (dR/dU)^T lam + w_obj * d(sum of
+ * Phi over interior cells)/dU, with Phi = tau_ij du_i/dx_j the viscous
+ * dissipation function; *dtheta (may be NULL) = d/d(1/Re) of
+ * lam . R + w_obj * sum Phi. */
+int ns3d_adj_rhs(ns3d_ctx *ctx, const double *const lam[5], double w_obj,
+                 double *const out[5], double *dtheta);
+/* Volume-mean dissipation eps = mean over interior cells of Phi (same
+ * gradient scheme and mu(T) as the viscous flux). */
+int ns3d_dissipation(ns3d_ctx *ctx, double *eps);
+/* Run nsteps RK3 steps of fixed dt from the current state, storing a
+ * checkpoint every ckpt_every steps, and return
+ *     J = sum_{n=0}^{nsteps-1} dt * eps(U^n).
+ * The context then holds U^nsteps. */
+int ns3d_adj_forward(ns3d_ctx *ctx, int nsteps, double dt, int ckpt_every, double *J);
+/* Reverse pass of the last ns3d_adj_forward (checkpoint and recompute):
+ * lam0 = dJ/dU^0 + (dU^N/dU^0)^T lamN (lamN may be NULL for zero), conserved
+ * variables, and *dJ_dinvRe = the derivative with respect to 1/Re (may be
+ * NULL).  The context is left holding U^nsteps again. */
+int ns3d_adj_backward(ns3d_ctx *ctx, const double *const lamN[5],
+                      double *const lam0[5], double *dJ_dinvRe);
+/* Device bytes held for differentiation (adjoint arrays + checkpoints). */
+int ns3d_adj_memory(const ns3d_ctx *ctx, long long *bytes);
+
 #ifdef __cplusplus
 }
 #endif

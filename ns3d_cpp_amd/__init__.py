@@ -223,3 +223,54 @@ class Solver:
             self._check(self._L.ns3d_profile_entry(self.ctx, k, buf, 64, C.byref(ms), C.byref(cnt)), "profile_entry")
             out[buf.value.decode()] = (ms.value, cnt.value)
         return out
+
+    # -- differentiation -----------------------------------------------------
+    def _set5(self, arrs):
+        full = [self._full(a) for a in arrs]
+        return full, (_lib.DP * 5)(*[_ptr(a) for a in full])
+
+    def _out5(self):
+        out = [np.zeros(self.shape) for _ in range(5)]
+        return out, (_lib.DP * 5)(*[_ptr(a) for a in out])
+
+    def tlm_rhs(self, dU):
+        """dR/dU * dU at the current state (5 conserved arrays)."""
+        keep, src = self._set5(dU)
+        out, dst = self._out5()
+        self._check(self._L.ns3d_tlm_rhs(self.ctx, src, dst), "tlm_rhs")
+        return out
+
+    def adj_rhs(self, lam, w_obj=0.0):
+        """((dR/dU)^T lam + w_obj d(sum Phi)/dU, d/d(1/Re) of lam.R + w_obj sum Phi)."""
+        keep, src = self._set5(lam)
+        out, dst = self._out5()
+        th = C.c_double()
+        self._check(self._L.ns3d_adj_rhs(self.ctx, src, float(w_obj), dst, C.byref(th)), "adj_rhs")
+        return out, th.value
+
+    def dissipation(self):
+        """Volume-mean viscous dissipation of the current state."""
+        e = C.c_double()
+        self._check(self._L.ns3d_dissipation(self.ctx, C.byref(e)), "dissipation")
+        return e.value
+
+    def adj_forward(self, nsteps, dt, ckpt_every=None):
+        """J = sum_{n<nsteps} dt * dissipation(U^n); stores checkpoints."""
+        J = C.c_double()
+        ck = int(ckpt_every) if ckpt_every else int(nsteps)
+        self._check(self._L.ns3d_adj_forward(self.ctx, int(nsteps), float(dt), ck, C.byref(J)), "adj_forward")
+        return J.value
+
+    def adj_backward(self, lamN=None):
+        """(dJ/dU^0 as 5 conserved arrays, dJ/d(1/Re)) of the last adj_forward."""
+        keep, src = self._set5(lamN) if lamN is not None else (None, None)
+        out, dst = self._out5()
+        g = C.c_double()
+        srcp = C.cast(src, C.c_void_p) if src is not None else None
+        self._check(self._L.ns3d_adj_backward(self.ctx, srcp, dst, C.byref(g)), "adj_backward")
+        return out, g.value
+
+    def adj_memory(self):
+        b = C.c_longlong()
+        self._check(self._L.ns3d_adj_memory(self.ctx, C.byref(b)), "adj_memory")
+        return b.value

@@ -87,6 +87,12 @@ SIGNATURES = {
     "ns3d_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "ns3d_profile_count": (C.c_int, [C.c_void_p]),
     "ns3d_profile_entry": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_longlong)]),
+    "ns3d_tlm_rhs": (C.c_int, [C.c_void_p, DP * 5, DP * 5]),
+    "ns3d_adj_rhs": (C.c_int, [C.c_void_p, DP * 5, C.c_double, DP * 5, DP]),
+    "ns3d_dissipation": (C.c_int, [C.c_void_p, DP]),
+    "ns3d_adj_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, DP]),
+    "ns3d_adj_backward": (C.c_int, [C.c_void_p, C.c_void_p, DP * 5, DP]),
+    "ns3d_adj_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
 }
 
 

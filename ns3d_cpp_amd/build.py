@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_api.cpp", "ns3d_params.cpp"]
+SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_api.cpp", "ns3d_params.cpp"]
 # the mixed-precision path does not reproduce the reference's rounding, so
 # its translation unit may contract a*b+c into FMAs
 FMA_SOURCES = {"ns3d_mixed.hip"}

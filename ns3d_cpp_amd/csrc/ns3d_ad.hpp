@@ -1,0 +1,317 @@
+// ns3d_ad.hpp -- differentiation helpers of the FP64 RHS: dual numbers for
+// forward-mode (tangent) evaluation, scalar-generic copies of the split and
+// reconstructions, and hand-written reverse-mode (vector-Jacobian) rules of
+// the reconstructions and of the viscous flux.
+//
+// The generic functions restate split_state / recon_* / the viscous flux of
+// ns3d_device.hpp for any scalar type T with the same formulas; with T =
+// Dual<N> they carry N directional derivatives.  |x| is differentiated as
+// sign(x) (0 at 0): the plain Steger-Warming and Lax-Friedrichs splits have
+// a kink where an eigenvalue vanishes, which the epsilon-modified split
+// (sw_eps > 0) removes.
+#pragma once
+
+#include "ns3d_device.hpp"
+
+namespace ns3d {
+
+template <int N>
+struct Dual {
+    double v;
+    double d[N];
+};
+
+template <int N>
+__device__ __forceinline__ Dual<N> dconst(double c)
+{
+    Dual<N> r;
+    r.v = c;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = 0.0;
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, const Dual<N> &b)
+{
+    Dual<N> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, const Dual<N> &b)
+{
+    Dual<N> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a)
+{
+    Dual<N> r;
+    r.v = -a.v;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N> &a, const Dual<N> &b)
+{
+    Dual<N> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(const Dual<N> &a, const Dual<N> &b)
+{
+    Dual<N> r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, double c) { Dual<N> r = a; r.v += c; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator+(double c, const Dual<N> &a) { return a + c; }
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, double c) { Dual<N> r = a; r.v -= c; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator-(double c, const Dual<N> &a) { return -a + c; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N> &a, double c)
+{
+    Dual<N> r;
+    r.v = a.v * c;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * c;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator*(double c, const Dual<N> &a) { return a * c; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(const Dual<N> &a, double c) { return a * (1.0 / c); }
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(double c, const Dual<N> &b) { return dconst<N>(c) / b; }
+
+template <int N>
+__device__ __forceinline__ Dual<N> sqrt_t(const Dual<N> &a)
+{
+    Dual<N> r;
+    r.v = sqrt(a.v);
+    const double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * h;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> fabs_t(const Dual<N> &a)
+{
+    const double s = a.v > 0.0 ? 1.0 : (a.v < 0.0 ? -1.0 : 0.0);
+    Dual<N> r;
+    r.v = fabs(a.v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * s;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> pow_t(const Dual<N> &a, double e)
+{
+    Dual<N> r;
+    r.v = pow(a.v, e);
+    const double g = a.v != 0.0 ? e * r.v / a.v : 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * g;
+    return r;
+}
+__device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
+__device__ __forceinline__ double fabs_t(double a) { return fabs(a); }
+__device__ __forceinline__ double pow_t(double a, double e) { return pow(a, e); }
+__device__ __forceinline__ double value_of(double a) { return a; }
+template <int N> __device__ __forceinline__ double value_of(const Dual<N> &a) { return a.v; }
+
+// ------------------------------------------------------ generic forward
+
+template <class T>
+__device__ __forceinline__ void lam_split_t(T lam, double eps, T &lp, T &lm)
+{
+    if (eps == 0.0) {
+        lp = 0.5 * (lam + fabs_t(lam));
+        lm = 0.5 * (lam - fabs_t(lam));
+    } else {
+        T r = sqrt_t(lam * lam + eps * eps);
+        lp = 0.5 * (lam + r);
+        lm = 0.5 * (lam - r);
+    }
+}
+
+// split_state (InviscousFlux_new.cpp:439-530) for a generic scalar
+template <int FVS, int DIR, class T>
+__device__ __forceinline__ void split_t(const DevParams &P, T rho, T u, T v, T w, T p, T Fp[5], T Fm[5])
+{
+    const double gamma = P.gamma;
+    T kinetic = 0.5 * (u * u + v * v + w * w);
+    T e_internal = p / ((gamma - 1.0) * rho);
+    T E = rho * (e_internal + kinetic);
+    T c = sqrt_t(gamma * p / rho);
+    T un = DIR == 0 ? u : (DIR == 1 ? v : w);
+    if constexpr (FVS == FVS_SW) {
+        T l1p, l1m, l2p, l2m, l3p, l3m;
+        lam_split_t(un - c, P.sw_eps, l1p, l1m);
+        lam_split_t(un, P.sw_eps, l2p, l2m);
+        lam_split_t(un + c, P.sw_eps, l3p, l3m);
+        T S1_p = l1p + l3p, S1_m = l1m + l3m;
+        T S2_p = l3p - l1p, S2_m = l3m - l1m;
+        T H = (E + p) / rho;
+        T ga1 = rho / (2.0 * gamma);
+        const double ga2 = 2.0 * (gamma - 1.0);
+        T V = 0.5 * (u * u + v * v + w * w);
+        T cp = S2_p * c, cm = S2_m * c;
+        Fp[0] = ga1 * (S1_p + ga2 * l2p);
+        Fp[1] = ga1 * (S1_p * u + ga2 * l2p * u + (DIR == 0 ? cp : cp * 0.0));
+        Fp[2] = ga1 * (S1_p * v + ga2 * l2p * v + (DIR == 1 ? cp : cp * 0.0));
+        Fp[3] = ga1 * (S1_p * w + ga2 * l2p * w + (DIR == 2 ? cp : cp * 0.0));
+        Fp[4] = ga1 * (S1_p * H + ga2 * l2p * V + cp * un);
+        Fm[0] = ga1 * (S1_m + ga2 * l2m);
+        Fm[1] = ga1 * (S1_m * u + ga2 * l2m * u + (DIR == 0 ? cm : cm * 0.0));
+        Fm[2] = ga1 * (S1_m * v + ga2 * l2m * v + (DIR == 1 ? cm : cm * 0.0));
+        Fm[3] = ga1 * (S1_m * w + ga2 * l2m * w + (DIR == 2 ? cm : cm * 0.0));
+        Fm[4] = ga1 * (S1_m * H + ga2 * l2m * V + cm * un);
+    } else {
+        T a_max = fabs_t(un) + c;
+        T F1 = rho * un;
+        T F2 = rho * u * un + (DIR == 0 ? p : p * 0.0);
+        T F3 = rho * v * un + (DIR == 1 ? p : p * 0.0);
+        T F4 = rho * w * un + (DIR == 2 ? p : p * 0.0);
+        T F5 = (E + p) * un;
+        Fp[0] = 0.5 * (F1 + a_max * rho);
+        Fp[1] = 0.5 * (F2 + a_max * rho * u);
+        Fp[2] = 0.5 * (F3 + a_max * rho * v);
+        Fp[3] = 0.5 * (F4 + a_max * rho * w);
+        Fp[4] = 0.5 * (F5 + a_max * E);
+        Fm[0] = 0.5 * (F1 - a_max * rho);
+        Fm[1] = 0.5 * (F2 - a_max * rho * u);
+        Fm[2] = 0.5 * (F3 - a_max * rho * v);
+        Fm[3] = 0.5 * (F4 - a_max * rho * w);
+        Fm[4] = 0.5 * (F5 - a_max * E);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T recon_weno5_t(const T *s)
+{
+    T f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    const double eps = 1e-6;
+    T d0 = f0 - 2.0 * f1 + f2, e0 = f0 - 4.0 * f1 + 3.0 * f2;
+    T d1 = f1 - 2.0 * f2 + f3, e1 = f1 - f3;
+    T d2 = f2 - 2.0 * f3 + f4, e2 = 3.0 * f2 - 4.0 * f3 + f4;
+    T b0 = (13.0 / 12.0) * d0 * d0 + 0.25 * e0 * e0;
+    T b1 = (13.0 / 12.0) * d1 * d1 + 0.25 * e1 * e1;
+    T b2 = (13.0 / 12.0) * d2 * d2 + 0.25 * e2 * e2;
+    T a0 = 0.1 / ((eps + b0) * (eps + b0));
+    T a1 = 0.6 / ((eps + b1) * (eps + b1));
+    T a2 = 0.3 / ((eps + b2) * (eps + b2));
+    T as = a0 + a1 + a2;
+    T p0 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) / 6.0;
+    T p1 = (-f1 + 5.0 * f2 + 2.0 * f3) / 6.0;
+    T p2 = (2.0 * f2 + 5.0 * f3 - f4) / 6.0;
+    return (a0 * p0 + a1 * p1 + a2 * p2) / as;
+}
+
+// linear reconstruction coefficients (reconstruction.cpp:88-90, 126-135,
+// 227-249), stencil order as reconstruct_select passes it
+template <int RECON>
+__device__ __forceinline__ void linear_coeffs(const DevParams &P, double *c)
+{
+    if constexpr (RECON == R_LINEAR) {
+        c[0] = 1.0; c[1] = 0.0;
+    } else if constexpr (RECON == R_MDCD) {
+        const double disp = P.mdcd_disp, diss = P.mdcd_diss;
+        c[0] = 1.0 / 2.0 * disp + 1.0 / 2.0 * diss;
+        c[1] = -3.0 / 2.0 * disp - 5.0 / 2.0 * diss - 1.0 / 12.0;
+        c[2] = 7.0 / 12.0 + disp + 5.0 * diss;
+        c[3] = 7.0 / 12.0 + disp - 5.0 * diss;
+        c[4] = -3.0 / 2.0 * disp + 5.0 / 2.0 * diss - 1.0 / 12.0;
+        c[5] = 1.0 / 2.0 * disp - 1.0 / 2.0 * diss;
+    } else if constexpr (RECON == R_UD7) {
+        c[0] = -1.0 / 140.0; c[1] = 5.0 / 84.0; c[2] = -101.0 / 420.0; c[3] = 319.0 / 420.0;
+        c[4] = 107.0 / 210.0; c[5] = -19.0 / 210.0; c[6] = 1.0 / 105.0; c[7] = 0.0;
+    } else if constexpr (RECON == R_UD7L) {
+        const double alpha = P.ud7l_alpha;
+        c[0] = -3.0 / 420.0 * alpha - 3.0 / 840.0 * (1.0 - alpha);
+        c[1] = 25.0 / 420.0 * alpha + 29.0 / 840.0 * (1.0 - alpha);
+        c[2] = -101.0 / 420.0 * alpha - 139.0 / 840.0 * (1.0 - alpha);
+        c[3] = 319.0 / 420.0 * alpha + 533.0 / 840.0 * (1.0 - alpha);
+        c[4] = 214.0 / 420.0 * alpha + 533.0 / 840.0 * (1.0 - alpha);
+        c[5] = -38.0 / 420.0 * alpha - 139.0 / 840.0 * (1.0 - alpha);
+        c[6] = 4.0 / 420.0 * alpha + 29.0 / 840.0 * (1.0 - alpha);
+        c[7] = (-3.0 / 840.0) * (1.0 - alpha);
+    }
+}
+
+template <int RECON, class T>
+__device__ __forceinline__ T recon_t(const DevParams &P, const T *s)
+{
+    if constexpr (RECON == R_WENO5) {
+        return recon_weno5_t(s);
+    } else {
+        constexpr int S = Stencil<RECON>::S;
+        double c[S];
+        linear_coeffs<RECON>(P, c);
+        T r = s[0] * c[0];
+#pragma unroll
+        for (int q = 1; q < S; ++q) r = r + s[q] * c[q];
+        return r;
+    }
+}
+
+// ------------------------------------------------------------ reverse rules
+
+// sbar[s] = gbar * d recon / d s[s]
+template <int RECON>
+__device__ __forceinline__ void recon_vjp(const DevParams &P, const double *s, double gbar, double *sbar)
+{
+    constexpr int S = Stencil<RECON>::S;
+    if constexpr (RECON == R_WENO5) {
+        const double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+        const double eps = 1e-6;
+        const double d0 = f0 - 2.0 * f1 + f2, e0 = f0 - 4.0 * f1 + 3.0 * f2;
+        const double d1 = f1 - 2.0 * f2 + f3, e1 = f1 - f3;
+        const double d2 = f2 - 2.0 * f3 + f4, e2 = 3.0 * f2 - 4.0 * f3 + f4;
+        const double b0 = (13.0 / 12.0) * d0 * d0 + 0.25 * e0 * e0;
+        const double b1 = (13.0 / 12.0) * d1 * d1 + 0.25 * e1 * e1;
+        const double b2 = (13.0 / 12.0) * d2 * d2 + 0.25 * e2 * e2;
+        const double t0 = eps + b0, t1 = eps + b1, t2 = eps + b2;
+        const double a0 = 0.1 / (t0 * t0), a1 = 0.6 / (t1 * t1), a2 = 0.3 / (t2 * t2);
+        const double as = a0 + a1 + a2;
+        const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+        const double p0 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) / 6.0;
+        const double p1 = (-f1 + 5.0 * f2 + 2.0 * f3) / 6.0;
+        const double p2 = (2.0 * f2 + 5.0 * f3 - f4) / 6.0;
+        const double val = w0 * p0 + w1 * p1 + w2 * p2;
+        // d val / d p_k = w_k ; d val / d a_k = (p_k - val) / as ;
+        // d a_k / d b_k = -2 a_k / t_k
+        const double ab0 = gbar * (p0 - val) / as * (-2.0 * a0 / t0);
+        const double ab1 = gbar * (p1 - val) / as * (-2.0 * a1 / t1);
+        const double ab2 = gbar * (p2 - val) / as * (-2.0 * a2 / t2);
+        const double gd0 = ab0 * (13.0 / 6.0) * d0, ge0 = ab0 * 0.5 * e0;
+        const double gd1 = ab1 * (13.0 / 6.0) * d1, ge1 = ab1 * 0.5 * e1;
+        const double gd2 = ab2 * (13.0 / 6.0) * d2, ge2 = ab2 * 0.5 * e2;
+        const double g0 = gbar * w0 / 6.0, g1 = gbar * w1 / 6.0, g2 = gbar * w2 / 6.0;
+        sbar[0] = 2.0 * g0 + gd0 + ge0;
+        sbar[1] = -7.0 * g0 - g1 - 2.0 * gd0 - 4.0 * ge0 + gd1 + ge1;
+        sbar[2] = 11.0 * g0 + 5.0 * g1 + 2.0 * g2 + gd0 + 3.0 * ge0 - 2.0 * gd1 + gd2 + 3.0 * ge2;
+        sbar[3] = 2.0 * g1 + 5.0 * g2 + gd1 - ge1 - 2.0 * gd2 - 4.0 * ge2;
+        sbar[4] = -g2 + gd2 + ge2;
+        sbar[5] = 0.0;
+    } else {
+        double c[S];
+        linear_coeffs<RECON>(P, c);
+#pragma unroll
+        for (int q = 0; q < S; ++q) sbar[q] = gbar * c[q];
+    }
+}
+
+}  // namespace ns3d

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ns3d_gpu.h"
@@ -50,11 +51,23 @@ struct ns3d_ctx {
     std::vector<std::string> kind_name;
     std::vector<double> kind_ms;
     std::vector<long long> kind_n;
+    // differentiation state, allocated on first use
+    double *adj_slab = nullptr;     // the AdjArrays cell arrays
+    AdjArrays adj{};
+    double *adj_part = nullptr;     // per-block partials of the adjoint kernels
+    double *adj_sums = nullptr;     // one reduced sum per (step, stage)
+    long long adj_nsums = 0;
+    double *ck_slab = nullptr;      // checkpoints, stage snapshots, final state
+    long long ck_states = 0;        // 6-array states held in ck_slab
+    int fw_steps = 0, fw_ckpt = 0;
+    double fw_dt = 0.0;
+    bool fw_valid = false;          // a forward pass matches the current state
 };
 
 namespace {
 
 constexpr int kCellArrays = 6 + 5 + 5 + 12 + 12;
+constexpr int kAdjArrays = 5 * 5 + 12;   // X Y Z T C, then V and G share storage
 
 int hip_fail(ns3d_ctx *c, hipError_t e, const char *where)
 {
@@ -269,6 +282,100 @@ int copy_out(ns3d_ctx *c, double *dst, const double *src)
     return NS3D_OK;
 }
 
+
+// ------------------------------------------------------- differentiation
+
+int adj_check(ns3d_ctx *c)
+{
+    if (c->mixed || c->P.char_recon || !ad_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
+    if (c->dp.nx < 8 || c->dp.ny < 8 || c->dp.nz < 8) return NS3D_ERR_GRID_TOO_SMALL;
+    return NS3D_OK;
+}
+
+int ensure_adj(ns3d_ctx *c)
+{
+    if (c->adj_slab) return NS3D_OK;
+    const size_t bytes = (size_t)c->ncell * kAdjArrays * sizeof(double);
+    HIPCALL(c, hipMalloc((void **)&c->adj_slab, bytes));
+    HIPCALL(c, hipMemsetAsync(c->adj_slab, 0, bytes, c->stream));
+    HIPCALL(c, hipMalloc((void **)&c->adj_part, (size_t)ad_blocks(c->dp) * sizeof(double)));
+    double *q = c->adj_slab;
+    AdjArrays &A = c->adj;
+    for (int m = 0; m < 5; ++m) { A.X[m] = q; q += c->ncell; }
+    for (int m = 0; m < 5; ++m) { A.Y[m] = q; q += c->ncell; }
+    for (int m = 0; m < 5; ++m) { A.Z[m] = q; q += c->ncell; }
+    for (int m = 0; m < 5; ++m) { A.T[m] = q; q += c->ncell; }
+    for (int m = 0; m < 5; ++m) { A.C[m] = q; q += c->ncell; }
+    // the TLM's viscous-flux tangents and the adjoint's gradient adjoints are
+    // never live at the same time
+    for (int m = 0; m < 12; ++m) { A.V[m] = q; A.G[m] = q; q += c->ncell; }
+    return NS3D_OK;
+}
+
+int ensure_sums(ns3d_ctx *c, long long n)
+{
+    if (c->adj_nsums >= n) return NS3D_OK;
+    if (c->adj_sums) { HIPCALL(c, hipFree(c->adj_sums)); c->adj_sums = nullptr; c->adj_nsums = 0; }
+    HIPCALL(c, hipMalloc((void **)&c->adj_sums, (size_t)n * sizeof(double)));
+    c->adj_nsums = n;
+    return NS3D_OK;
+}
+
+int upload_set(ns3d_ctx *c, double *const dst[5], const double *const src[5])
+{
+    for (int m = 0; m < 5; ++m) {
+        if (src && src[m]) {
+            int st = copy_in(c, dst[m], src[m]);
+            if (st) return st;
+        } else {
+            HIPCALL(c, hipMemsetAsync(dst[m], 0, (size_t)c->ncell * sizeof(double), c->stream));
+        }
+    }
+    return NS3D_OK;
+}
+
+int download_set(ns3d_ctx *c, double *const dst[5], double *const src[5])
+{
+    for (int m = 0; m < 5; ++m) {
+        int st = copy_out(c, dst ? dst[m] : nullptr, src[m]);
+        if (st) return st;
+    }
+    return NS3D_OK;
+}
+
+double compensated_sum(const std::vector<double> &h)
+{
+    double s = 0.0, comp = 0.0;
+    for (double x : h) {
+        double y = x - comp;
+        double t = s + y;
+        comp = (t - s) - y;
+        s = t;
+    }
+    return s;
+}
+
+// Y <- W-adjoint of one RHS at the state of `F`, sum of the 1/Re partials
+// into adj_sums[slot]
+int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot)
+{
+    HIPCALL(c, hipMemsetAsync(c->adj.Y[0], 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
+    const int seg = c->seg_len[1];
+    TIMED(c, "adj_rhs", launch_adj_rhs(c->stream, c->dp, F, c->adj, w_obj, c->adj_part, seg));
+    TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), c->adj_sums + slot));
+    return NS3D_OK;
+}
+
+// 6-array primitive state (W incl. T, all cells) of checkpoint slot q
+double *ck_state(ns3d_ctx *c, long long q) { return c->ck_slab + q * 6 * c->ncell; }
+
+int copy_state(ns3d_ctx *c, double *dst, const double *src)
+{
+    HIPCALL(c, hipMemcpyAsync(dst, src, (size_t)c->ncell * 6 * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+    return NS3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -399,6 +506,10 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_ptrs) (void)hipFree(c->d_ptrs);
+    if (c->adj_slab) (void)hipFree(c->adj_slab);
+    if (c->adj_part) (void)hipFree(c->adj_part);
+    if (c->adj_sums) (void)hipFree(c->adj_sums);
+    if (c->ck_slab) (void)hipFree(c->ck_slab);
     for (auto &q : c->pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -422,6 +533,7 @@ int ns3d_upload_primitive(ns3d_ctx *c, const double *rho, const double *u, const
                           const double *w, const double *p)
 {
     if (!c || !rho || !u || !v || !w || !p) return NS3D_ERR_INVALID_ARG;
+    c->fw_valid = false;
     const double *src[5] = {rho, u, v, w, p};
     if (c->mixed) {
         for (int m = 0; m < 5; ++m) {
@@ -449,6 +561,7 @@ int ns3d_upload_conserved(ns3d_ctx *c, const double *rho, const double *ru, cons
                           const double *rw, const double *E)
 {
     if (!c || !rho || !ru || !rv || !rw || !E) return NS3D_ERR_INVALID_ARG;
+    c->fw_valid = false;
     const double *src[5] = {rho, ru, rv, rw, E};
     if (c->mixed) {
         // keep U exactly; the shadow comes from C2P(U)
@@ -608,6 +721,7 @@ int ns3d_compute_rhs(ns3d_ctx *c)
 int ns3d_rk3_step(ns3d_ctx *c, double dt)
 {
     if (!c) return NS3D_ERR_INVALID_ARG;
+    c->fw_valid = false;
     int st = rk3_enqueue(c, dt);
     if (st) return st;
     return check_flag(c);
@@ -616,6 +730,7 @@ int ns3d_rk3_step(ns3d_ctx *c, double dt)
 int ns3d_advance(ns3d_ctx *c, int nsteps, double dt)
 {
     if (!c || nsteps < 0) return NS3D_ERR_INVALID_ARG;
+    c->fw_valid = false;
     for (int s = 0; s < nsteps; ++s) {
         int st = rk3_enqueue(c, dt);
         if (st) return st;
@@ -748,6 +863,171 @@ int ns3d_profile_entry(ns3d_ctx *c, int idx, char *name, int name_len, double *t
     }
     if (total_ms) *total_ms = c->kind_ms[idx];
     if (launches) *launches = c->kind_n[idx];
+    return NS3D_OK;
+}
+
+int ns3d_tlm_rhs(ns3d_ctx *c, const double *const dU[5], double *const out[5])
+{
+    if (!c || !dU || !out) return NS3D_ERR_INVALID_ARG;
+    int st = adj_check(c);
+    if (!st) st = ensure_adj(c);
+    if (!st) st = upload_set(c, c->adj.X, dU);
+    if (st) return st;
+    TIMED(c, "tlm_rhs", launch_tlm_rhs(c->stream, c->dp, c->f, c->adj));
+    st = download_set(c, out, c->adj.Z);
+    if (st) return st;
+    return stream_sync(c);
+}
+
+int ns3d_adj_rhs(ns3d_ctx *c, const double *const lam[5], double w_obj, double *const out[5], double *dtheta)
+{
+    if (!c || !lam || !out) return NS3D_ERR_INVALID_ARG;
+    int st = adj_check(c);
+    if (!st) st = ensure_adj(c);
+    if (!st) st = ensure_sums(c, 1);
+    if (!st) st = upload_set(c, c->adj.X, lam);
+    if (!st) st = adj_rhs_enqueue(c, c->f, w_obj, 0);
+    if (st) return st;
+    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, c->f, c->adj, 0.0, 0.0, 1.0, 1, 0));
+    st = download_set(c, out, c->adj.Z);
+    if (st) return st;
+    double h = 0.0;
+    HIPCALL(c, hipMemcpyAsync(&h, c->adj_sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    if (dtheta) *dtheta = h;
+    return NS3D_OK;
+}
+
+int ns3d_dissipation(ns3d_ctx *c, double *eps)
+{
+    if (!c || !eps) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;
+    int st = ensure_adj(c);
+    if (st) return st;
+    TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, c->f, c->adj_part));
+    std::vector<double> h((size_t)ad_blocks(c->dp));
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->adj_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    *eps = compensated_sum(h) / ((double)c->dp.nx * c->dp.ny * c->dp.nz);
+    return NS3D_OK;
+}
+
+int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double *J)
+{
+    if (!c || nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0) || !J) return NS3D_ERR_INVALID_ARG;
+    int st = adj_check(c);
+    if (!st) st = ensure_adj(c);
+    if (st) return st;
+    c->fw_valid = false;
+    if (ckpt_every > nsteps) ckpt_every = nsteps;
+    const long long ncp = (nsteps + ckpt_every - 1) / ckpt_every;
+    // checkpoints, 3 stage inputs per step of one segment, the final state
+    const long long states = ncp + 3LL * ckpt_every + 1;
+    if (states > c->ck_states) {
+        if (c->ck_slab) { HIPCALL(c, hipFree(c->ck_slab)); c->ck_slab = nullptr; c->ck_states = 0; }
+        HIPCALL(c, hipMalloc((void **)&c->ck_slab, (size_t)states * 6 * c->ncell * sizeof(double)));
+        c->ck_states = states;
+    }
+    st = ensure_sums(c, 4LL * nsteps);
+    if (st) return st;
+    // forward: J terms into adj_sums[3*nsteps + n]
+    double *jsum = c->adj_sums + 3LL * nsteps;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % ckpt_every == 0) {
+            st = copy_state(c, ck_state(c, n / ckpt_every), c->f.W[0]);
+            if (st) return st;
+        }
+        TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, c->f, c->adj_part));
+        TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), jsum + n));
+        st = rk3_enqueue(c, dt);
+        if (st) return st;
+    }
+    st = copy_state(c, ck_state(c, c->ck_states - 1), c->f.W[0]);
+    if (st) return st;
+    st = check_flag(c);
+    if (st) return st;
+    std::vector<double> h((size_t)nsteps);
+    HIPCALL(c, hipMemcpyAsync(h.data(), jsum, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    *J = dt * compensated_sum(h) / ((double)c->dp.nx * c->dp.ny * c->dp.nz);
+    c->fw_steps = nsteps;
+    c->fw_ckpt = ckpt_every;
+    c->fw_dt = dt;
+    c->fw_valid = true;
+    return NS3D_OK;
+}
+
+int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const lam0[5], double *dJ_dinvRe)
+{
+    if (!c || !lam0) return NS3D_ERR_INVALID_ARG;
+    if (!c->fw_valid) return NS3D_ERR_INVALID_ARG;
+    int st = adj_check(c);
+    if (st) return st;
+    const int N = c->fw_steps, K = c->fw_ckpt;
+    const double dt = c->fw_dt;
+    const double w_obj = 1.0 / ((double)c->dp.nx * c->dp.ny * c->dp.nz);
+    // SSP-RK3 stage coefficients, range_kutta.cpp:102-147
+    const double a2 = 0.75, b2 = 0.25, a3 = 1.0 / 3.0, b3 = 2.0 / 3.0;
+    st = upload_set(c, c->adj.X, lamN);
+    if (st) return st;
+    const long long snap0 = (N + K - 1) / K;   // first stage-snapshot slot
+    for (int seg = (N - 1) / K; seg >= 0; --seg) {
+        const int n0 = seg * K, n1 = n0 + K < N ? n0 + K : N;
+        // recompute the segment from its checkpoint, keeping every stage input
+        st = copy_state(c, c->f.W[0], ck_state(c, seg));
+        if (st) return st;
+        for (int n = n0; n < n1; ++n)
+            for (int stage = 1; stage <= 3; ++stage) {
+                st = copy_state(c, ck_state(c, snap0 + 3LL * (n - n0) + stage - 1), c->f.W[0]);
+                if (!st) st = rhs_pass(c, stage, dt);
+                if (!st) st = fill_ghosts(c);
+                if (st) return st;
+            }
+        for (int n = n1 - 1; n >= n0; --n) {
+            for (int stage = 3; stage >= 1; --stage) {
+                Fields F = c->f;
+                double *W = ck_state(c, snap0 + 3LL * (n - n0) + stage - 1);
+                for (int m = 0; m < 6; ++m) F.W[m] = W + m * c->ncell;
+                st = adj_rhs_enqueue(c, F, stage == 1 ? w_obj : 0.0, 3LL * n + stage - 1);
+                if (st) return st;
+                if (stage == 3)
+                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, a3, b3, b3 * dt, 1, 0));
+                else if (stage == 2)
+                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, a2, b2, b2 * dt, 0, 0));
+                else
+                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, 0.0, 1.0, dt, 0, 1));
+                for (int m = 0; m < 5; ++m) std::swap(c->adj.X[m], c->adj.Z[m]);
+            }
+        }
+    }
+    // leave the context at the final state of the forward pass
+    st = copy_state(c, c->f.W[0], ck_state(c, c->ck_states - 1));
+    if (!st) st = download_set(c, lam0, c->adj.X);
+    if (st) return st;
+    std::vector<double> h(3 * (size_t)N);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->adj_sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    if (dJ_dinvRe) {
+        const double wst[3] = {dt, b2 * dt, b3 * dt};
+        std::vector<double> t(h.size());
+        for (size_t q = 0; q < h.size(); ++q) t[q] = wst[q % 3] * h[q];
+        *dJ_dinvRe = compensated_sum(t);
+    }
+    return NS3D_OK;
+}
+
+int ns3d_adj_memory(const ns3d_ctx *c, long long *bytes)
+{
+    if (!c || !bytes) return NS3D_ERR_INVALID_ARG;
+    long long b = 0;
+    if (c->adj_slab) b += c->ncell * kAdjArrays * (long long)sizeof(double);
+    b += c->ck_states * 6 * c->ncell * (long long)sizeof(double);
+    b += c->adj_nsums * (long long)sizeof(double);
+    *bytes = b;
     return NS3D_OK;
 }
 

@@ -26,6 +26,20 @@ struct MFields {
     float *VF[12];   // viscous fluxes
 };
 
+// Arrays of the tangent-linear and adjoint RHS (FP64, interior cells used).
+//   TLM:     X = dU (in), Y = dW, T = face-flux tangents / dT, V = dVF, Z = dR (out)
+//   adjoint: X = lambda (in), Y = lambda_W, G = lambda of the gradients,
+//            C = RK3 accumulator of lambda_U0, Z = combined result (out)
+struct AdjArrays {
+    double *X[5];
+    double *Y[5];
+    double *Z[5];
+    double *T[5];
+    double *C[5];
+    double *V[12];
+    double *G[12];
+};
+
 __device__ __forceinline__ int wrapi_m(int q, int ng, int n)
 {
     return q < ng ? q + n : (q >= ng + n ? q - n : q);
@@ -67,5 +81,16 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, unsigned long long key);
 hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
+
+// tangent-linear / adjoint RHS (ns3d_adjoint.hip)
+bool ad_supported(const DevParams &P);
+int ad_blocks(const DevParams &P);
+hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A);
+hipError_t launch_adj_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                          double *partial, int seg);
+hipError_t launch_adj_combine(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,
+                              double bx, double bl, int init, int add_acc);
+hipError_t launch_dissip(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
+hipError_t launch_sum_partials(hipStream_t s, const double *partial, int n, double *out);
 
 }  // namespace ns3d

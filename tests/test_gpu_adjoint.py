@@ -1,0 +1,316 @@
+"""Tangent-linear and adjoint RHS, and the checkpointed adjoint of the RK3
+time loop, on the GPU.
+
+Checks that pin them independently of the code under test:
+  * the TLM against central finite differences of the GPU RHS (O(h^2));
+  * the adjoint against the TLM by the dot-product identity
+    <y, R' x> = <R'^T y, x> (the TLM is forward-mode dual numbers, the
+    adjoint hand-written reverse rules: they share no derivative code);
+  * the gradient of J = sum_n dt * eps(U^n) against finite differences of J
+    evaluated by the GPU forward path and by the CPU oracle's RK3 loop, the
+    latter with eps computed in numpy from the oracle's gradients;
+  * d/d(1/Re) against finite differences in Re;
+  * checkpoint spacing changes nothing (the recomputation is bitwise).
+The split uses sw_eps > 0 where finite differences are taken, so the flux is
+smooth; the dot-product test also covers the plain |lambda| splits.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ns3d_cpp_amd as ns  # noqa: E402
+from ns3d_cpp_amd import fields  # noqa: E402
+from oracle import Oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(Re=100.0, Pr=0.72, sutherland_s=110.4 / 273.15, cfl=0.5)
+
+
+def state(n, seed=3, Mt=0.3):
+    """Smooth compressible state: HIT velocity with density/pressure ripples."""
+    st, Ma = fields.hit(n, k0=2, Mt=Mt, seed=seed)
+    rho, u, v, w, p = [np.array(a) for a in st]
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    X, Y, Z = X.transpose(2, 1, 0), Y.transpose(2, 1, 0), Z.transpose(2, 1, 0)
+    rho = rho * (1.0 + 0.05 * np.sin(X + 2 * Y) * np.cos(Z))
+    p = p * (1.0 + 0.05 * np.cos(2 * X - Z) * np.sin(Y))
+    return (rho, u, v, w, p), Ma
+
+
+def conserved_of(prims, gamma=1.4):
+    rho, u, v, w, p = prims
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return [rho, rho * u, rho * v, rho * w, E]
+
+
+def smooth_dir(n, seed, scale):
+    """Random smooth direction (low Fourier modes), scaled per variable."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    out = []
+    for m in range(5):
+        a = np.zeros((n, n, n))
+        for _ in range(4):
+            k = rng.integers(-2, 3, size=3)
+            ph = rng.uniform(0, 2 * np.pi)
+            a += rng.normal() * np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)
+        out.append(scale[m] * a.transpose(2, 1, 0))
+    return out
+
+
+def interior_dot(s, a, b):
+    return float(sum(np.sum(s.interior(x) * s.interior(y)) for x, y in zip(a, b)))
+
+
+def solver(n, prims, Ma, **cfg):
+    kw = dict(BASE)
+    kw.update(cfg)
+    p, g = ns.make_params(n, Ma=Ma, **kw)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*prims)
+    return s
+
+
+DOT_CASES = [
+    dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th"),
+    dict(recon="WENO5", fvs_type="LaxFriedrichs", vis_scheme="C6th"),
+    dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C4th", sw_eps=0.2, mu_law="power", mu_power=0.76),
+    dict(recon="LINEAR", fvs_type="StagerWarming", vis_scheme="C8th", mu_law="constant"),
+    dict(recon="MDCD", fvs_type="LaxFriedrichs", vis_scheme="C8th"),
+    dict(recon="UPWIND_7TH", fvs_type="StagerWarming", vis_scheme="C8th"),
+    dict(recon="UD7L", fvs_type="StagerWarming", vis_scheme="C6th", ud7l_alpha=0.3),
+]
+
+
+@pytest.mark.parametrize("cfg", DOT_CASES, ids=lambda c: f"{c['recon']}-{c['fvs_type']}-{c['vis_scheme']}")
+def test_dot_product_identity(gpu, cfg):
+    n = 16
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, **cfg)
+    rng = np.random.default_rng(11)
+    x = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    y = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    Rx = s.tlm_rhs(x)
+    RTy, _ = s.adj_rhs(y, 0.0)
+    lhs = interior_dot(s, y, Rx)
+    rhs = interior_dot(s, RTy, x)
+    scale = np.sqrt(interior_dot(s, y, y) * interior_dot(s, Rx, Rx))
+    print(cfg, lhs, rhs, scale)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def _fd_rhs(n, prims, Ma, cfg, dU, h):
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    out = []
+    for sg in (+1, -1):
+        s.upload_conserved(*[a + sg * h * d for a, d in zip(U, dU)])
+        out.append(s.compute_rhs())
+    return [(a - b) / (2 * h) for a, b in zip(*out)]
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3),
+    dict(recon="UPWIND_7TH", fvs_type="StagerWarming", vis_scheme="C6th", sw_eps=0.3, mu_law="power",
+         mu_power=0.7),
+], ids=["weno5", "ud7"])
+def test_tlm_against_finite_differences(gpu, cfg):
+    n = 16
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, **cfg)
+    dU = [s._full(a) for a in smooth_dir(n, 5, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    tl = s.tlm_rhs(dU)
+    fd = _fd_rhs(n, prims, Ma, cfg, dU, 1e-5)
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(tl, fd)))
+    den = np.sqrt(sum(np.sum(s.interior(b) ** 2) for b in fd))
+    print("TLM vs FD rel", num / den)
+    assert num / den < 1e-7
+
+
+def _sum_phi(s):
+    g = s.grid
+    return s.dissipation() * g.nx * g.ny * g.nz
+
+
+def test_adjoint_objective_and_theta(gpu):
+    """w_obj * d(sum Phi)/dU and d/d(1/Re) of lam.R + w_obj sum Phi."""
+    n = 16
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    rng = np.random.default_rng(2)
+    lam = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    w = 0.37
+    g, th = s.adj_rhs(lam, w)
+    # state direction
+    dU = [s._full(a) for a in smooth_dir(n, 9, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    h = 1e-5
+    val = []
+    for sg in (+1, -1):
+        s.upload_conserved(*[a + sg * h * d for a, d in zip(U, dU)])
+        R = s.compute_rhs()
+        val.append(interior_dot(s, lam, R) + w * _sum_phi(s))
+    fd = (val[0] - val[1]) / (2 * h)
+    ad = interior_dot(s, g, dU)
+    print("state", fd, ad)
+    assert abs(fd - ad) <= 1e-7 * abs(fd)
+    # 1/Re direction
+    theta = 1.0 / BASE["Re"]
+    ht = 1e-6 * theta
+    val = []
+    for sg in (+1, -1):
+        st_ = solver(n, prims, Ma, **dict(cfg, Re=1.0 / (theta + sg * ht)))
+        R = st_.compute_rhs()
+        val.append(interior_dot(st_, lam, R) + w * _sum_phi(st_))
+    fdt = (val[0] - val[1]) / (2 * ht)
+    print("theta", fdt, th)
+    assert abs(fdt - th) <= 1e-7 * abs(fdt)
+
+
+def _J_gpu(n, prims, Ma, cfg, U, dU, h, nsteps, dt):
+    s = solver(n, prims, Ma, **cfg)
+    s.upload_conserved(*[a + h * d for a, d in zip(U, dU)])
+    return s.adj_forward(nsteps, dt, nsteps)
+
+
+def test_gradient_of_time_integrated_dissipation(gpu):
+    n, nsteps, dt = 16, 7, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    s.upload_conserved(*U)
+    J = s.adj_forward(nsteps, dt, 3)
+    lam0, dth = s.adj_backward()
+    dU = [s._full(a) for a in smooth_dir(n, 21, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    h = 1e-5
+    Jp = _J_gpu(n, prims, Ma, cfg, U, dU, +h, nsteps, dt)
+    Jm = _J_gpu(n, prims, Ma, cfg, U, dU, -h, nsteps, dt)
+    J0 = _J_gpu(n, prims, Ma, cfg, U, dU, 0.0, nsteps, dt)
+    assert J0 == J
+    fd = (Jp - Jm) / (2 * h)
+    ad = interior_dot(s, lam0, dU)
+    print("J", J, "fd", fd, "adjoint", ad)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    # 1/Re
+    theta = 1.0 / BASE["Re"]
+    ht = 1e-6 * theta
+    Jt = []
+    for sg in (+1, -1):
+        st_ = solver(n, prims, Ma, **dict(cfg, Re=1.0 / (theta + sg * ht)))
+        st_.upload_conserved(*U)
+        Jt.append(st_.adj_forward(nsteps, dt, nsteps))
+    fdt = (Jt[0] - Jt[1]) / (2 * ht)
+    print("dJ/dtheta fd", fdt, "adjoint", dth)
+    assert abs(fdt - dth) <= 1e-6 * abs(fdt)
+
+
+def test_terminal_adjoint_and_checkpoint_spacing(gpu):
+    """lam0 = dJ/dU0 + (dU^N/dU^0)^T lamN, the same for any checkpoint spacing."""
+    n, nsteps, dt = 16, 5, 0.01
+    cfg = dict(recon="WENO5", fvs_type="LaxFriedrichs", vis_scheme="C6th")
+    prims, Ma = state(n, seed=8)
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    rng = np.random.default_rng(4)
+    lamN = [s._full(rng.normal(size=(n, n, n)) * 1e-3) for _ in range(5)]
+    res = []
+    for ck in (1, 2, 5):
+        s.upload_conserved(*U)
+        s.adj_forward(nsteps, dt, ck)
+        res.append(s.adj_backward(lamN))
+    for lam0, th in res[1:]:
+        assert th == res[0][1]
+        for a, b in zip(lam0, res[0][0]):
+            assert np.array_equal(s.interior(a), s.interior(b))
+    # the context is left at U^N
+    UN = s.conserved()
+    s2 = solver(n, prims, Ma, **cfg)
+    s2.upload_conserved(*U)
+    s2.advance(nsteps, dt)
+    for a, b in zip(UN, s2.conserved()):
+        assert np.array_equal(s.interior(a), s.interior(b))
+    # against FD of J + <lamN, U^N> (LF: |u_n| kinks are far from the
+    # perturbation size used here unless u_n ~ 0 within 1e-9 somewhere)
+    dU = [s._full(a) for a in smooth_dir(n, 17, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    h = 1e-6
+    val = []
+    for sg in (+1, -1):
+        s3 = solver(n, prims, Ma, **cfg)
+        s3.upload_conserved(*[a + sg * h * d for a, d in zip(U, dU)])
+        J = s3.adj_forward(nsteps, dt, nsteps)
+        val.append(J + interior_dot(s3, lamN, s3.conserved()))
+    fd = (val[0] - val[1]) / (2 * h)
+    ad = interior_dot(s, res[0][0], dU)
+    print("fd", fd, "adjoint", ad)
+    assert abs(fd - ad) <= 1e-5 * abs(fd)
+
+
+def _phi_sum_oracle(o, gamma_unused=None):
+    """sum over interior cells of Phi from the oracle's gradients and mu(T)."""
+    G = [o.interior(a) for a in o.compute_gradients()]
+    T = o.interior(o.primitive()["T"])
+    p = o.params
+    S = p.sutherland_s
+    mu = p.mu * T ** 1.5 * (1.0 + S) / (T + S)
+    ux, uy, uz, vx, vy, vz, wx, wy, wz = G[:9]
+    div = ux + vy + wz
+    phi = mu * (2 * (ux * ux + vy * vy + wz * wz) - (2.0 / 3.0) * div * div
+                + (uy + vx) ** 2 + (uz + wx) ** 2 + (vz + wy) ** 2)
+    return float(np.sum(phi))
+
+
+def test_gradient_against_oracle_finite_differences(gpu):
+    """The adjoint gradient of J against FD of J computed by the CPU oracle."""
+    n, nsteps, dt = 12, 4, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = state(n, seed=12)
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    s.upload_conserved(*U)
+    J = s.adj_forward(nsteps, dt, 2)
+    lam0, _ = s.adj_backward()
+    dU = [s._full(a) for a in smooth_dir(n, 33, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    kw = dict(BASE)
+    kw.update(cfg)
+    p, g = ns.make_params(n, Ma=Ma, **kw)
+
+    def J_oracle(h):
+        o = Oracle(p, g)
+        o.set_conserved(*[a + h * d for a, d in zip(U, dU)])
+        tot = 0.0
+        for _ in range(nsteps):
+            tot += dt * _phi_sum_oracle(o) / n ** 3
+            st, _ = o.step(dt)
+            assert st == 0
+        return tot
+
+    Jo = J_oracle(0.0)
+    assert abs(Jo - J) <= 1e-12 * abs(J)
+    h = 1e-5
+    fd = (J_oracle(h) - J_oracle(-h)) / (2 * h)
+    ad = interior_dot(s, lam0, dU)
+    print("oracle fd", fd, "adjoint", ad)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+def test_unsupported_settings(gpu):
+    n = 16
+    prims, Ma = state(n)
+    for cfg in (dict(recon="MDCD_HYBRID"), dict(precision="mixed"), dict(char_recon=1)):
+        s = solver(n, prims, Ma, **cfg)
+        with pytest.raises(ns.NS3DError):
+            s.tlm_rhs([np.zeros((n, n, n))] * 5)
+    s = solver(n, prims, Ma)
+    with pytest.raises(ns.NS3DError):
+        s.adj_backward()   # no forward pass recorded
