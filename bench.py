@@ -157,6 +157,64 @@ def mixed_error(ns, n, st, Ma, Re, recon, nsteps):
             "kinetic_energy_fp64": ek[0], "kinetic_energy_mixed": ek[1]}
 
 
+def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
+    """Checkpointed adjoint of J = sum_n dt * eps(U^n) over nsteps FP64 RK3
+    steps (isotropic turbulence n^3, k0=4, Mt=0.3, Re=100, WENO5+SW+C8,
+    fixed dt = the CFL dt of the initial field), against the plain forward
+    pass over the same steps.  Times are HIP events on the solver's stream."""
+    from ns3d_cpp_amd import fields
+    st, Ma = fields.hit(n, k0=4.0, Mt=0.3, seed=seed)
+    p, g = make(ns, n, Ma, 100.0, "WENO5", "fp64")
+    s = ns.Solver(p, g)
+    U0 = None
+    s.upload_primitive(*st)
+    U0 = s.conserved()
+    dt = s.timestep()
+    ev = [C.c_void_p() for _ in range(4)]
+    for e in ev:
+        H.hipEventCreate(C.byref(e))
+    # warm-up of every kernel involved
+    s.advance(2, dt)
+    s.upload_conserved(*U0)
+    s.adj_forward(2, dt, 1)
+    s.adj_backward()
+    s.upload_conserved(*U0)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    s.advance(nsteps, dt)
+    H.hipEventRecord(ev[1], s.stream())
+    H.hipEventSynchronize(ev[1])
+    fwd = C.c_float()
+    H.hipEventElapsedTime(C.byref(fwd), ev[0], ev[1])
+    s.upload_conserved(*U0)
+    s.profile(True)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    J = s.adj_forward(nsteps, dt, ckpt)
+    H.hipEventRecord(ev[1], s.stream())
+    lam0, dJ_dinvRe = s.adj_backward()
+    H.hipEventRecord(ev[2], s.stream())
+    H.hipEventSynchronize(ev[2])
+    t_f, t_b = C.c_float(), C.c_float()
+    H.hipEventElapsedTime(C.byref(t_f), ev[0], ev[1])
+    H.hipEventElapsedTime(C.byref(t_b), ev[1], ev[2])
+    rep = s.profile_report()
+    s.profile(False)
+    mem = s.adj_memory()
+    base = ns.lib().ns3d_cells(s.ctx) * 40 * 8
+    gnorm = float(np.sqrt(sum(np.sum(s.interior(a) ** 2) for a in lam0)))
+    s.close()
+    total = t_f.value + t_b.value
+    return {"config": f"isotropic turbulence {n}^3 FP64, k0=4, Mt=0.3, Re=100, WENO5+SW+C8, {nsteps} RK3 steps, "
+                      f"fixed dt={dt:.4g}, checkpoint every {ckpt}, J = sum dt * mean dissipation",
+            "J": J, "dJ_dinvRe": dJ_dinvRe, "grad_U0_l2": gnorm,
+            "forward_ms": round(fwd.value, 3), "adj_forward_ms": round(t_f.value, 3),
+            "adj_backward_ms": round(t_b.value, 3), "adjoint_total_ms": round(total, 3),
+            "cost_vs_forward": round(total / fwd.value, 3),
+            "device_bytes": int(mem + base), "device_GB": round((mem + base) / 1e9, 3),
+            "kernels_ms": {k: round(v[0], 2) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}}
+
+
 def cpu_baseline(n, nsteps, Re, Mt, k0, seed):
     """The CPU FP64 oracle on the host's cores, on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -191,6 +249,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--adj-n", type=int, default=128)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-ckpt", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -215,6 +276,9 @@ def main():
     if args.precision == "both" and args.err_steps > 0:
         err = mixed_error(ns, n, st, Ma, Re, args.recon, args.err_steps)
     del st
+    adj = None
+    if args.adj_steps > 0:
+        adj = adjoint_cost(ns, H, args.adj_n, args.adj_steps, args.adj_ckpt, args.seed)
 
     cpu = None
     if not args.no_cpu:
@@ -251,6 +315,8 @@ def main():
         line["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
     if err is not None:
         line["mixed_error_vs_fp64"] = err
+    if adj is not None:
+        line["adjoint"] = adj
     line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 

@@ -26,9 +26,9 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_api.cpp", "ns3d_params.cpp"]
-# the mixed-precision path does not reproduce the reference's rounding, so
-# its translation unit may contract a*b+c into FMAs
-FMA_SOURCES = {"ns3d_mixed.hip"}
+# the mixed-precision path and the derivative kernels do not reproduce the
+# reference's rounding, so their translation units may contract a*b+c into FMAs
+FMA_SOURCES = {"ns3d_mixed.hip", "ns3d_adjoint.hip"}
 
 
 def _run(cmd, verbose):

@@ -269,6 +269,18 @@ __device__ __forceinline__ T recon_t(const DevParams &P, const T *s)
 
 // ------------------------------------------------------------ reverse rules
 
+// 1/x: the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r) (quadratic convergence; x > 0 and normal here).
+// The derivative code needs the value, not the reference's rounding.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // sbar[s] = gbar * d recon / d s[s]
 template <int RECON>
 __device__ __forceinline__ void recon_vjp(const DevParams &P, const double *s, double gbar, double *sbar)
@@ -280,26 +292,27 @@ __device__ __forceinline__ void recon_vjp(const DevParams &P, const double *s, d
         const double d0 = f0 - 2.0 * f1 + f2, e0 = f0 - 4.0 * f1 + 3.0 * f2;
         const double d1 = f1 - 2.0 * f2 + f3, e1 = f1 - f3;
         const double d2 = f2 - 2.0 * f3 + f4, e2 = 3.0 * f2 - 4.0 * f3 + f4;
-        const double b0 = (13.0 / 12.0) * d0 * d0 + 0.25 * e0 * e0;
-        const double b1 = (13.0 / 12.0) * d1 * d1 + 0.25 * e1 * e1;
-        const double b2 = (13.0 / 12.0) * d2 * d2 + 0.25 * e2 * e2;
-        const double t0 = eps + b0, t1 = eps + b1, t2 = eps + b2;
-        const double a0 = 0.1 / (t0 * t0), a1 = 0.6 / (t1 * t1), a2 = 0.3 / (t2 * t2);
-        const double as = a0 + a1 + a2;
-        const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
-        const double p0 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) / 6.0;
-        const double p1 = (-f1 + 5.0 * f2 + 2.0 * f3) / 6.0;
-        const double p2 = (2.0 * f2 + 5.0 * f3 - f4) / 6.0;
+        const double c13 = 13.0 / 12.0, c16 = 1.0 / 6.0;
+        const double it0 = rcp64(eps + c13 * d0 * d0 + 0.25 * e0 * e0);
+        const double it1 = rcp64(eps + c13 * d1 * d1 + 0.25 * e1 * e1);
+        const double it2 = rcp64(eps + c13 * d2 * d2 + 0.25 * e2 * e2);
+        const double a0 = 0.1 * it0 * it0, a1 = 0.6 * it1 * it1, a2 = 0.3 * it2 * it2;
+        const double ias = rcp64(a0 + a1 + a2);
+        const double w0 = a0 * ias, w1 = a1 * ias, w2 = a2 * ias;
+        const double p0 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) * c16;
+        const double p1 = (-f1 + 5.0 * f2 + 2.0 * f3) * c16;
+        const double p2 = (2.0 * f2 + 5.0 * f3 - f4) * c16;
         const double val = w0 * p0 + w1 * p1 + w2 * p2;
         // d val / d p_k = w_k ; d val / d a_k = (p_k - val) / as ;
-        // d a_k / d b_k = -2 a_k / t_k
-        const double ab0 = gbar * (p0 - val) / as * (-2.0 * a0 / t0);
-        const double ab1 = gbar * (p1 - val) / as * (-2.0 * a1 / t1);
-        const double ab2 = gbar * (p2 - val) / as * (-2.0 * a2 / t2);
+        // d a_k / d beta_k = -2 a_k / (eps + beta_k)
+        const double m2g = -2.0 * gbar;
+        const double ab0 = m2g * w0 * (p0 - val) * it0;
+        const double ab1 = m2g * w1 * (p1 - val) * it1;
+        const double ab2 = m2g * w2 * (p2 - val) * it2;
         const double gd0 = ab0 * (13.0 / 6.0) * d0, ge0 = ab0 * 0.5 * e0;
         const double gd1 = ab1 * (13.0 / 6.0) * d1, ge1 = ab1 * 0.5 * e1;
         const double gd2 = ab2 * (13.0 / 6.0) * d2, ge2 = ab2 * 0.5 * e2;
-        const double g0 = gbar * w0 / 6.0, g1 = gbar * w1 / 6.0, g2 = gbar * w2 / 6.0;
+        const double g0 = gbar * w0 * c16, g1 = gbar * w1 * c16, g2 = gbar * w2 * c16;
         sbar[0] = 2.0 * g0 + gd0 + ge0;
         sbar[1] = -7.0 * g0 - g1 - 2.0 * gd0 - 4.0 * ge0 + gd1 + ge1;
         sbar[2] = 11.0 * g0 + 5.0 * g1 + 2.0 * g2 + gd0 + 3.0 * ge0 - 2.0 * gd1 + gd2 + 3.0 * ge2;
@@ -312,6 +325,146 @@ __device__ __forceinline__ void recon_vjp(const DevParams &P, const double *s, d
 #pragma unroll
         for (int q = 0; q < S; ++q) sbar[q] = gbar * c[q];
     }
+}
+
+
+// ------------------------------------------- one flux component per lane
+
+// Component m of the split fluxes F+ and F- (split_state) without branching
+// on m: F_m = ga1 (S1 q_m + ga2 l2 r_m + S2 c s_m) for Steger-Warming with
+// q = (1,u,v,w,H), r = (1,u,v,w,V), s = (0,n,un); Lax-Friedrichs
+// F_m = (Fc_m +- a U_m) / 2 with Fc_m = U_m un + p s_m.
+template <int FVS, int DIR>
+__device__ __forceinline__ void split_comp(const DevParams &P, double rho, double u, double v, double w, double p,
+                                           int m, double &fp, double &fm)
+{
+    const double gamma = P.gamma;
+    const double irho = rcp64(rho);
+    const double V = 0.5 * (u * u + v * v + w * w);
+    const double c2 = gamma * p * irho;
+    const double c = sqrt(c2);
+    const double un = DIR == 0 ? u : (DIR == 1 ? v : w);
+    const double um = m == 1 ? u : (m == 2 ? v : w);
+    const double sm = m == 4 ? un : (m == DIR + 1 ? 1.0 : 0.0);
+    const double ig1 = 1.0 / (gamma - 1.0);   // uniform
+    if constexpr (FVS == FVS_SW) {
+        double l1p, l1m, l2p, l2m, l3p, l3m;
+        lam_split(un - c, P.sw_eps, l1p, l1m);
+        lam_split(un, P.sw_eps, l2p, l2m);
+        lam_split(un + c, P.sw_eps, l3p, l3m);
+        const double H = c2 * ig1 + V;
+        const double q = m == 0 ? 1.0 : (m == 4 ? H : um);
+        const double r = m == 0 ? 1.0 : (m == 4 ? V : um);
+        const double ga1 = rho * (0.5 / gamma), ga2 = 2.0 * (gamma - 1.0);
+        fp = ga1 * ((l1p + l3p) * q + ga2 * l2p * r + (l3p - l1p) * c * sm);
+        fm = ga1 * ((l1m + l3m) * q + ga2 * l2m * r + (l3m - l1m) * c * sm);
+    } else {
+        const double E = p * ig1 + rho * V;
+        const double Um = m == 0 ? rho : (m == 4 ? E : rho * um);
+        const double fc = Um * un + p * sm;
+        const double a = fabs(un) + c;
+        fp = 0.5 * (fc + a * Um);
+        fm = 0.5 * (fc - a * Um);
+    }
+}
+
+// d l+/d lambda and d l-/d lambda of lam_split (|x|' = sign(x), 0 at 0)
+__device__ __forceinline__ void lam_split_d(double lam, double eps, double &dp, double &dm)
+{
+    double t;
+    if (eps == 0.0) t = lam > 0.0 ? 1.0 : (lam < 0.0 ? -1.0 : 0.0);
+    else t = lam / sqrt(lam * lam + eps * eps);
+    dp = 0.5 * (1.0 + t);
+    dm = 0.5 * (1.0 - t);
+}
+
+// Reverse rule of split_comp: g[0..4] = d(bp F+_m + bm F-_m)/d(rho,u,v,w,p).
+template <int FVS, int DIR>
+__device__ __forceinline__ void split_comp_vjp(const DevParams &P, double rho, double u, double v, double w,
+                                               double p, int m, double bp, double bm, double g[5])
+{
+    const double gamma = P.gamma;
+    const double irho = rcp64(rho);
+    const double V = 0.5 * (u * u + v * v + w * w);
+    const double c2 = gamma * p * irho;
+    const double c = sqrt(c2);
+    const double un = DIR == 0 ? u : (DIR == 1 ? v : w);
+    const double um = m == 1 ? u : (m == 2 ? v : w);
+    const double sm = m == 4 ? un : (m == DIR + 1 ? 1.0 : 0.0);
+    const double ig1 = 1.0 / (gamma - 1.0);   // uniform
+    double rb = 0.0, pb = 0.0, cb = 0.0, unb = 0.0, Vb = 0.0, umb = 0.0;
+    if constexpr (FVS == FVS_SW) {
+        double l1p, l1m, l2p, l2m, l3p, l3m;
+        lam_split(un - c, P.sw_eps, l1p, l1m);
+        lam_split(un, P.sw_eps, l2p, l2m);
+        lam_split(un + c, P.sw_eps, l3p, l3m);
+        const double H = c2 * ig1 + V;
+        const double q = m == 0 ? 1.0 : (m == 4 ? H : um);
+        const double r = m == 0 ? 1.0 : (m == 4 ? V : um);
+        const double i2g = 0.5 / gamma;   // uniform
+        const double ga1 = rho * i2g, ga2 = 2.0 * (gamma - 1.0);
+        const double Pp = (l1p + l3p) * q + ga2 * l2p * r + (l3p - l1p) * c * sm;
+        const double Pm = (l1m + l3m) * q + ga2 * l2m * r + (l3m - l1m) * c * sm;
+        rb = (bp * Pp + bm * Pm) * i2g;
+        const double ap = bp * ga1, am = bm * ga1;
+        // through P+ and P-
+        const double S1b_p = ap * q, S2b_p = ap * c * sm, S1b_m = am * q, S2b_m = am * c * sm;
+        const double l2b_p = ap * ga2 * r, l2b_m = am * ga2 * r;
+        const double qb = ap * (l1p + l3p) + am * (l1m + l3m);
+        const double rbar = (ap * l2p + am * l2m) * ga2;
+        const double sb = ap * (l3p - l1p) * c + am * (l3m - l1m) * c;
+        cb += ap * (l3p - l1p) * sm + am * (l3m - l1m) * sm;
+        const double l1b_p = S1b_p - S2b_p, l3b_p = S1b_p + S2b_p;
+        const double l1b_m = S1b_m - S2b_m, l3b_m = S1b_m + S2b_m;
+        double d1p, d1m, d2p, d2m, d3p, d3m;
+        lam_split_d(un - c, P.sw_eps, d1p, d1m);
+        lam_split_d(un, P.sw_eps, d2p, d2m);
+        lam_split_d(un + c, P.sw_eps, d3p, d3m);
+        const double lam1b = l1b_p * d1p + l1b_m * d1m;
+        const double lam2b = l2b_p * d2p + l2b_m * d2m;
+        const double lam3b = l3b_p * d3p + l3b_m * d3m;
+        unb += lam1b + lam2b + lam3b;
+        cb += lam3b - lam1b;
+        // q = H (m = 4) or u_m (m = 1..3); r = V (m = 4) or u_m
+        if (m == 4) {
+            cb += qb * 2.0 * c * ig1;
+            Vb += qb + rbar;
+            unb += sb;
+        } else if (m > 0) {
+            umb += qb + rbar;
+        }
+    } else {
+        const double E = p * ig1 + rho * V;
+        const double Um = m == 0 ? rho : (m == 4 ? E : rho * um);
+        const double a = fabs(un) + c;
+        const double beta = 0.5 * (bp + bm), delta = 0.5 * (bp - bm);
+        const double Ub = beta * un + delta * a;
+        unb += beta * Um;
+        pb += beta * sm;
+        if (m == 4) unb += beta * p;
+        const double ab = delta * Um;
+        unb += ab * (un > 0.0 ? 1.0 : (un < 0.0 ? -1.0 : 0.0));
+        cb += ab;
+        if (m == 0) {
+            rb += Ub;
+        } else if (m == 4) {
+            pb += Ub * ig1;
+            rb += Ub * V;
+            Vb += Ub * rho;
+        } else {
+            rb += Ub * um;
+            umb += Ub * rho;
+        }
+    }
+    // c = sqrt(gamma p / rho): dc/dp = gamma / (2 rho c), dc/drho = -c / (2 rho)
+    const double hc = 0.5 * cb * rcp64(c);
+    pb += hc * gamma * irho;
+    rb -= hc * c2 * irho;
+    g[0] = rb;
+    g[1] = Vb * u + (m == 1 ? umb : 0.0) + (DIR == 0 ? unb : 0.0);
+    g[2] = Vb * v + (m == 2 ? umb : 0.0) + (DIR == 1 ? unb : 0.0);
+    g[3] = Vb * w + (m == 3 ? umb : 0.0) + (DIR == 2 ? unb : 0.0);
+    g[4] = pb;
 }
 
 }  // namespace ns3d

@@ -43,6 +43,47 @@ __device__ __forceinline__ double dmu_dT(const DevParams &P, double T, double mu
     return mu * (1.5 / T - 1.0 / (T + P.sutherland_s));
 }
 
+// Periodic ghost fill (face regions, cross directions over interior cells)
+// of up to 12 adjoint arrays, for the central-difference stencils.
+struct ArraySet {
+    double *p[12];
+    int n;
+};
+
+__global__ void ka_fill(DevParams P, ArraySet A)
+{
+    const int axis = blockIdx.y;
+    const int ng = P.ng;
+    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const int na = axis == 0 ? P.ny : P.nx;
+    const int nb = axis == 2 ? P.ny : P.nz;
+    const int total = 2 * ng * na * nb;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int g = t % (2 * ng);
+    const int rest = t / (2 * ng);
+    const int a = ng + rest % na;
+    const int b = ng + rest / na;
+    int q, qs;
+    if (g < ng) { q = g; qs = g + nd; }
+    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
+    int dst, src;
+    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
+    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
+    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+    for (int m = 0; m < A.n; ++m) A.p[m][dst] = A.p[m][src];
+}
+
+static void fill_set(hipStream_t s, const DevParams &P, const ArraySet &A)
+{
+    int m = P.ny * P.nz;
+    if (P.nx * P.nz > m) m = P.nx * P.nz;
+    if (P.nx * P.ny > m) m = P.nx * P.ny;
+    const int total = 2 * P.ng * m;
+    hipLaunchKernelGGL(ka_fill, dim3((total + 255) / 256, 3), dim3(256), 0, s, P, A);
+}
+
+
 // -------------------------------------------------------------- utilities
 
 // TLM: dW = C2P'(U) dU at every interior cell
@@ -202,104 +243,194 @@ __global__ void kt_vdiv(DevParams P, AdjArrays A)
 
 // --------------------------------------------------------------- adjoint
 
-// Inviscid adjoint along DIR: one thread per column segment of interior
-// cells [c0, c1).  It marches over every periodic face whose stencil holds
-// one of its cells, keeping the split fluxes of the S stencil cells and
-// their adjoint accumulators in registers; the reconstruction's reverse rule
-// spreads lambda_F over the stencil, and when a cell leaves the window its
-// accumulated split-flux adjoint goes through the split Jacobian (5-direction
-// dual numbers) into LW (rho, u, v, w, p).  Cells are owned by one thread:
-// no atomics.
+// Inviscid adjoint along DIR with one flux component per lane: lanes 5g..5g+4
+// of a wave march the same column (group g, 12 columns per wave), lane m
+// keeping the S-cell windows of F+_m, F-_m and of their adjoints.  Each cell
+// is owned by the segment that contains it; when a cell leaves the window,
+// every lane applies the reverse rule of its component of the split and the
+// 5 lanes of the group add their (rho,u,v,w,p) contributions through LDS.
+constexpr int kAdjCols = 12;
+
 template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(64) void ka_sweep(DevParams P, Fields F, AdjArrays A, int seg_len)
+__global__ __launch_bounds__(64) void ka_sweepc(DevParams P, Fields F, AdjArrays A, int seg_len)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
+    __shared__ double sh[kAdjCols + 1][5][5];
+    const int lane = threadIdx.x;
+    const int g = lane / 5;
+    const int m = lane - 5 * g;          // lanes 60..63: g = 12, m = 0..3
     const int na = DIR == 0 ? P.ny : P.nx;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= na) return;
+    const int a = blockIdx.x * kAdjCols + g;
+    const bool valid = g < kAdjCols && a < na;
+    const int ac = valid ? a : 0;        // inactive lanes read column 0
     const int b = blockIdx.y;
     const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
     const int c0 = P.ng + blockIdx.z * seg_len;
     const int c1 = min(c0 + seg_len, P.ng + n);
-    if (c0 >= c1) return;
+    if (c0 >= c1) return;                // uniform over the block
     int base, st;
-    if (DIR == 0) { base = ((P.ng + b) * P.sy + (P.ng + a)) * P.sx; st = 1; }
-    else if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + a); st = P.sx; }
-    else { base = (P.ng + b) * P.sx + (P.ng + a); st = P.sx * P.sy; }
+    if (DIR == 0) { base = ((P.ng + b) * P.sy + (P.ng + ac)) * P.sx; st = 1; }
+    else if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + ac); st = P.sx; }
+    else { base = (P.ng + b) * P.sx + (P.ng + ac); st = P.sx * P.sy; }
     const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
     auto cid = [&](int q) { return base + awrap(q, P.ng, n) * st; };
+    const double *Xm = A.X[m];
 
-    double fp[S][5], fm[S][5], fpb[S][5], fmb[S][5];
+    double fp[S], fm[S], fpb[S], fmb[S];
     const int fA = c0 - S + 1 + MID, fB = c1 - 1 + MID;
 #pragma unroll
     for (int s = 1; s < S; ++s) {
         const int id = cid(fA - 1 - MID + s);
-        split_t<FVS, DIR>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], fp[s], fm[s]);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) { fpb[s][m] = 0.0; fmb[s][m] = 0.0; }
+        split_comp<FVS, DIR>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], m, fp[s], fm[s]);
+        fpb[s] = 0.0;
+        fmb[s] = 0.0;
     }
-    double ll[5];
+    double ll = Xm[cid(fA)];
+    // loads of the next iteration are issued one face ahead
+    double nw[5], nxv, qw[5];
     {
-        const int id = cid(fA);
+        const int id = cid(fA - MID + S - 1);
+        const int iq = cid(fA - MID);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) ll[m] = A.X[m][id];
+        for (int t = 0; t < 5; ++t) { nw[t] = F.W[t][id]; qw[t] = F.W[t][iq]; }
+        nxv = Xm[cid(fA + 1)];
     }
+#pragma unroll 1
     for (int f = fA; f <= fB; ++f) {
 #pragma unroll
-        for (int s = 0; s < S - 1; ++s)
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m];
-                fpb[s][m] = fpb[s + 1][m]; fmb[s][m] = fmb[s + 1][m];
-            }
-        {
-            const int id = cid(f - MID + S - 1);
-            split_t<FVS, DIR>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], fp[S - 1], fm[S - 1]);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) { fpb[S - 1][m] = 0.0; fmb[S - 1][m] = 0.0; }
+        for (int s = 0; s < S - 1; ++s) {
+            fp[s] = fp[s + 1]; fm[s] = fm[s + 1];
+            fpb[s] = fpb[s + 1]; fmb[s] = fmb[s + 1];
         }
-        double gF[5];
+        const double w0 = nw[0], w1 = nw[1], w2 = nw[2], w3 = nw[3], w4 = nw[4];
+        const double lr = nxv;
         {
-            const int id = cid(f + 1);
+            const int id = cid(f + 1 - MID + S - 1);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const double lr = A.X[m][id];
-                gF[m] = (lr - ll[m]) * inv_d;
-                ll[m] = lr;
-            }
+            for (int t = 0; t < 5; ++t) nw[t] = F.W[t][id];
+            nxv = Xm[cid(f + 2)];
         }
+        split_comp<FVS, DIR>(P, w0, w1, w2, w3, w4, m, fp[S - 1], fm[S - 1]);
+        fpb[S - 1] = 0.0;
+        fmb[S - 1] = 0.0;
+        const double gF = (lr - ll) * inv_d;
+        ll = lr;
+        {
+            double sm[S], gb[S];
+            recon_vjp<RECON>(P, fp, gF, gb);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            double sp[S], sm[S], gb[S];
+            for (int s = 0; s < S; ++s) fpb[s] += gb[s];
 #pragma unroll
-            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[S - 1 - s] = fm[s][m]; }
-            recon_vjp<RECON>(P, sp, gF[m], gb);
+            for (int s = 0; s < S; ++s) sm[S - 1 - s] = fm[s];
+            recon_vjp<RECON>(P, sm, gF, gb);
 #pragma unroll
-            for (int s = 0; s < S; ++s) fpb[s][m] += gb[s];
-            recon_vjp<RECON>(P, sm, gF[m], gb);
-#pragma unroll
-            for (int s = 0; s < S; ++s) fmb[s][m] += gb[S - 1 - s];
+            for (int s = 0; s < S; ++s) fmb[s] += gb[S - 1 - s];
         }
         const int q = f - MID;
+        const double q0 = qw[0], q1 = qw[1], q2 = qw[2], q3 = qw[3], q4 = qw[4];
+        {
+            const int iq = cid(q + 1);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) qw[t] = F.W[t][iq];
+        }
         if (q >= c0 && q < c1) {
             const int id = base + q * st;
-            Dual<5> x[5], Fp[5], Fm[5];
+            double gv[5];
+            split_comp_vjp<FVS, DIR>(P, q0, q1, q2, q3, q4, m, fpb[0], fmb[0], gv);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                x[m].v = F.W[m][id];
-#pragma unroll
-                for (int t = 0; t < 5; ++t) x[m].d[t] = t == m ? 1.0 : 0.0;
+            for (int t = 0; t < 5; ++t) sh[g][m][t] = gv[t];
+            __syncthreads();
+            if (valid) {
+                const double sum = (sh[g][0][m] + sh[g][1][m]) + (sh[g][2][m] + sh[g][3][m]) + sh[g][4][m];
+                A.Y[m][id] += sum;
             }
-            split_t<FVS, DIR>(P, x[0], x[1], x[2], x[3], x[4], Fp, Fm);
-#pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < 5; ++m) s += fpb[0][m] * Fp[m].d[t] + fmb[0][m] * Fm[m].d[t];
-                A.Y[t][id] += s;
-            }
+            __syncthreads();
         }
+    }
+}
+
+// Inviscid adjoint along x with one block per x-row and one thread per cell
+// (store = 1 writes Y instead of adding to it)
+// (the row is periodic and held whole in LDS).  Per flux component: the
+// split values of the row go to LDS; thread f forms lambda_F of its face
+// (the face right of cell f) and the reverse rule of both reconstructions,
+// leaving the contribution to every stencil slot in LDS; thread q then sums
+// the S contributions that land on its cell.  After the 5 components the
+// reverse rule of the full split at the cell adds into Y.  Coalesced along x
+// and without recomputation; LDS (2 + 2S) * nx doubles.
+template <int RECON, int FVS>
+__global__ __launch_bounds__(1024) void ka_sweep_xrow(DevParams P, Fields F, AdjArrays A, int store)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    extern __shared__ double lds[];
+    const int nx = P.nx;
+    double *spl = lds;             // [2][nx]: F+_m, F-_m of the row
+    double *con = lds + 2 * nx;    // [2S][nx]: reverse contributions per slot
+    const int q = threadIdx.x;
+    const bool act = q < nx;
+    const int row = ((P.ng + blockIdx.y) * P.sy + (P.ng + blockIdx.x)) * P.sx + P.ng;
+    const int qr = act ? q : 0;
+    const int qn = qr + 1 == nx ? 0 : qr + 1;
+    double w[5], fp[5], fm[5], lp[5], lm[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) w[t] = F.W[t][row + qr];
+    split_t<FVS, 0>(P, w[0], w[1], w[2], w[3], w[4], fp, fm);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        if (act) {
+            spl[q] = fp[m];
+            spl[nx + q] = fm[m];
+        }
+        __syncthreads();
+        if (act) {
+            const double gF = (A.X[m][row + qn] - A.X[m][row + qr]) * P.idx;
+            double sp[S], sm[S], gb[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                int c = q - MID + s;
+                c = c < 0 ? c + nx : (c >= nx ? c - nx : c);
+                sp[s] = spl[c];
+                sm[S - 1 - s] = spl[nx + c];
+            }
+            recon_vjp<RECON>(P, sp, gF, gb);
+#pragma unroll
+            for (int s = 0; s < S; ++s) con[s * nx + q] = gb[s];
+            recon_vjp<RECON>(P, sm, gF, gb);
+#pragma unroll
+            for (int s = 0; s < S; ++s) con[(S + s) * nx + q] = gb[S - 1 - s];
+        }
+        __syncthreads();
+        if (act) {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                int f = q + MID - s;   // face whose slot s is this cell
+                f = f < 0 ? f + nx : (f >= nx ? f - nx : f);
+                a += con[s * nx + f];
+                b += con[(S + s) * nx + f];
+            }
+            lp[m] = a;
+            lm[m] = b;
+        }
+        __syncthreads();
+    }
+    if (!act) return;
+    double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        double gv[5];
+        split_comp_vjp<FVS, 0>(P, w[0], w[1], w[2], w[3], w[4], m, lp[m], lm[m], gv);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) g[t] += gv[t];
+    }
+    if (store) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) A.Y[t][row + q] = g[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) A.Y[t][row + q] += g[t];
     }
 }
 
@@ -342,10 +473,11 @@ __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays
         const double txx = mu * sxx, tyy = mu * syy, tzz = mu * szz, txy = mu * sxy_, txz = mu * sxz, tyz = mu * syz;
         double lv[12];
         const double h[3] = {P.dx, P.dy, P.dz};
+        const int strd[3] = {1, P.sx, sxy};
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) lv[4 * d + m] = -dwrap<VIS>(P, A.X[m + 1], i, j, k, d, h[d]);
+            for (int m = 0; m < 4; ++m) lv[4 * d + m] = -diff_central<VIS>(A.X[m + 1] + id, strd[d], h[d]);
         const double Ex = lv[3], Ey = lv[7], Ez = lv[11];
         const double btxx = lv[0] + u * Ex;
         const double btxy = lv[1] + lv[4] + v * Ex + u * Ey;
@@ -407,9 +539,18 @@ __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays
         partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// gradient adjoint: LW += D^T LG (= -D LG, periodic) for u, v, w, T
+// RK3 reverse combination applied to lambda_U = C2P'(U)^T LW:
+//   acc = (init ? 0 : acc) + a * L_out
+//   out = bx * L_out + bl * lambda_U + (add_acc ? acc : 0)
+struct Combine {
+    double a, bx, bl;
+    int init, add_acc;
+};
+
+// Gradient adjoint LW += D^T LG (= -D LG on periodic arrays) for u, v, w, T,
+// which completes LW at the cell, then C2P'^T and the RK3 combination.
 template <int VIS>
-__global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays A)
+__global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays A, Combine cb)
 {
     const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
@@ -417,47 +558,35 @@ __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays 
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int id = (k * P.sy + j) * P.sx + i;
     const double h[3] = {P.dx, P.dy, P.dz};
+    const int strd[3] = {1, P.sx, P.sx * P.sy};
     double lq[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s -= dwrap<VIS>(P, A.G[3 * q + d], i, j, k, d, h[d]);
+        for (int d = 0; d < 3; ++d) s -= diff_central<VIS>(A.G[3 * q + d] + id, strd[d], h[d]);
         lq[q] = s;
     }
-    const double rho = F.W[0][id], T = F.W[5][id];
-    A.Y[0][id] += -lq[3] * T / rho;
-    A.Y[1][id] += lq[0];
-    A.Y[2][id] += lq[1];
-    A.Y[3][id] += lq[2];
-    A.Y[4][id] += lq[3] / (rho * P.Rgas);
-}
-
-// lambda_U = C2P'(U)^T LW, then the RK3 reverse combination:
-//   acc = (init ? 0 : acc) + a * L_out
-//   out = bx * L_out + bl * lambda_U + (add_acc ? acc : 0)
-__global__ void ka_combine(DevParams P, Fields F, AdjArrays A, double a, double bx, double bl, int init, int add_acc)
-{
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
-    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
-    const int id = (k * P.sy + j) * P.sx + i;
-    const double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id];
+    const double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], T = F.W[5][id];
+    const double irho = 1.0 / rho;
+    const double br = A.Y[0][id] - lq[3] * T * irho;
+    const double bu = A.Y[1][id] + lq[0];
+    const double bv = A.Y[2][id] + lq[1];
+    const double bw = A.Y[3][id] + lq[2];
+    const double bp = A.Y[4][id] + lq[3] * irho / P.Rgas;
     const double g1 = P.gamma - 1.0;
-    const double br = A.Y[0][id], bu = A.Y[1][id], bv = A.Y[2][id], bw = A.Y[3][id], bp = A.Y[4][id];
     double lu[5];
-    lu[0] = br - (u * bu + v * bv + w * bw) / rho + g1 * 0.5 * (u * u + v * v + w * w) * bp;
-    lu[1] = bu / rho - g1 * u * bp;
-    lu[2] = bv / rho - g1 * v * bp;
-    lu[3] = bw / rho - g1 * w * bp;
+    lu[0] = br - (u * bu + v * bv + w * bw) * irho + g1 * 0.5 * (u * u + v * v + w * w) * bp;
+    lu[1] = bu * irho - g1 * u * bp;
+    lu[2] = bv * irho - g1 * v * bp;
+    lu[3] = bw * irho - g1 * w * bp;
     lu[4] = g1 * bp;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const double lo = A.X[m][id];
-        const double acc = (init ? 0.0 : A.C[m][id]) + a * lo;
+        const double acc = (cb.init ? 0.0 : A.C[m][id]) + cb.a * lo;
         A.C[m][id] = acc;
-        A.Z[m][id] = bx * lo + bl * lu[m] + (add_acc ? acc : 0.0);
+        A.Z[m][id] = cb.bx * lo + cb.bl * lu[m] + (cb.add_acc ? acc : 0.0);
     }
 }
 
@@ -515,6 +644,21 @@ __global__ __launch_bounds__(256) void k_dissip(DevParams P, Fields F, double *p
 static inline dim3 agrid(const DevParams &P) { return dim3((P.nx + 63) / 64, (P.ny + 3) / 4, P.nz); }
 static const dim3 kABlock(64, 4, 1);
 
+// column segments of the adjoint sweeps: about 8192 waves in flight over
+// the device, segments no shorter than 16 cells
+int ad_seg(const DevParams &P, int dir)
+{
+    const int na = dir == 0 ? P.ny : P.nx;
+    const int nb = dir == 2 ? P.ny : P.nz;
+    const int nd = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
+    const long long cols = (long long)((na + kAdjCols - 1) / kAdjCols) * nb;
+    long long nseg = (8192 + cols - 1) / cols;
+    const long long maxseg = nd / 16 > 0 ? nd / 16 : 1;
+    if (nseg > maxseg) nseg = maxseg;
+    if (nseg < 1) nseg = 1;
+    return (int)((nd + nseg - 1) / nseg);
+}
+
 int ad_blocks(const DevParams &P) { return ((P.nx + 63) / 64) * ((P.ny + 3) / 4) * P.nz; }
 
 bool ad_supported(const DevParams &P)
@@ -571,16 +715,31 @@ hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, co
     return hipGetLastError();
 }
 
+static size_t xrow_lds(const DevParams &P)
+{
+    return (size_t)(2 + 2 * Stencil<R_UD7>::S) * P.nx * sizeof(double);
+}
+
+// the x adjoint sweep holds whole rows in LDS (and then stores Y)
+bool adj_xrow(const DevParams &P) { return P.nx <= 1024 && xrow_lds(P) <= 65536; }
+
 template <int RECON, int FVS>
 static void adj_sweep_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg)
 {
+    if (dir == 0 && adj_xrow(P)) {
+        constexpr int S = Stencil<RECON>::S;
+        const size_t lds = (size_t)(2 + 2 * S) * P.nx * sizeof(double);
+        const int nt = (P.nx + 63) / 64 * 64;
+        hipLaunchKernelGGL((ka_sweep_xrow<RECON, FVS>), dim3(P.ny, P.nz), dim3(nt), lds, s, P, F, A, 1);
+        return;
+    }
     const int na = dir == 0 ? P.ny : P.nx;
     const int nb = dir == 2 ? P.ny : P.nz;
     const int nd = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
-    dim3 g((na + 63) / 64, nb, (nd + seg - 1) / seg);
-    if (dir == 0) hipLaunchKernelGGL((ka_sweep<0, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
-    else if (dir == 1) hipLaunchKernelGGL((ka_sweep<1, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
-    else hipLaunchKernelGGL((ka_sweep<2, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
+    dim3 g((na + kAdjCols - 1) / kAdjCols, nb, (nd + seg - 1) / seg);
+    if (dir == 0) hipLaunchKernelGGL((ka_sweepc<0, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
+    else if (dir == 1) hipLaunchKernelGGL((ka_sweepc<1, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
+    else hipLaunchKernelGGL((ka_sweepc<2, RECON, FVS>), g, dim3(64), 0, s, P, F, A, seg);
 }
 
 template <int FVS>
@@ -597,34 +756,40 @@ static void adj_sweep_f(hipStream_t s, const DevParams &P, const Fields &F, cons
 
 template <int VIS>
 static void adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                     double *partial)
+                     double *partial, const Combine &cb)
 {
+    ArraySet xs{{A.X[1], A.X[2], A.X[3], A.X[4]}, 4};
+    fill_set(s, P, xs);
     hipLaunchKernelGGL(ka_vflux<VIS>, agrid(P), kABlock, 0, s, P, F, A, w_obj, partial);
-    hipLaunchKernelGGL(ka_grad<VIS>, agrid(P), kABlock, 0, s, P, F, A);
+    ArraySet gs{};
+    for (int m = 0; m < 12; ++m) gs.p[m] = A.G[m];
+    gs.n = 12;
+    fill_set(s, P, gs);
+    hipLaunchKernelGGL(ka_grad<VIS>, agrid(P), kABlock, 0, s, P, F, A, cb);
 }
 
-// Y <- W-adjoint of X^T R(U) (+ w_obj * the dissipation sum), assuming Y = 0
-// on entry; block partials of d/d(1/Re) into `partial`
-hipError_t launch_adj_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                          double *partial, int seg)
+// Y += W-adjoint of X . R_inv(U) along `dir` (Y = 0 before the first call)
+hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg)
 {
     if (!ad_supported(P)) return hipErrorInvalidValue;
-    for (int d = 0; d < 3; ++d) {
-        if (P.fvs == FVS_SW) adj_sweep_f<FVS_SW>(s, P, F, A, d, seg);
-        else adj_sweep_f<FVS_LF>(s, P, F, A, d, seg);
-    }
-    switch (P.vis) {
-    case VIS_C8: adj_visc<VIS_C8>(s, P, F, A, w_obj, partial); break;
-    case VIS_C6: adj_visc<VIS_C6>(s, P, F, A, w_obj, partial); break;
-    default: adj_visc<VIS_C4>(s, P, F, A, w_obj, partial); break;
-    }
+    if (P.fvs == FVS_SW) adj_sweep_f<FVS_SW>(s, P, F, A, dir, seg);
+    else adj_sweep_f<FVS_LF>(s, P, F, A, dir, seg);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_combine(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,
-                              double bx, double bl, int init, int add_acc)
+// Viscous part of the adjoint RHS, which completes LW = Y, then
+// Z = bx X + bl C2P'^T LW (+ the accumulator C, updated with a X); block
+// partials of the 1/Re derivative of X . R_vis + w_obj * sum Phi into
+// `partial`.
+hipError_t launch_adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                           double *partial, double a, double bx, double bl, int init, int add_acc)
 {
-    hipLaunchKernelGGL(ka_combine, agrid(P), kABlock, 0, s, P, F, A, a, bx, bl, init, add_acc);
+    const Combine cb{a, bx, bl, init, add_acc};
+    switch (P.vis) {
+    case VIS_C8: adj_visc<VIS_C8>(s, P, F, A, w_obj, partial, cb); break;
+    case VIS_C6: adj_visc<VIS_C6>(s, P, F, A, w_obj, partial, cb); break;
+    default: adj_visc<VIS_C4>(s, P, F, A, w_obj, partial, cb); break;
+    }
     return hipGetLastError();
 }
 

@@ -62,6 +62,7 @@ struct ns3d_ctx {
     int fw_steps = 0, fw_ckpt = 0;
     double fw_dt = 0.0;
     bool fw_valid = false;          // a forward pass matches the current state
+    bool fw_last_valid = false;     // stage inputs of the last segment are stored
 };
 
 namespace {
@@ -355,13 +356,18 @@ double compensated_sum(const std::vector<double> &h)
     return s;
 }
 
-// Y <- W-adjoint of one RHS at the state of `F`, sum of the 1/Re partials
-// into adj_sums[slot]
-int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot)
+// One adjoint RHS at the state of `F` with cotangent X, finished by the RK3
+// reverse combination Z = bx X + bl C2P'^T LW (+ acc); the sum of the 1/Re
+// partials goes to adj_sums[slot]
+int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot, double a, double bx, double bl,
+                    int init, int add_acc)
 {
-    HIPCALL(c, hipMemsetAsync(c->adj.Y[0], 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
-    const int seg = c->seg_len[1];
-    TIMED(c, "adj_rhs", launch_adj_rhs(c->stream, c->dp, F, c->adj, w_obj, c->adj_part, seg));
+    if (!adj_xrow(c->dp))
+        HIPCALL(c, hipMemsetAsync(c->adj.Y[0], 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
+    static const char *sname[3] = {"adj_sweep_x", "adj_sweep_y", "adj_sweep_z"};
+    for (int d = 0; d < 3; ++d)
+        TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d)));
+    TIMED(c, "adj_visc", launch_adj_visc(c->stream, c->dp, F, c->adj, w_obj, c->adj_part, a, bx, bl, init, add_acc));
     TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), c->adj_sums + slot));
     return NS3D_OK;
 }
@@ -886,9 +892,8 @@ int ns3d_adj_rhs(ns3d_ctx *c, const double *const lam[5], double w_obj, double *
     if (!st) st = ensure_adj(c);
     if (!st) st = ensure_sums(c, 1);
     if (!st) st = upload_set(c, c->adj.X, lam);
-    if (!st) st = adj_rhs_enqueue(c, c->f, w_obj, 0);
+    if (!st) st = adj_rhs_enqueue(c, c->f, w_obj, 0, 0.0, 0.0, 1.0, 1, 0);
     if (st) return st;
-    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, c->f, c->adj, 0.0, 0.0, 1.0, 1, 0));
     st = download_set(c, out, c->adj.Z);
     if (st) return st;
     double h = 0.0;
@@ -934,6 +939,10 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     if (st) return st;
     // forward: J terms into adj_sums[3*nsteps + n]
     double *jsum = c->adj_sums + 3LL * nsteps;
+    // the last segment's stage inputs are kept from this pass, so the
+    // reverse pass starts without recomputing it
+    const long long snap0 = ncp;
+    const int last0 = (int)((ncp - 1) * ckpt_every);
     for (int n = 0; n < nsteps; ++n) {
         if (n % ckpt_every == 0) {
             st = copy_state(c, ck_state(c, n / ckpt_every), c->f.W[0]);
@@ -941,8 +950,18 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
         }
         TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, c->f, c->adj_part));
         TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), jsum + n));
-        st = rk3_enqueue(c, dt);
-        if (st) return st;
+        if (n >= last0) {
+            for (int stage = 1; stage <= 3; ++stage) {
+                st = copy_state(c, ck_state(c, snap0 + 3LL * (n - last0) + stage - 1), c->f.W[0]);
+                if (!st) st = rhs_pass(c, stage, dt);
+                if (!st) st = fill_ghosts(c);
+                if (st) return st;
+            }
+            c->steps_done++;
+        } else {
+            st = rk3_enqueue(c, dt);
+            if (st) return st;
+        }
     }
     st = copy_state(c, ck_state(c, c->ck_states - 1), c->f.W[0]);
     if (st) return st;
@@ -957,6 +976,7 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     c->fw_ckpt = ckpt_every;
     c->fw_dt = dt;
     c->fw_valid = true;
+    c->fw_last_valid = true;
     return NS3D_OK;
 }
 
@@ -976,10 +996,15 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
     const long long snap0 = (N + K - 1) / K;   // first stage-snapshot slot
     for (int seg = (N - 1) / K; seg >= 0; --seg) {
         const int n0 = seg * K, n1 = n0 + K < N ? n0 + K : N;
+        int n1r;
         // recompute the segment from its checkpoint, keeping every stage input
+        // (the last segment's inputs are still there from the forward pass)
         st = copy_state(c, c->f.W[0], ck_state(c, seg));
         if (st) return st;
-        for (int n = n0; n < n1; ++n)
+        if (seg == (N - 1) / K && c->fw_last_valid) n1r = n0;
+        else n1r = n1;
+        c->fw_last_valid = false;
+        for (int n = n0; n < n1r; ++n)
             for (int stage = 1; stage <= 3; ++stage) {
                 st = copy_state(c, ck_state(c, snap0 + 3LL * (n - n0) + stage - 1), c->f.W[0]);
                 if (!st) st = rhs_pass(c, stage, dt);
@@ -991,14 +1016,11 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
                 Fields F = c->f;
                 double *W = ck_state(c, snap0 + 3LL * (n - n0) + stage - 1);
                 for (int m = 0; m < 6; ++m) F.W[m] = W + m * c->ncell;
-                st = adj_rhs_enqueue(c, F, stage == 1 ? w_obj : 0.0, 3LL * n + stage - 1);
+                const long long slot = 3LL * n + stage - 1;
+                if (stage == 3) st = adj_rhs_enqueue(c, F, 0.0, slot, a3, b3, b3 * dt, 1, 0);
+                else if (stage == 2) st = adj_rhs_enqueue(c, F, 0.0, slot, a2, b2, b2 * dt, 0, 0);
+                else st = adj_rhs_enqueue(c, F, w_obj, slot, 0.0, 1.0, dt, 0, 1);
                 if (st) return st;
-                if (stage == 3)
-                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, a3, b3, b3 * dt, 1, 0));
-                else if (stage == 2)
-                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, a2, b2, b2 * dt, 0, 0));
-                else
-                    TIMED(c, "adj_combine", launch_adj_combine(c->stream, c->dp, F, c->adj, 0.0, 1.0, dt, 0, 1));
                 for (int m = 0; m < 5; ++m) std::swap(c->adj.X[m], c->adj.Z[m]);
             }
         }

@@ -85,11 +85,13 @@ hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double 
 // tangent-linear / adjoint RHS (ns3d_adjoint.hip)
 bool ad_supported(const DevParams &P);
 int ad_blocks(const DevParams &P);
+int ad_seg(const DevParams &P, int dir);
 hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A);
-hipError_t launch_adj_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                          double *partial, int seg);
-hipError_t launch_adj_combine(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,
-                              double bx, double bl, int init, int add_acc);
+hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir,
+                            int seg);
+hipError_t launch_adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                           double *partial, double a, double bx, double bl, int init, int add_acc);
+bool adj_xrow(const DevParams &P);
 hipError_t launch_dissip(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 hipError_t launch_sum_partials(hipStream_t s, const double *partial, int n, double *out);
 

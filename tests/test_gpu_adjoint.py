@@ -314,3 +314,38 @@ def test_unsupported_settings(gpu):
     s = solver(n, prims, Ma)
     with pytest.raises(ns.NS3DError):
         s.adj_backward()   # no forward pass recorded
+
+
+@pytest.mark.parametrize("kind", ["smooth", "white"])
+def test_gradient_fd_32_isotropic_turbulence(gpu, kind):
+    """The adjoint workload's setting (HIT k0=4, Mt=0.3, Re=100, WENO5 +
+    Steger-Warming, Sutherland) at 32^3 over 10 steps, along a seeded random
+    direction: adjoint gradient against a fourth-order central difference of
+    J.  The split carries sw_eps = 0.05 so that J is smooth at the step sizes
+    used (with the plain |lambda| a few of the 10^7 eigenvalue evaluations
+    cross zero inside the difference stencil)."""
+    n, nsteps = 32, 10
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.05)
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=77)
+    s = solver(n, st, Ma, **cfg)
+    dt = s.timestep()
+    U = [s._full(a) for a in conserved_of(st)]
+    s.upload_conserved(*U)
+    J = s.adj_forward(nsteps, dt, 4)
+    lam0, _ = s.adj_backward()
+    scale = [0.01, 0.01, 0.01, 0.01, 0.2]
+    if kind == "smooth":
+        dU = [s._full(a) for a in smooth_dir(n, 123, scale)]
+    else:
+        rng = np.random.default_rng(123)
+        dU = [s._full(c * rng.normal(size=(n, n, n))) for c in scale]
+    h = 1e-4
+    val = {}
+    for m in (-2, -1, 1, 2):
+        s2 = solver(n, st, Ma, **cfg)
+        s2.upload_conserved(*[a + m * h * d for a, d in zip(U, dU)])
+        val[m] = s2.adj_forward(nsteps, dt, nsteps)
+    fd = (8.0 * (val[1] - val[-1]) - (val[2] - val[-2])) / (12.0 * h)
+    ad = interior_dot(s, lam0, dU)
+    print(kind, "J", J, "fd", fd, "adjoint", ad, "rel", abs(fd - ad) / abs(fd))
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
