@@ -350,87 +350,106 @@ __global__ __launch_bounds__(64) void ka_sweepc(DevParams P, Fields F, AdjArrays
     }
 }
 
-// Inviscid adjoint along x with one block per x-row and one thread per cell
-// (store = 1 writes Y instead of adding to it)
-// (the row is periodic and held whole in LDS).  Per flux component: the
-// split values of the row go to LDS; thread f forms lambda_F of its face
-// (the face right of cell f) and the reverse rule of both reconstructions,
-// leaving the contribution to every stencil slot in LDS; thread q then sums
-// the S contributions that land on its cell.  After the 5 components the
-// reverse rule of the full split at the cell adds into Y.  Coalesced along x
-// and without recomputation; LDS (2 + 2S) * nx doubles.
-template <int RECON, int FVS>
-__global__ __launch_bounds__(1024) void ka_sweep_xrow(DevParams P, Fields F, AdjArrays A, int store)
+// Inviscid adjoint along DIR with whole periodic lines in LDS: a block holds
+// TX = blockDim.x neighbouring lines (x-columns for DIR = 1, 2; a single
+// x-row for DIR = 0) and one thread per cell, threadIdx.y along the line.
+// Per flux component and side: the split values of the lines go to LDS;
+// thread f forms lambda_F of its face (the face right of cell f) and the
+// reverse rule of the reconstruction, leaving the contribution to every
+// stencil slot in LDS; thread q then sums the S contributions that land on
+// its cell.  After the 5 components the reverse rule of the full split at the
+// cell adds into Y (store = 1 writes Y instead).  Loads and stores are
+// coalesced over the TX lines, nothing is recomputed, and LDS holds
+// (1 + S) * n * TX doubles.
+template <int DIR, int RECON, int FVS>
+__global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, AdjArrays A, int store)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
     extern __shared__ double lds[];
-    const int nx = P.nx;
-    double *spl = lds;             // [2][nx]: F+_m, F-_m of the row
-    double *con = lds + 2 * nx;    // [2S][nx]: reverse contributions per slot
-    const int q = threadIdx.x;
-    const bool act = q < nx;
-    const int row = ((P.ng + blockIdx.y) * P.sy + (P.ng + blockIdx.x)) * P.sx + P.ng;
-    const int qr = act ? q : 0;
-    const int qn = qr + 1 == nx ? 0 : qr + 1;
+    const int TX = blockDim.x;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const int tx = threadIdx.x, q = threadIdx.y;
+    const int L = n * TX;                // one LDS plane
+    double *spl = lds;                   // [n][TX] split values of one side
+    double *con = lds + L;               // [S][n][TX] reverse contributions
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    int base, st, ncols;
+    if (DIR == 0) {
+        base = ((P.ng + by) * P.sy + (P.ng + bx)) * P.sx + P.ng;
+        st = 1;
+        ncols = 1;
+    } else if (DIR == 1) {
+        base = ((P.ng + by) * P.sy + P.ng) * P.sx + P.ng + bx * TX;
+        st = P.sx;
+        ncols = P.nx - bx * TX;
+    } else {
+        base = (P.ng * P.sy + (P.ng + by)) * P.sx + P.ng + bx * TX;
+        st = P.sx * P.sy;
+        ncols = P.nx - bx * TX;
+    }
+    const bool act = q < n && tx < ncols;
+    const int qr = act ? q : 0, txr = act ? tx : 0;
+    const int qn = qr + 1 == n ? 0 : qr + 1;
+    const int id = base + txr + qr * st;
+    const int idn = base + txr + qn * st;
+    const int me = q * TX + tx;
     double w[5], fp[5], fm[5], lp[5], lm[5];
 #pragma unroll
-    for (int t = 0; t < 5; ++t) w[t] = F.W[t][row + qr];
-    split_t<FVS, 0>(P, w[0], w[1], w[2], w[3], w[4], fp, fm);
+    for (int t = 0; t < 5; ++t) w[t] = F.W[t][id];
+    split_t<FVS, DIR>(P, w[0], w[1], w[2], w[3], w[4], fp, fm);
+    const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        if (act) {
-            spl[q] = fp[m];
-            spl[nx + q] = fm[m];
-        }
-        __syncthreads();
-        if (act) {
-            const double gF = (A.X[m][row + qn] - A.X[m][row + qr]) * P.idx;
-            double sp[S], sm[S], gb[S];
+        const double gF = (A.X[m][idn] - A.X[m][id]) * inv_d;
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                int c = q - MID + s;
-                c = c < 0 ? c + nx : (c >= nx ? c - nx : c);
-                sp[s] = spl[c];
-                sm[S - 1 - s] = spl[nx + c];
+        for (int side = 0; side < 2; ++side) {
+            if (act) spl[me] = side == 0 ? fp[m] : fm[m];
+            __syncthreads();
+            if (act) {
+                double sv[S], gb[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    int c = q - MID + s;
+                    c = c < 0 ? c + n : (c >= n ? c - n : c);
+                    // the minus side reads its stencil reversed
+                    sv[side == 0 ? s : S - 1 - s] = spl[c * TX + tx];
+                }
+                recon_vjp<RECON>(P, sv, gF, gb);
+#pragma unroll
+                for (int s = 0; s < S; ++s) con[s * L + me] = gb[side == 0 ? s : S - 1 - s];
             }
-            recon_vjp<RECON>(P, sp, gF, gb);
+            __syncthreads();
+            if (act) {
+                double a = 0.0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) con[s * nx + q] = gb[s];
-            recon_vjp<RECON>(P, sm, gF, gb);
-#pragma unroll
-            for (int s = 0; s < S; ++s) con[(S + s) * nx + q] = gb[S - 1 - s];
-        }
-        __syncthreads();
-        if (act) {
-            double a = 0.0, b = 0.0;
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                int f = q + MID - s;   // face whose slot s is this cell
-                f = f < 0 ? f + nx : (f >= nx ? f - nx : f);
-                a += con[s * nx + f];
-                b += con[(S + s) * nx + f];
+                for (int s = 0; s < S; ++s) {
+                    int f = q + MID - s;   // face whose slot s is this cell
+                    f = f < 0 ? f + n : (f >= n ? f - n : f);
+                    a += con[s * L + f * TX + tx];
+                }
+                if (side == 0) lp[m] = a;
+                else lm[m] = a;
             }
-            lp[m] = a;
-            lm[m] = b;
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (!act) return;
     double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         double gv[5];
-        split_comp_vjp<FVS, 0>(P, w[0], w[1], w[2], w[3], w[4], m, lp[m], lm[m], gv);
+        split_comp_vjp<FVS, DIR>(P, w[0], w[1], w[2], w[3], w[4], m, lp[m], lm[m], gv);
 #pragma unroll
         for (int t = 0; t < 5; ++t) g[t] += gv[t];
     }
     if (store) {
 #pragma unroll
-        for (int t = 0; t < 5; ++t) A.Y[t][row + q] = g[t];
+        for (int t = 0; t < 5; ++t) A.Y[t][id] = g[t];
     } else {
 #pragma unroll
-        for (int t = 0; t < 5; ++t) A.Y[t][row + q] += g[t];
+        for (int t = 0; t < 5; ++t) A.Y[t][id] += g[t];
     }
 }
 
@@ -444,9 +463,11 @@ template <int VIS>
 __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays A, double w_obj, double *partial)
 {
     __shared__ double red[4];
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + by * blockDim.y + threadIdx.y;
+    const int k = P.ng + bz;
     double contrib = 0.0;
     if (i < P.ng + P.nx && j < P.ng + P.ny) {
         const int id = (k * P.sy + j) * P.sx + i;
@@ -552,9 +573,11 @@ struct Combine {
 template <int VIS>
 __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays A, Combine cb)
 {
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + by * blockDim.y + threadIdx.y;
+    const int k = P.ng + bz;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int id = (k * P.sy + j) * P.sx + i;
     const double h[3] = {P.dx, P.dy, P.dz};
@@ -608,9 +631,11 @@ template <int VIS>
 __global__ __launch_bounds__(256) void k_dissip(DevParams P, Fields F, double *partial)
 {
     __shared__ double red[4];
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + by * blockDim.y + threadIdx.y;
+    const int k = P.ng + bz;
     double phi = 0.0;
     if (i < P.ng + P.nx && j < P.ng + P.ny) {
         const int id = (k * P.sy + j) * P.sx + i;
@@ -715,22 +740,41 @@ hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, co
     return hipGetLastError();
 }
 
-static size_t xrow_lds(const DevParams &P)
+// Lines per block of the whole-line adjoint sweep along `dir` (0: the sweep
+// falls back to the per-component marching kernel): as many neighbouring
+// x-columns as fit 1024 threads and 64 KiB of LDS for the widest stencil.
+int adj_line_tx(const DevParams &P, int dir)
 {
-    return (size_t)(2 + 2 * Stencil<R_UD7>::S) * P.nx * sizeof(double);
+    const int n = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
+    const long long per_line = (long long)(1 + Stencil<R_UD7>::S) * n * sizeof(double);
+    if (!P.adj_lines || n > 1024 || per_line > 65536) return 0;
+    if (dir == 0) return 1;
+    int tx = 16;
+    while (tx > 1 && ((long long)tx * n > 1024 || tx * per_line > 65536)) tx >>= 1;
+    return tx;
 }
 
-// the x adjoint sweep holds whole rows in LDS (and then stores Y)
-bool adj_xrow(const DevParams &P) { return P.nx <= 1024 && xrow_lds(P) <= 65536; }
+// the x adjoint sweep runs first and stores Y when it holds whole rows
+bool adj_xrow(const DevParams &P) { return adj_line_tx(P, 0) > 0; }
 
 template <int RECON, int FVS>
 static void adj_sweep_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg)
 {
-    if (dir == 0 && adj_xrow(P)) {
+    const int tx = adj_line_tx(P, dir);
+    if (tx > 0) {
         constexpr int S = Stencil<RECON>::S;
-        const size_t lds = (size_t)(2 + 2 * S) * P.nx * sizeof(double);
-        const int nt = (P.nx + 63) / 64 * 64;
-        hipLaunchKernelGGL((ka_sweep_xrow<RECON, FVS>), dim3(P.ny, P.nz), dim3(nt), lds, s, P, F, A, 1);
+        const int n = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
+        const size_t lds = (size_t)(1 + S) * n * tx * sizeof(double);
+        const dim3 blk(tx, n);
+        const int store = dir == 0 ? 1 : 0;
+        if (dir == 0)
+            hipLaunchKernelGGL((ka_sweep_line<0, RECON, FVS>), dim3(P.ny, P.nz), blk, lds, s, P, F, A, store);
+        else if (dir == 1)
+            hipLaunchKernelGGL((ka_sweep_line<1, RECON, FVS>), dim3((P.nx + tx - 1) / tx, P.nz), blk, lds, s, P, F,
+                               A, store);
+        else
+            hipLaunchKernelGGL((ka_sweep_line<2, RECON, FVS>), dim3((P.nx + tx - 1) / tx, P.ny), blk, lds, s, P, F,
+                               A, store);
         return;
     }
     const int na = dir == 0 ? P.ny : P.nx;
@@ -755,12 +799,17 @@ static void adj_sweep_f(hipStream_t s, const DevParams &P, const Fields &F, cons
 }
 
 template <int VIS>
-static void adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                     double *partial, const Combine &cb)
+static void adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                      double *partial)
 {
     ArraySet xs{{A.X[1], A.X[2], A.X[3], A.X[4]}, 4};
     fill_set(s, P, xs);
     hipLaunchKernelGGL(ka_vflux<VIS>, agrid(P), kABlock, 0, s, P, F, A, w_obj, partial);
+}
+
+template <int VIS>
+static void adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const Combine &cb)
+{
     ArraySet gs{};
     for (int m = 0; m < 12; ++m) gs.p[m] = A.G[m];
     gs.n = 12;
@@ -777,18 +826,30 @@ hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, 
     return hipGetLastError();
 }
 
-// Viscous part of the adjoint RHS, which completes LW = Y, then
-// Z = bx X + bl C2P'^T LW (+ the accumulator C, updated with a X); block
-// partials of the 1/Re derivative of X . R_vis + w_obj * sum Phi into
-// `partial`.
-hipError_t launch_adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                           double *partial, double a, double bx, double bl, int init, int add_acc)
+// Viscous flux part of the adjoint RHS: LW += the reverse rule of the
+// viscous flux at X, LG <- gradient adjoints; block partials of the 1/Re
+// derivative of X . R_vis + w_obj * sum Phi into `partial`.
+hipError_t launch_adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                            double *partial)
+{
+    switch (P.vis) {
+    case VIS_C8: adj_vflux<VIS_C8>(s, P, F, A, w_obj, partial); break;
+    case VIS_C6: adj_vflux<VIS_C6>(s, P, F, A, w_obj, partial); break;
+    default: adj_vflux<VIS_C4>(s, P, F, A, w_obj, partial); break;
+    }
+    return hipGetLastError();
+}
+
+// Gradient part, which completes LW, then Z = bx X + bl C2P'^T LW (+ the
+// accumulator C, updated with a X).
+hipError_t launch_adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,
+                           double bx, double bl, int init, int add_acc)
 {
     const Combine cb{a, bx, bl, init, add_acc};
     switch (P.vis) {
-    case VIS_C8: adj_visc<VIS_C8>(s, P, F, A, w_obj, partial, cb); break;
-    case VIS_C6: adj_visc<VIS_C6>(s, P, F, A, w_obj, partial, cb); break;
-    default: adj_visc<VIS_C4>(s, P, F, A, w_obj, partial, cb); break;
+    case VIS_C8: adj_grad<VIS_C8>(s, P, F, A, cb); break;
+    case VIS_C6: adj_grad<VIS_C6>(s, P, F, A, cb); break;
+    default: adj_grad<VIS_C4>(s, P, F, A, cb); break;
     }
     return hipGetLastError();
 }

@@ -367,7 +367,8 @@ int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot, 
     static const char *sname[3] = {"adj_sweep_x", "adj_sweep_y", "adj_sweep_z"};
     for (int d = 0; d < 3; ++d)
         TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d)));
-    TIMED(c, "adj_visc", launch_adj_visc(c->stream, c->dp, F, c->adj, w_obj, c->adj_part, a, bx, bl, init, add_acc));
+    TIMED(c, "adj_vflux", launch_adj_vflux(c->stream, c->dp, F, c->adj, w_obj, c->adj_part));
+    TIMED(c, "adj_grad", launch_adj_grad(c->stream, c->dp, F, c->adj, a, bx, bl, init, add_acc));
     TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), c->adj_sums + slot));
     return NS3D_OK;
 }
@@ -485,6 +486,10 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         if (nseg < 1) nseg = 1;
         c->seg_z = (int)((g.nz + nseg - 1) / nseg);
     }
+    // NS3D_ADJ_MARCH=1 selects the column-marching adjoint sweeps everywhere
+    // (they serve lines too long for LDS)
+    d.adj_lines = 1;
+    if (const char *env = std::getenv("NS3D_ADJ_MARCH")) d.adj_lines = std::atoi(env) ? 0 : 1;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
     if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
     if (const char *env = std::getenv("NS3D_SEG_Z")) {

@@ -18,6 +18,7 @@ struct DevParams {
     int nx, ny, nz, ng, sx, sy, sz;
     int periodic[3];
     double rho_ref, p_ref;   // reference state of the mixed-precision shadow
+    int adj_lines;           // 1: adjoint sweeps may hold whole lines in LDS
 };
 
 enum { FVS_SW = 0, FVS_LF = 1 };

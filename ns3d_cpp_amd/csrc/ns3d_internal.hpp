@@ -89,8 +89,10 @@ int ad_seg(const DevParams &P, int dir);
 hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A);
 hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir,
                             int seg);
-hipError_t launch_adj_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
-                           double *partial, double a, double bx, double bl, int init, int add_acc);
+hipError_t launch_adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
+                            double *partial);
+hipError_t launch_adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,
+                           double bx, double bl, int init, int add_acc);
 bool adj_xrow(const DevParams &P);
 hipError_t launch_dissip(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 hipError_t launch_sum_partials(hipStream_t s, const double *partial, int n, double *out);

@@ -108,6 +108,30 @@ def test_dot_product_identity(gpu, cfg):
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
+@pytest.mark.parametrize("cfg", DOT_CASES[:2] + DOT_CASES[5:6], ids=["weno5-sw", "weno5-lf", "ud7"])
+def test_dot_product_marching_sweeps(gpu, cfg, monkeypatch):
+    """The column-marching adjoint sweeps (used for lines too long for LDS)
+    against the TLM, and against the whole-line sweeps."""
+    n = 16
+    prims, Ma = state(n)
+    rng = np.random.default_rng(19)
+    x = [rng.normal(size=(n, n, n)) for _ in range(5)]
+    y = [rng.normal(size=(n, n, n)) for _ in range(5)]
+    ref = solver(n, prims, Ma, **cfg)
+    RTy_line, th_line = ref.adj_rhs(y, 0.5)
+    monkeypatch.setenv("NS3D_ADJ_MARCH", "1")
+    s = solver(n, prims, Ma, **cfg)
+    RTy, th = s.adj_rhs(y, 0.5)
+    Rx = s.tlm_rhs(x)
+    RTy0, _ = s.adj_rhs(y, 0.0)
+    lhs = interior_dot(s, [s._full(a) for a in y], Rx)
+    rhs = interior_dot(s, RTy0, [s._full(a) for a in x])
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    for a, b in zip(RTy, RTy_line):
+        assert np.allclose(s.interior(a), s.interior(b), rtol=1e-11, atol=1e-11 * np.abs(b).max())
+    assert abs(th - th_line) <= 1e-12 * abs(th_line)
+
+
 def _fd_rhs(n, prims, Ma, cfg, dU, h):
     s = solver(n, prims, Ma, **cfg)
     U = [s._full(a) for a in conserved_of(prims)]
