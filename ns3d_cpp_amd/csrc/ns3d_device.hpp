@@ -70,13 +70,30 @@ __device__ __forceinline__ void p2c_energy_sound(double gamma, double rr, double
     c = sqrt(gamma * p / rr);
 }
 
+// T^1.5 for the Sutherland law: T * sqrt(T) evaluated with an exact FMA
+// residual for the product and one Newton correction of the square root
+// (Dekker-style compensation), i.e. correctly rounded except within a tiny
+// fraction of an ulp of a rounding midpoint.  The reference calls the host
+// libm pow(T, 1.5), which is correctly rounded in all but ~0.1% of inputs;
+// the device library's pow is faithfully rounded only (off by one ulp for
+// ~15% of T in [0.5, 2]), which would make the FP64 path drift from the
+// reference at the last bit.
+__device__ __forceinline__ double pow15(double T)
+{
+    const double s = sqrt(T);
+    const double es = fma(-s, s, T) / (2.0 * s);   // sqrt(T) - s, first order
+    const double p = T * s;
+    const double pe = fma(T, s, -p);                // exact rounding error of T*s
+    return p + (pe + T * es);
+}
+
 // SolverParams::get_mu (field_structures.h:94-105) + selectable laws.
 __device__ __forceinline__ double get_mu(const DevParams &P, double T)
 {
     if (P.mu_law == 1) return P.mu;
     if (P.mu_law == 2) return P.mu * pow(T, P.mu_power);
     double S_ref = P.sutherland_s;
-    return P.mu * pow(T, 1.5) * (1.0 + S_ref) / (T + S_ref);
+    return P.mu * pow15(T) * (1.0 + S_ref) / (T + S_ref);
 }
 
 // --------------------------------------------------------------- splits

@@ -373,3 +373,65 @@ def test_gradient_fd_32_isotropic_turbulence(gpu, kind):
     ad = interior_dot(s, lam0, dU)
     print(kind, "J", J, "fd", fd, "adjoint", ad, "rel", abs(fd - ad) / abs(fd))
     assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+class _DissipationJ:
+    """torch.autograd.Function over the C-ABI forward/backward pair: the
+    time-integrated dissipation J(U0, 1/Re).  Only the tests use torch here;
+    the library itself takes and returns plain host arrays."""
+
+    @staticmethod
+    def build():
+        import torch
+
+        class Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, U0, inv_re, solver_factory, nsteps, dt, ckpt):
+                s = solver_factory(float(inv_re))
+                arrs = [s._full(U0[m].detach().cpu().numpy()) for m in range(5)]
+                s.upload_conserved(*arrs)
+                J = s.adj_forward(nsteps, dt, ckpt)
+                ctx.solver = s
+                return torch.tensor(J, dtype=torch.float64)
+
+            @staticmethod
+            def backward(ctx, gJ):
+                s = ctx.solver
+                lam0, dth = s.adj_backward()
+                g = torch.tensor(np.stack([s.interior(a) for a in lam0]), dtype=torch.float64)
+                return gJ * g, gJ * torch.tensor(dth, dtype=torch.float64), None, None, None, None
+
+        return Fn
+
+
+def test_torch_autograd_wrapper(gpu):
+    import torch
+    n, nsteps, dt = 16, 4, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = state(n, seed=41)
+
+    def factory(inv_re):
+        return solver(n, prims, Ma, **dict(cfg, Re=1.0 / inv_re))
+
+    Fn = _DissipationJ.build()
+    U0 = torch.tensor(np.stack(conserved_of(prims)), dtype=torch.float64, requires_grad=True)
+    th = torch.tensor(1.0 / BASE["Re"], dtype=torch.float64, requires_grad=True)
+    J = Fn.apply(U0, th, factory, nsteps, dt, 2)
+    (2.0 * J).backward()
+    # the same gradient through the Solver API
+    s = factory(1.0 / BASE["Re"])
+    s.upload_conserved(*[s._full(a) for a in conserved_of(prims)])
+    J2 = s.adj_forward(nsteps, dt, 2)
+    lam0, dth = s.adj_backward()
+    assert J.item() == J2
+    assert np.array_equal(U0.grad.numpy(), 2.0 * np.stack([s.interior(a) for a in lam0]))
+    assert th.grad.item() == 2.0 * dth
+    # a directional derivative through the wrapper against central differences
+    d = torch.tensor(np.stack(smooth_dir(n, 3, [0.05, 0.05, 0.05, 0.05, 1.0])), dtype=torch.float64)
+    h = 1e-5
+    with torch.no_grad():
+        Jp = Fn.apply(U0 + h * d, th, factory, nsteps, dt, 2)
+        Jm = Fn.apply(U0 - h * d, th, factory, nsteps, dt, 2)
+    fd = (Jp - Jm).item() / (2 * h)
+    ad = float(torch.sum(U0.grad * d)) / 2.0
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
