@@ -27,7 +27,11 @@ HIP_FLAGS = [
 ]
 SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_api.cpp", "ns3d_params.cpp"]
 # the mixed-precision path and the derivative kernels do not reproduce the
-# reference's rounding, so their translation units may contract a*b+c into FMAs
+# reference's rounding, so their translation units contract a*b+c into FMAs
+# within an expression (-ffp-contract=on).  Not "fast": that lets the backend
+# fuse across statements wherever the schedule allows, so one inline function
+# can round differently at two call sites, and the sweeps difference split
+# values computed at different call sites.
 FMA_SOURCES = {"ns3d_mixed.hip", "ns3d_adjoint.hip"}
 
 
@@ -50,7 +54,7 @@ def build(verbose=True):
         objs.append(obj)
         flags = list(HIP_FLAGS)
         if src in FMA_SOURCES:
-            flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast"
+            flags[flags.index("-ffp-contract=off")] = "-ffp-contract=on"
         jobs.append([HIPCC, *flags, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))

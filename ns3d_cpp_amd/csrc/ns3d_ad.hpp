@@ -269,17 +269,6 @@ __device__ __forceinline__ T recon_t(const DevParams &P, const T *s)
 
 // ------------------------------------------------------------ reverse rules
 
-// 1/x: the hardware estimate refined by two Newton-Raphson steps
-// r <- r + r (1 - x r) (quadratic convergence; x > 0 and normal here).
-// The derivative code needs the value, not the reference's rounding.
-__device__ __forceinline__ double rcp64(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
 
 // sbar[s] = gbar * d recon / d s[s]
 template <int RECON>

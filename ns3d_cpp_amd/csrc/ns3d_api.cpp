@@ -453,7 +453,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.p_ref = p.Rgas;   // p at rho = 1, T = 1: 1 / (gamma Ma^2)
     c->mixed = p.precision == NS3D_PREC_MIXED;
     if (c->mixed) {
-        if (!xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
+        if (!m_xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
         const int nf = 6 + 5 + 12;
         e = hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float));
         if (e == hipSuccess) e = hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream);

@@ -57,6 +57,19 @@ __device__ __forceinline__ void xcd_block3(int &bx, int &by, int &bz)
     bz = l / (gx * gy);
 }
 
+// 1/x: the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r) (quadratic convergence; x > 0 and normal where used).
+// For the mixed-precision and derivative kernels, which need the value and
+// not the reference's rounding; the FP64 parity path divides.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // ----------------------------------------------------------- conversions
 
 // E and c of primitiveToConserved (field_structures.h:445-449) from a

@@ -76,6 +76,7 @@ hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields 
 hipError_t launch_m_to_prims(hipStream_t s, const DevParams &P, const MFields &M, double *const *dst);
 hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields &M);
 hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out);
+bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,

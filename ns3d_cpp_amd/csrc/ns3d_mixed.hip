@@ -123,9 +123,10 @@ __device__ __forceinline__ void shadow_prims(const DevParams &P, const MFields &
 
 // ---------------------------------------------------------------- sweeps
 
-// x sweep, one block per full periodic x-row (as k_sweep_xrow).
+// x sweep with FP64 split values in LDS, one block per full periodic x-row
+// (as k_sweep_xrow); used for MDCD_HYBRID, which reconstructs in FP64.
 template <int RECON, int FVS>
-__global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
+__global__ __launch_bounds__(1024) void km_sweep_xrow64(DevParams P, MFields M)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
@@ -166,10 +167,9 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
     }
 }
 
-// y / z sweeps: column marching as k_sweep, FP64 split window, FP32
-// reconstruction of residuals; accumulates into the FP32 accumulator.
+// y / z sweeps with an FP64 split window (as k_sweep); used for MDCD_HYBRID.
 template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(64, 2) void km_sweep(DevParams P, MFields M, int seg_len)
+__global__ __launch_bounds__(64, 2) void km_sweep64(DevParams P, MFields M, int seg_len)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
@@ -231,6 +231,321 @@ __global__ __launch_bounds__(64, 2) void km_sweep(DevParams P, MFields M, int se
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) fprev[m] = ff[m];
+    }
+}
+
+// ------------------------------------- reconstruction in difference form
+//
+// The split fluxes are formed in FP64 (their low-Mach pressure and enthalpy
+// parts are large) and each cell's difference to its neighbour along the
+// sweep, d(c) = F(c) - F(c-1), is rounded to FP32 once.  A face value is the
+// upwind cell's value plus a reconstruction of the differences; every scheme
+// here is translation invariant (WENO5 weights see differences only, linear
+// stencils sum to 1), so that is the same face value.  The flux difference of
+// cell c then needs no FP64 face value at all:
+//   F(c) - F(c-1) = d+(c) + d-(c+1) + [rel(c) - rel(c-1)],
+// rel(f) = rel+(f) + rel-(f), the two sides' reconstructed differences.
+// D[1..S-1] are the stencil's differences in reconstruct_select's
+// orientation: D[t] = f[t] - f[t-1].
+
+// WENO5-JS face value minus f[2] from D[1..4]: one reciprocal for the
+// weights, after scaling the (eps + beta)^2 by their maximum.
+__device__ __forceinline__ float rel32_weno5(const float *D)
+{
+    const float D1 = D[1], D2 = D[2], D3 = D[3], D4 = D[4];
+    const float c13 = 13.0f / 12.0f;
+    const float s0 = D2 - D1, r0 = 3.0f * D2 - D1;
+    const float s1 = D3 - D2, r1 = D2 + D3;
+    const float s2 = D4 - D3, r2 = D4 - 3.0f * D3;
+    const float t0 = 1e-6f + c13 * s0 * s0 + 0.25f * r0 * r0;
+    const float t1 = 1e-6f + c13 * s1 * s1 + 0.25f * r1 * r1;
+    const float t2 = 1e-6f + c13 * s2 * s2 + 0.25f * r2 * r2;
+    float T0 = t0 * t0, T1 = t1 * t1, T2 = t2 * t2;
+    const float im = rcpf(fmaxf(T0, fmaxf(T1, T2)));
+    T0 *= im; T1 *= im; T2 *= im;
+    // alpha_k = d_k / T_k, scaled by T0 T1 T2 (the largest T is 1 now)
+    const float a0 = 0.1f * T1 * T2, a1 = 0.6f * T0 * T2, a2 = 0.3f * T0 * T1;
+    const float q0 = 5.0f * D2 - 2.0f * D1;
+    const float q1 = D2 + 2.0f * D3;
+    const float q2 = 4.0f * D3 - D4;
+    return (a0 * q0 + a1 * q1 + a2 * q2) * (rcpf(a0 + a1 + a2) * (1.0f / 6.0f));
+}
+
+// coefficients e[t] of a linear scheme in difference form:
+// sum_s c_s f_s - f_MID = sum_t e_t D_t, e_t = sum_{s>=t} c_s (t > MID),
+// -sum_{s<t} c_s (t <= MID)
+template <int RECON>
+__device__ __forceinline__ void diff_coeffs(const DevParams &P, float *e)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    double c[S];
+    if constexpr (RECON == R_LINEAR) {
+        c[0] = 1.0; c[1] = 0.0;
+    } else if constexpr (RECON == R_MDCD) {
+        const double disp = P.mdcd_disp, diss = P.mdcd_diss;
+        c[0] = 0.5 * disp + 0.5 * diss;
+        c[1] = -1.5 * disp - 2.5 * diss - 1.0 / 12.0;
+        c[2] = 7.0 / 12.0 + disp + 5.0 * diss;
+        c[3] = 7.0 / 12.0 + disp - 5.0 * diss;
+        c[4] = -1.5 * disp + 2.5 * diss - 1.0 / 12.0;
+        c[5] = 0.5 * disp - 0.5 * diss;
+    } else if constexpr (RECON == R_UD7) {
+        c[0] = -1.0 / 140.0; c[1] = 5.0 / 84.0; c[2] = -101.0 / 420.0; c[3] = 319.0 / 420.0;
+        c[4] = 107.0 / 210.0; c[5] = -19.0 / 210.0; c[6] = 1.0 / 105.0; c[7] = 0.0;
+    } else {
+        const double a = P.ud7l_alpha, b = 1.0 - a;
+        c[0] = -3.0 / 420.0 * a - 3.0 / 840.0 * b;
+        c[1] = 25.0 / 420.0 * a + 29.0 / 840.0 * b;
+        c[2] = -101.0 / 420.0 * a - 139.0 / 840.0 * b;
+        c[3] = 319.0 / 420.0 * a + 533.0 / 840.0 * b;
+        c[4] = 214.0 / 420.0 * a + 533.0 / 840.0 * b;
+        c[5] = -38.0 / 420.0 * a - 139.0 / 840.0 * b;
+        c[6] = 4.0 / 420.0 * a + 29.0 / 840.0 * b;
+        c[7] = -3.0 / 840.0 * b;
+    }
+#pragma unroll
+    for (int t = 1; t < S; ++t) {
+        double acc = 0.0;
+        if (t > MID) {
+#pragma unroll
+            for (int q = 0; q < S; ++q) acc += q >= t ? c[q] : 0.0;
+        } else {
+#pragma unroll
+            for (int q = 0; q < S; ++q) acc -= q < t ? c[q] : 0.0;
+        }
+        e[t] = (float)acc;
+    }
+}
+
+template <int RECON>
+__device__ __forceinline__ float rel32(const float *e, const float *D)
+{
+    constexpr int S = Stencil<RECON>::S;
+    if constexpr (RECON == R_WENO5) {
+        return rel32_weno5(D);
+    } else {
+        float r = e[1] * D[1];
+#pragma unroll
+        for (int t = 2; t < S; ++t) r += e[t] * D[t];
+        return r;
+    }
+}
+
+// split_state with reciprocals instead of divisions (this path does not
+// reproduce the reference's rounding).  Contraction into FMAs is off here:
+// the sweeps call it from several places and difference the results of
+// neighbouring cells, so a uniform state must give bitwise equal values
+// whichever call site evaluated them (free-stream preservation).
+template <int FVS, int DIR>
+__device__ __forceinline__ void split_fast(const DevParams &P, double rho, double u, double v, double w, double p,
+                                           double Fp[5], double Fm[5])
+{
+#pragma clang fp contract(off)
+    const double gamma = P.gamma;
+    const double irho = rcp64(rho);
+    const double V = 0.5 * (u * u + v * v + w * w);
+    const double ig1 = 1.0 / (gamma - 1.0);
+    const double E = p * ig1 + rho * V;
+    const double c2 = gamma * p * irho;
+    const double c = sqrt(c2);
+    const double un = DIR == 0 ? u : (DIR == 1 ? v : w);
+    if constexpr (FVS == FVS_SW) {
+        double l1p, l1m, l2p, l2m, l3p, l3m;
+        lam_split(un - c, P.sw_eps, l1p, l1m);
+        lam_split(un, P.sw_eps, l2p, l2m);
+        lam_split(un + c, P.sw_eps, l3p, l3m);
+        const double H = (E + p) * irho;
+        const double ga1 = rho * (0.5 / gamma), ga2 = 2.0 * (gamma - 1.0);
+        const double S1p = l1p + l3p, S1m = l1m + l3m;
+        const double S2p = (l3p - l1p) * c, S2m = (l3m - l1m) * c;
+        const double g2p = ga2 * l2p, g2m = ga2 * l2m;
+        Fp[0] = ga1 * (S1p + g2p);
+        Fp[1] = ga1 * ((S1p + g2p) * u + (DIR == 0 ? S2p : 0.0));
+        Fp[2] = ga1 * ((S1p + g2p) * v + (DIR == 1 ? S2p : 0.0));
+        Fp[3] = ga1 * ((S1p + g2p) * w + (DIR == 2 ? S2p : 0.0));
+        Fp[4] = ga1 * (S1p * H + g2p * V + S2p * un);
+        Fm[0] = ga1 * (S1m + g2m);
+        Fm[1] = ga1 * ((S1m + g2m) * u + (DIR == 0 ? S2m : 0.0));
+        Fm[2] = ga1 * ((S1m + g2m) * v + (DIR == 1 ? S2m : 0.0));
+        Fm[3] = ga1 * ((S1m + g2m) * w + (DIR == 2 ? S2m : 0.0));
+        Fm[4] = ga1 * (S1m * H + g2m * V + S2m * un);
+    } else {
+        const double a = fabs(un) + c;
+        const double F1 = rho * un;
+        const double F5 = (E + p) * un;
+        Fp[0] = 0.5 * (F1 + a * rho);
+        Fp[1] = 0.5 * (F1 * u + (DIR == 0 ? p : 0.0) + a * rho * u);
+        Fp[2] = 0.5 * (F1 * v + (DIR == 1 ? p : 0.0) + a * rho * v);
+        Fp[3] = 0.5 * (F1 * w + (DIR == 2 ? p : 0.0) + a * rho * w);
+        Fp[4] = 0.5 * (F5 + a * E);
+        Fm[0] = 0.5 * (F1 - a * rho);
+        Fm[1] = 0.5 * (F1 * u + (DIR == 0 ? p : 0.0) - a * rho * u);
+        Fm[2] = 0.5 * (F1 * v + (DIR == 1 ? p : 0.0) - a * rho * v);
+        Fm[3] = 0.5 * (F1 * w + (DIR == 2 ? p : 0.0) - a * rho * w);
+        Fm[4] = 0.5 * (F5 - a * E);
+    }
+}
+
+// x sweep, one block per full periodic x-row.  Each thread splits one cell in
+// FP64 and differences it with the left neighbour's split values, taken from
+// the next lower lane: wave w covers cells 63w - 1 .. 63w + 62 of the row's
+// slots, its first lane only supplying the left neighbour of the second.  LDS
+// holds the FP32 differences of the row and its ghost cells, and rel per
+// face.  Launched with 64 * ceil((nx + S) / 63) threads.
+template <int RECON, int FVS>
+__global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    extern __shared__ float ldsf[];
+    const int TX = P.nx;
+    const int NSLOT = TX + S;                 // cells ng - MID - 1 .. ng + TX + S - MID - 2
+    float *dl = ldsf;                         // [10][NSLOT] differences (slot 0 unused)
+    float *rl = dl + 10 * NSLOT;              // [5][TX] rel per face
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int j = P.ng + blockIdx.x;
+    const int k = P.ng + blockIdx.y;
+    const int row = (k * P.sy + j) * P.sx;
+    const int i0 = P.ng - MID - 1;
+    {
+        const int q = 63 * (t >> 6) + lane - 1;
+        const int qc = q < 0 ? 0 : (q < NSLOT ? q : NSLOT - 1);
+        double rho, u, v, w, p, F[10];
+        shadow_prims(P, M, row + i0 + qc, rho, u, v, w, p);
+        split_fast<FVS, 0>(P, rho, u, v, w, p, F, F + 5);
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            const double L = __shfl_up(F[m], 1, 64);
+            if (lane > 0 && q > 0 && q < NSLOT) dl[m * NSLOT + q] = (float)(F[m] - L);
+        }
+    }
+    __syncthreads();
+    float e[S];
+    diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);   // unused for WENO5
+    if (t < TX) {
+        // face right of cell ng + t: stencil cells ng + t - MID + s = slots t + 1 + s
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            float D[S], Dm[S];
+#pragma unroll
+            for (int tt = 1; tt < S; ++tt) {
+                D[tt] = dl[m * NSLOT + t + 1 + tt];
+                Dm[tt] = -dl[(5 + m) * NSLOT + t + 1 + S - tt];
+            }
+            rl[m * TX + t] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
+        }
+    }
+    __syncthreads();
+    if (t < TX) {
+        const int tl = t == 0 ? TX - 1 : t - 1;
+        const int id = row + P.ng + t;
+        const float ix = (float)P.idx;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const float dF = dl[m * NSLOT + t + MID + 1] + dl[(5 + m) * NSLOT + t + MID + 2] +
+                             (rl[m * TX + t] - rl[m * TX + tl]);
+            M.A[m][id] = dF * ix;
+        }
+    }
+}
+
+// y / z sweeps: one thread marches a column segment keeping, per component
+// and side, the S FP32 differences of the stencil and the FP64 split value of
+// the last cell (to difference the next one); accumulates into A.
+template <int DIR, int RECON, int FVS>
+__global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_len)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    constexpr int T0 = S == 2 ? 0 : 1;       // lowest window slot that is read
+    int bs, bt, bb;
+    xcd_block3(bs, bt, bb);   // grid (nseg, lane tiles, b), as the launch
+    const int a = bt * blockDim.x + threadIdx.x;
+    if (a >= P.nx) return;
+    const int b = bb;
+    const int nd = DIR == 1 ? P.ny : P.nz;
+    const int c0 = P.ng + bs * seg_len;
+    const int c1 = min(c0 + seg_len, P.ng + nd);
+    if (c0 >= c1) return;
+    int base, st;
+    if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + a); st = P.sx; }
+    else { base = (P.ng + b) * P.sx + (P.ng + a); st = P.sx * P.sy; }
+    const float inv_d = (float)(DIR == 1 ? P.idy : P.idz);
+    float e[S];
+    diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);
+    float dw[10][S];          // window slot t <-> d(f - MID + t)
+    double prev[10];
+    {
+        double rho, u, v, w, p, Fp[5], Fm[5];
+        shadow_prims(P, M, base + (c0 - 2 - MID + T0) * st, rho, u, v, w, p);
+        split_fast<FVS, DIR>(P, rho, u, v, w, p, Fp, Fm);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { prev[m] = Fp[m]; prev[5 + m] = Fm[m]; }
+    }
+#pragma unroll
+    for (int t = T0 + 1; t < S; ++t) {
+        double rho, u, v, w, p, Fp[5], Fm[5];
+        shadow_prims(P, M, base + (c0 - 2 - MID + t) * st, rho, u, v, w, p);
+        split_fast<FVS, DIR>(P, rho, u, v, w, p, Fp, Fm);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            dw[m][t] = (float)(Fp[m] - prev[m]);
+            dw[5 + m][t] = (float)(Fm[m] - prev[5 + m]);
+            prev[m] = Fp[m];
+            prev[5 + m] = Fm[m];
+        }
+    }
+    float nx_[5];
+    {
+        const int id = base + (c0 - 1 - MID + S - 1) * st;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+    }
+    float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int f = c0 - 1; f < c1; ++f) {
+#pragma unroll
+        for (int t = T0; t < S - 1; ++t)
+#pragma unroll
+            for (int m = 0; m < 10; ++m) dw[m][t] = dw[m][t + 1];
+        {
+            float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
+            if (f + 1 < c1) {
+                const int id = base + (f + 1 - MID + S - 1) * st;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+            }
+            double Fp[5], Fm[5];
+            split_fast<FVS, DIR>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
+                                 P.p_ref + (double)cur[4], Fp, Fm);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                dw[m][S - 1] = (float)(Fp[m] - prev[m]);
+                dw[5 + m][S - 1] = (float)(Fm[m] - prev[5 + m]);
+                prev[m] = Fp[m];
+                prev[5 + m] = Fm[m];
+            }
+        }
+        float rel[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            float D[S], Dm[S];
+#pragma unroll
+            for (int t = 1; t < S; ++t) {
+                D[t] = dw[m][t];
+                Dm[t] = -dw[5 + m][S - t];
+            }
+            rel[m] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
+        }
+        if (f >= c0) {
+            const int idc = base + f * st;
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                M.A[m][idc] += (dw[m][MID] + dw[5 + m][MID + 1] + (rel[m] - relprev[m])) * inv_d;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
     }
 }
 
@@ -559,7 +874,7 @@ static MXrowFn m_pick_xrow(int recon)
     case R_WENO5: return km_sweep_xrow<R_WENO5, FVS>;
     case R_LINEAR: return km_sweep_xrow<R_LINEAR, FVS>;
     case R_MDCD: return km_sweep_xrow<R_MDCD, FVS>;
-    case R_MDCD_HYBRID: return km_sweep_xrow<R_MDCD_HYBRID, FVS>;
+    case R_MDCD_HYBRID: return km_sweep_xrow64<R_MDCD_HYBRID, FVS>;
     case R_UD7: return km_sweep_xrow<R_UD7, FVS>;
     case R_UD7L: return km_sweep_xrow<R_UD7L, FVS>;
     default: return nullptr;
@@ -573,22 +888,35 @@ static MSweepFn m_pick_sweep(int recon)
     case R_WENO5: return km_sweep<DIR, R_WENO5, FVS>;
     case R_LINEAR: return km_sweep<DIR, R_LINEAR, FVS>;
     case R_MDCD: return km_sweep<DIR, R_MDCD, FVS>;
-    case R_MDCD_HYBRID: return km_sweep<DIR, R_MDCD_HYBRID, FVS>;
+    case R_MDCD_HYBRID: return km_sweep64<DIR, R_MDCD_HYBRID, FVS>;
     case R_UD7: return km_sweep<DIR, R_UD7, FVS>;
     case R_UD7L: return km_sweep<DIR, R_UD7L, FVS>;
     default: return nullptr;
     }
 }
 
+// the mixed x sweep holds whole rows (1024 threads, 64 KiB of LDS at most)
+bool m_xrow_supported(const DevParams &P)
+{
+    const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
+                  : P.recon == R_LINEAR ? 2 : 8;
+    if (P.recon == R_MDCD_HYBRID) return xrow_supported(P);
+    const size_t lds = (size_t)(10 * (P.nx + S) + 5 * P.nx) * sizeof(float);
+    return P.periodic[0] && 64 * ((P.nx + S + 62) / 63) <= 1024 && lds <= 65536;
+}
+
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len)
 {
     if (dir == 0) {
         MXrowFn fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW>(P.recon) : m_pick_xrow<FVS_LF>(P.recon);
-        if (!fn || !xrow_supported(P)) return hipErrorInvalidValue;
+        if (!fn || !m_xrow_supported(P)) return hipErrorInvalidValue;
         const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
                       : P.recon == R_LINEAR ? 2 : 8;
-        const size_t lds = (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double);
-        const int threads = ((P.nx + 63) / 64) * 64;
+        const bool hyb = P.recon == R_MDCD_HYBRID;
+        const size_t lds = hyb ? (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double)
+                               : (size_t)(10 * (P.nx + S) + 5 * P.nx) * sizeof(float);
+        const int threads = hyb ? ((P.nx + 63) / 64) * 64 : 64 * ((P.nx + S + 62) / 63);
+        if (threads > 1024 || lds > 65536) return hipErrorInvalidValue;
         hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, M);
         return hipGetLastError();
     }
