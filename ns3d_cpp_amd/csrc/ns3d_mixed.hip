@@ -454,8 +454,11 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
 // y / z sweeps: one thread marches a column segment keeping, per component
 // and side, the S FP32 differences of the stencil and the FP64 split value of
 // the last cell (to difference the next one); accumulates into A.
+// WENO5 fits 4 waves per SIMD without spilling; the 8-point linear schemes
+// spill there and run at 3
 template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_len)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RECON == R_WENO5 ? 4 : 3, 8)))
+void km_sweep(DevParams P, MFields M, int seg_len)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;

@@ -18,6 +18,13 @@ import re
 def kind_of(name):
     """bench.py kernel kind of a HIP kernel symbol."""
     n = name.replace("ns3d::", "")
+    m = re.search(r"ka_sweep(?:_line|c)<(\d)", n)
+    if m:
+        return ["adj_sweep_x", "adj_sweep_y", "adj_sweep_z"][int(m.group(1))]
+    for key, kind in (("ka_vflux", "adj_vflux"), ("ka_grad", "adj_grad"), ("ka_fill", "adj_fill"),
+                      ("k_dissip", "dissipation"), ("k_sum_partials", "adj_sum")):
+        if key in n:
+            return kind
     if re.search(r"k_sweep_xrow|km_sweep_xrow", n):
         return "sweep_x"
     m = re.search(r"km?_sweep<(\d)", n)
