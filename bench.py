@@ -252,6 +252,7 @@ def main():
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
+    ap.add_argument("--adj-only", action="store_true", help="only the adjoint leg (for profiling it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -262,8 +263,12 @@ def main():
     from ns3d_cpp_amd import fields
 
     n, Re, Mt, k0 = args.n, 200.0, 0.3, 4.0
-    st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=args.seed)
     H = hip()
+    if args.adj_only:
+        adj = adjoint_cost(ns, H, args.adj_n, args.adj_steps, args.adj_ckpt, args.seed)
+        print(json.dumps({"adjoint": adj}), flush=True)
+        return
+    st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=args.seed)
     # achievable streaming bandwidth on this device
     p, g = make(ns, n, Ma, Re, args.recon, "fp64")
     probe = ns.Solver(p, g)
