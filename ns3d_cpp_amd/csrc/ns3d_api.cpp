@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -40,6 +41,7 @@ struct ns3d_ctx {
     int seg_z = 32;                          // z segments of the z-marching kernels
     int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
     int sweep_minw = 2;                      // waves-per-EU floor of k_sweep
+    int m_gf_block[2] = {64, 4};             // workgroup shape of km_gradflux
     unsigned long long bad_key = ~0ull;
     long long steps_done = 0;
     std::string err;
@@ -224,7 +226,7 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
 {
     static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
     for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
-    TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z));
+    TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1]));
     const unsigned long long key = (unsigned long long)c->steps_done << 40;
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
           launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, key));
@@ -449,19 +451,34 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     for (int m = 0; m < 5; ++m) { c->f.A[m] = q; q += ncell; }
     for (int m = 0; m < 12; ++m) { c->f.G[m] = q; q += ncell; }
     for (int m = 0; m < 12; ++m) { c->f.VF[m] = q; q += ncell; }
+    // symmetric stress: the y and z fluxes' txy, txz, tyz are the x and y
+    // fluxes' arrays (the fused kernels write 9 arrays; k_vflux writes the
+    // same values twice)
+    c->f.VF[4] = c->f.VF[1];
+    c->f.VF[8] = c->f.VF[2];
+    c->f.VF[9] = c->f.VF[6];
     d.rho_ref = 1.0;
     d.p_ref = p.Rgas;   // p at rho = 1, T = 1: 1 / (gamma Ma^2)
     c->mixed = p.precision == NS3D_PREC_MIXED;
     if (c->mixed) {
         if (!m_xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
-        const int nf = 6 + 5 + 12;
+        const int nf = 6 + 5 + 9;
         e = hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float));
         if (e == hipSuccess) e = hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream);
         if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc mixed"); ns3d_destroy(c); return r; }
         float *fq = c->fslab;
         for (int m = 0; m < 6; ++m) { c->m.S[m] = fq; fq += ncell; }
         for (int m = 0; m < 5; ++m) { c->m.A[m] = fq; fq += ncell; }
-        for (int m = 0; m < 12; ++m) { c->m.VF[m] = fq; fq += ncell; }
+        // 9 distinct viscous-flux arrays: the y and z fluxes' txy, txz, tyz
+        // are the x and y fluxes' (symmetric stress)
+        for (int m = 0; m < 12; ++m) {
+            if (m == 4 || m == 8 || m == 9) continue;
+            c->m.VF[m] = fq;
+            fq += ncell;
+        }
+        c->m.VF[4] = c->m.VF[1];
+        c->m.VF[8] = c->m.VF[2];
+        c->m.VF[9] = c->m.VF[6];
         for (int m = 0; m < 5; ++m) {
             c->m.U[m] = c->f.W[m];      // conserved state lives in the W arrays
             c->m.U0[m] = c->f.U0[m];
@@ -492,6 +509,13 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     if (const char *env = std::getenv("NS3D_ADJ_MARCH")) d.adj_lines = std::atoi(env) ? 0 : 1;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
     if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
+    if (const char *env = std::getenv("NS3D_M_GRADFLUX_BLOCK")) {   // "BXxBY"
+        int bx = 0, by = 0;
+        if (std::sscanf(env, "%dx%d", &bx, &by) == 2 && bx > 0 && by > 0 && bx % 32 == 0 && bx * by <= 256) {
+            c->m_gf_block[0] = bx;
+            c->m_gf_block[1] = by;
+        }
+    }
     if (const char *env = std::getenv("NS3D_SEG_Z")) {
         int v = std::atoi(env);
         if (v > 0) c->seg_z = v;

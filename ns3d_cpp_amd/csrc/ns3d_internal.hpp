@@ -78,7 +78,7 @@ hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields
 hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out);
 bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
-hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len);
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, unsigned long long key);
 hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial);

@@ -248,8 +248,9 @@ __global__ __launch_bounds__(1024) void k_sweep_xrow(DevParams P, Fields F)
 // (ViscousFlux.cpp:282-339): one thread per (i, j) column marching along z
 // with the R planes above and below held in registers, so the z stencil
 // reads each plane once; x and y neighbours come through the caches.
-// Writes only the 12 viscous-flux arrays.  Same arithmetic and order as
-// k_grad + k_vflux.
+// Writes only the viscous-flux arrays (9 distinct: the stress is symmetric and
+// the context aliases VF[4], VF[8], VF[9] to VF[1], VF[2], VF[6]).  Same
+// arithmetic and order as k_grad + k_vflux.
 template <int VIS, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, int seg_len)
 {
@@ -306,12 +307,9 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
         F.VF[1][id] = tau_xy;
         F.VF[2][id] = tau_xz;
         F.VF[3][id] = u * tau_xx + v * tau_xy + w * tau_xz + qx;
-        F.VF[4][id] = tau_xy;
         F.VF[5][id] = tau_yy;
         F.VF[6][id] = tau_yz;
         F.VF[7][id] = u * tau_xy + v * tau_yy + w * tau_yz + qy;
-        F.VF[8][id] = tau_xz;
-        F.VF[9][id] = tau_yz;
         F.VF[10][id] = tau_zz;
         F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
     }

@@ -576,7 +576,9 @@ __device__ __forceinline__ float diff32(const float *v, float h_inv)
 }
 
 // gradients + viscous flux in FP32 from the shadow (u, v, w, T-1),
-// z-marching register window as k_gradflux.
+// z-marching register window as k_gradflux.  The stress tensor is symmetric:
+// VF[4], VF[8], VF[9] (txy, txz, tyz of the y and z fluxes) alias VF[1],
+// VF[2], VF[6], so 9 arrays are written.
 template <int VIS>
 __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
 {
@@ -631,12 +633,9 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
         M.VF[1][id] = txy;
         M.VF[2][id] = txz;
         M.VF[3][id] = u * txx + v * txy + w * txz + mk * g[9];
-        M.VF[4][id] = txy;
         M.VF[5][id] = tyy;
         M.VF[6][id] = tyz;
         M.VF[7][id] = u * txy + v * tyy + w * tyz + mk * g[10];
-        M.VF[8][id] = txz;
-        M.VF[9][id] = tyz;
         M.VF[10][id] = tzz;
         M.VF[11][id] = u * txz + v * tyz + w * tzz + mk * g[11];
     }
@@ -933,10 +932,11 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
     return hipGetLastError();
 }
 
-hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len)
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by)
 {
-    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg_len - 1) / seg_len);
-    dim3 b(64, 4, 1);
+    if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
+    dim3 g((P.nx + bx - 1) / bx, (P.ny + by - 1) / by, (P.nz + seg_len - 1) / seg_len);
+    dim3 b(bx, by, 1);
     switch (P.vis) {
     case VIS_C8: hipLaunchKernelGGL(km_gradflux<VIS_C8>, g, b, 0, s, P, M, seg_len); break;
     case VIS_C6: hipLaunchKernelGGL(km_gradflux<VIS_C6>, g, b, 0, s, P, M, seg_len); break;
