@@ -30,22 +30,23 @@ sys.path.insert(0, ROOT)
 METRIC = "RK3 cell-updates/s @256^3 FP64 vs mixed-FP32; % HBM peak; adjoint/fwd cost"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per interior cell per launch of each kernel (every
-# array touched once per launch; halos reused on chip).
+# array touched once per launch; halos reused on chip).  The stress tensor is
+# symmetric, so 9 distinct viscous-flux arrays are written and read.
 ALG_BYTES = {
     "fp64": {
         "sweep_x": 80,        # read rho,u,v,w,p; write A(5)
         "sweep_y": 120,       # + read A(5)
         "sweep_z": 120,
-        "gradflux": 128,      # read u,v,w,T; write 12 viscous fluxes
-        "vdiv_stage": 264,    # read 12 fluxes, A(5), W(5), U0(5); write W(6)
+        "gradflux": 104,      # read u,v,w,T; write 9 viscous fluxes
+        "vdiv_stage": 240,    # read 9 fluxes, A(5), W(5), U0(5); write W(6)
         "cfl": 40,            # read rho,u,v,w,p
     },
     "mixed": {
         "sweep_x": 40,        # read FP32 shadow (5); write FP32 A(5)
         "sweep_y": 60,        # + read A(5)
         "sweep_z": 60,
-        "gradflux": 64,       # read u,v,w,T-1 (FP32); write 12 FP32 fluxes
-        "vdiv_stage": 212,    # read 12 FP32 fluxes, A32, U64, U0; write U64, shadow(6)
+        "gradflux": 52,       # read u,v,w,T-1 (FP32); write 9 FP32 fluxes
+        "vdiv_stage": 200,    # read 9 FP32 fluxes, A32, U64, U0; write U64, shadow(6)
         "cfl": 20,            # read the FP32 shadow
     },
 }

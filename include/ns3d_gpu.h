@@ -152,6 +152,11 @@ int ns3d_download_gradients(ns3d_ctx *ctx, double *const grads[12]);
 int ns3d_compute_viscous_flux(ns3d_ctx *ctx);
 int ns3d_download_viscous_flux(ns3d_ctx *ctx, double *const flux[12]);
 
+/* compute_vis_flux (ViscousFlux.cpp:341-386) from the fluxes stored by
+ * ns3d_compute_viscous_flux, on rhs = 0: the 5 RHS arrays (download with
+ * ns3d_download_rhs) receive ((0 + Dx Fv) + Dy Hv) + Dz Gv, flux ghosts read
+ * as periodic images (exchange_halos_viscous_flux, field_structures.h:736-791). */
+int ns3d_compute_vis_flux(ns3d_ctx *ctx);
 /* compute_rhs (range_kutta.cpp:11-99) with the fused sweep kernels:
  * rhs = viscous divergence - inviscid flux difference, interior cells. */
 int ns3d_compute_rhs(ns3d_ctx *ctx);
@@ -167,6 +172,11 @@ int ns3d_advance(ns3d_ctx *ctx, int nsteps, double dt);
 int ns3d_compute_timestep_local(ns3d_ctx *ctx, double *dt);
 /* compute_total_energy (postprocess.cpp:103-128): sum 0.5 rho |u|^2 dV. */
 int ns3d_total_energy(ns3d_ctx *ctx, double *ek);
+
+/* Switch the arithmetic of an existing context between NS3D_PREC_FP64 and
+ * NS3D_PREC_MIXED, keeping its state (SolverParams has no counterpart: the
+ * reference is FP64 only). */
+int ns3d_set_precision(ns3d_ctx *ctx, int precision);
 
 /* Synchronise the context's stream. */
 int ns3d_sync(ns3d_ctx *ctx);

@@ -172,6 +172,19 @@ class Solver:
         self._check(self._L.ns3d_download_viscous_flux(self.ctx, arr), "download_viscous_flux")
         return out
 
+    def compute_vis_flux(self):
+        """Viscous divergence of the stored fluxes on rhs = 0 (5 arrays)."""
+        self._check(self._L.ns3d_compute_vis_flux(self.ctx), "compute_vis_flux")
+        out = [np.empty(self.shape) for _ in range(5)]
+        self._check(self._L.ns3d_download_rhs(self.ctx, *[_ptr(a) for a in out]), "download_rhs")
+        return out
+
+    def set_precision(self, precision):
+        """Switch between "fp64" and "mixed" arithmetic, keeping the state."""
+        p = _enum(_lib.PREC, precision)
+        self._check(self._L.ns3d_set_precision(self.ctx, p), "set_precision")
+        self.params.precision = p
+
     def rk3_step(self, dt):
         st = self._L.ns3d_rk3_step(self.ctx, float(dt))
         self._check(st, "rk3_step")

@@ -76,6 +76,8 @@ SIGNATURES = {
     "ns3d_compute_viscous_flux": (C.c_int, [C.c_void_p]),
     "ns3d_download_viscous_flux": (C.c_int, [C.c_void_p, DP * 12]),
     "ns3d_compute_rhs": (C.c_int, [C.c_void_p]),
+    "ns3d_compute_vis_flux": (C.c_int, [C.c_void_p]),
+    "ns3d_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "ns3d_rk3_step": (C.c_int, [C.c_void_p, C.c_double]),
     "ns3d_nonphysical_cell": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 3),
     "ns3d_advance": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),

@@ -286,6 +286,36 @@ int copy_out(ns3d_ctx *c, double *dst, const double *src)
 }
 
 
+
+// FP32 arrays of the mixed path and the MFields view of the context
+int alloc_mixed(ns3d_ctx *c)
+{
+    if (c->fslab) return NS3D_OK;
+    const long long ncell = c->ncell;
+    const int nf = 6 + 5 + 9;
+    HIPCALL(c, hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float)));
+    HIPCALL(c, hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream));
+    float *fq = c->fslab;
+    for (int m = 0; m < 6; ++m) { c->m.S[m] = fq; fq += ncell; }
+    for (int m = 0; m < 5; ++m) { c->m.A[m] = fq; fq += ncell; }
+    // 9 distinct viscous-flux arrays: the y and z fluxes' txy, txz, tyz are
+    // the x and y fluxes' (symmetric stress)
+    for (int m = 0; m < 12; ++m) {
+        if (m == 4 || m == 8 || m == 9) continue;
+        c->m.VF[m] = fq;
+        fq += ncell;
+    }
+    c->m.VF[4] = c->m.VF[1];
+    c->m.VF[8] = c->m.VF[2];
+    c->m.VF[9] = c->m.VF[6];
+    for (int m = 0; m < 5; ++m) {
+        c->m.U[m] = c->f.W[m];      // conserved state lives in the W arrays
+        c->m.U0[m] = c->f.U0[m];
+        c->m.out[m] = c->f.G[m];    // RHS of ns3d_compute_rhs
+    }
+    return NS3D_OK;
+}
+
 // ------------------------------------------------------- differentiation
 
 int adj_check(ns3d_ctx *c)
@@ -462,28 +492,8 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     c->mixed = p.precision == NS3D_PREC_MIXED;
     if (c->mixed) {
         if (!m_xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
-        const int nf = 6 + 5 + 9;
-        e = hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float));
-        if (e == hipSuccess) e = hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream);
-        if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc mixed"); ns3d_destroy(c); return r; }
-        float *fq = c->fslab;
-        for (int m = 0; m < 6; ++m) { c->m.S[m] = fq; fq += ncell; }
-        for (int m = 0; m < 5; ++m) { c->m.A[m] = fq; fq += ncell; }
-        // 9 distinct viscous-flux arrays: the y and z fluxes' txy, txz, tyz
-        // are the x and y fluxes' (symmetric stress)
-        for (int m = 0; m < 12; ++m) {
-            if (m == 4 || m == 8 || m == 9) continue;
-            c->m.VF[m] = fq;
-            fq += ncell;
-        }
-        c->m.VF[4] = c->m.VF[1];
-        c->m.VF[8] = c->m.VF[2];
-        c->m.VF[9] = c->m.VF[6];
-        for (int m = 0; m < 5; ++m) {
-            c->m.U[m] = c->f.W[m];      // conserved state lives in the W arrays
-            c->m.U0[m] = c->f.U0[m];
-            c->m.out[m] = c->f.G[m];    // RHS of ns3d_compute_rhs
-        }
+        const int st2 = alloc_mixed(c);
+        if (st2) { ns3d_destroy(c); return st2; }
     }
     c->nblocks = ke_blocks(d);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_flag, 4 * sizeof(unsigned long long));
@@ -742,6 +752,43 @@ int ns3d_download_viscous_flux(ns3d_ctx *c, double *const flux[12])
         int st = copy_out(c, flux[m], c->f.VF[m]);
         if (st) return st;
     }
+    return stream_sync(c);
+}
+
+int ns3d_set_precision(ns3d_ctx *c, int precision)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    if (precision != NS3D_PREC_FP64 && precision != NS3D_PREC_MIXED) return NS3D_ERR_INVALID_ARG;
+    const bool to_mixed = precision == NS3D_PREC_MIXED;
+    if (to_mixed == c->mixed) return NS3D_OK;
+    c->fw_valid = false;
+    if (to_mixed) {
+        if (c->P.char_recon || !m_xrow_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
+        int st = alloc_mixed(c);
+        if (st) return st;
+        // conserved U (in place over the primitives) and the FP32 shadow
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.W, 5 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_from_prims(c->stream, c->dp, c->m, c->d_ptrs, 1));
+        c->mixed = true;
+    } else {
+        // primitives and T from U, in place
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.W, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_to_prims(c->stream, c->dp, c->m, c->d_ptrs));
+        c->mixed = false;
+    }
+    c->P.precision = precision;
+    int st = fill_ghosts(c);
+    if (st) return st;
+    return stream_sync(c);
+}
+
+int ns3d_compute_vis_flux(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
+    // rhs = 0, then the viscous divergence of the stored fluxes
+    HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[0], 0ull, (int)(5 * c->ncell)));
+    HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0ull));
     return stream_sync(c);
 }
 
