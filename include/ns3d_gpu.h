@@ -77,6 +77,8 @@ typedef struct ns3d_params {
     double mu_power;           /* exponent of the power law                   */
     double sw_eps;             /* 0 => plain |lambda| (InviscousFlux_new.cpp:461) */
     int precision;             /* NS3D_PREC_*                                 */
+    /* post-processing switches (field_structures.h:160-165) */
+    int post_basicfield, isotropic_analyse, monitor_res, monitor_energy;
     /* derived */
     double Cv, Cp, Rgas, mu;
     int stencil, ghost_layers;
@@ -172,6 +174,22 @@ int ns3d_advance(ns3d_ctx *ctx, int nsteps, double dt);
 int ns3d_compute_timestep_local(ns3d_ctx *ctx, double *dt);
 /* compute_total_energy (postprocess.cpp:103-128): sum 0.5 rho |u|^2 dV. */
 int ns3d_total_energy(ns3d_ctx *ctx, double *ek);
+
+/* ---- monitors (post-processing of the reference, computed on the device) */
+/* Shell spectrum E(k), k = 0 .. nbins-1 (the reference writes Kmax + 1 =
+ * 2*nx + 1 bins), of compute_energy_spectrum (EnergySpectrum.cpp:62-154):
+ * E(k) = sum over modes with floor(|k|) = k of (|u^|^2 + |v^|^2 + |w^|^2)/2,
+ * coefficients normalised by 1/(nx*ny*nz); shells >= nbins are dropped. */
+int ns3d_energy_spectrum(ns3d_ctx *ctx, double *Ek, int nbins);
+/* compute_turbulence_statistics (ana_isotropic.cpp:226-330) with the spectrum
+ * at full precision: out[0..12] = kinetic_energy, tke, u_rms, dissipation
+ * (2 nu sum k^2 E(k)), taylor, eta, u_eta, tau_eta, taylor_Li, Re_lambda,
+ * Mach_t, mean_mu, mean_rho.  n must be >= 13. */
+int ns3d_turbulence_statistics(ns3d_ctx *ctx, double *out, int n);
+/* compute_diagnostics (postprocess.cpp:7-96): out[m] = sqrt(sum (U_m -
+ * U0_m)^2 dx^3) / max |U_m| for rho, rho u, rho v, rho w, E, with U0 the
+ * state at the start of the last RK3 step. */
+int ns3d_residuals(ns3d_ctx *ctx, double out[5]);
 
 /* Switch the arithmetic of an existing context between NS3D_PREC_FP64 and
  * NS3D_PREC_MIXED, keeping its state (SolverParams has no counterpart: the

@@ -287,3 +287,26 @@ class Solver:
         b = C.c_longlong()
         self._check(self._L.ns3d_adj_memory(self.ctx, C.byref(b)), "adj_memory")
         return b.value
+
+    # -- monitors ------------------------------------------------------------
+    STAT_NAMES = ("kinetic_energy", "tke", "u_rms", "dissipation", "taylor", "eta", "u_eta", "tau_eta",
+                  "taylor_Li", "Re_lambda", "Mach_t", "mean_mu", "mean_rho")
+
+    def energy_spectrum(self, nbins=None):
+        """(k, E(k)) of the current velocity, computed on the device."""
+        nb = int(nbins) if nbins else 2 * self.grid.nx + 1
+        ek = np.zeros(nb)
+        self._check(self._L.ns3d_energy_spectrum(self.ctx, _ptr(ek), nb), "energy_spectrum")
+        return np.arange(nb), ek
+
+    def turbulence_statistics(self):
+        """The reference's isotropic-turbulence statistics, on the device."""
+        out = np.zeros(len(self.STAT_NAMES))
+        self._check(self._L.ns3d_turbulence_statistics(self.ctx, _ptr(out), len(out)), "turbulence_statistics")
+        return dict(zip(self.STAT_NAMES, out.tolist()))
+
+    def residuals(self):
+        """Normalised L2 change of each conserved variable over the last step."""
+        out = np.zeros(5)
+        self._check(self._L.ns3d_residuals(self.ctx, _ptr(out)), "residuals")
+        return out

@@ -35,6 +35,8 @@ class Params(C.Structure):
         ("output_timefreq", C.c_double), ("total_time", C.c_double),
         ("mu_law", C.c_int), ("sutherland_s", C.c_double), ("mu_power", C.c_double),
         ("sw_eps", C.c_double), ("precision", C.c_int),
+        ("post_basicfield", C.c_int), ("isotropic_analyse", C.c_int), ("monitor_res", C.c_int),
+        ("monitor_energy", C.c_int),
         ("Cv", C.c_double), ("Cp", C.c_double), ("Rgas", C.c_double), ("mu", C.c_double),
         ("stencil", C.c_int), ("ghost_layers", C.c_int),
     ]
@@ -78,6 +80,9 @@ SIGNATURES = {
     "ns3d_compute_rhs": (C.c_int, [C.c_void_p]),
     "ns3d_compute_vis_flux": (C.c_int, [C.c_void_p]),
     "ns3d_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "ns3d_energy_spectrum": (C.c_int, [C.c_void_p, DP, C.c_int]),
+    "ns3d_turbulence_statistics": (C.c_int, [C.c_void_p, DP, C.c_int]),
+    "ns3d_residuals": (C.c_int, [C.c_void_p, DP]),
     "ns3d_rk3_step": (C.c_int, [C.c_void_p, C.c_double]),
     "ns3d_nonphysical_cell": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 3),
     "ns3d_advance": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),

@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_api.cpp", "ns3d_params.cpp"]
+SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_monitors.hip", "ns3d_api.cpp",
+           "ns3d_params.cpp"]
 # the mixed-precision path and the derivative kernels do not reproduce the
 # reference's rounding, so their translation units contract a*b+c into FMAs
 # within an expression (-ffp-contract=on).  Not "fast": that lets the backend
@@ -59,7 +60,7 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"], verbose)
     os.replace(tmp, LIB)
     drv_src = os.path.join(HOST, "ns3d_gpu_run.cpp")
     if os.path.exists(drv_src):

@@ -4,7 +4,9 @@
 // the scratch it needs.  Every entry point returns a status; no exception
 // leaves this file.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -65,6 +67,13 @@ struct ns3d_ctx {
     double fw_dt = 0.0;
     bool fw_valid = false;          // a forward pass matches the current state
     bool fw_last_valid = false;     // stage inputs of the last segment are stored
+    // monitors, allocated on first use
+    hipfftHandle fft = 0;
+    bool fft_ready = false;
+    double *fft_in = nullptr;       // interior of one velocity component
+    void *fft_out = nullptr;        // its R2C transform
+    double *mon_part = nullptr;     // per-block partials of the monitor kernels
+    long long mon_part_n = 0;
 };
 
 namespace {
@@ -415,6 +424,83 @@ int copy_state(ns3d_ctx *c, double *dst, const double *src)
     return NS3D_OK;
 }
 
+
+// ---------------------------------------------------------------- monitors
+
+int ensure_mon_part(ns3d_ctx *c, long long n)
+{
+    if (c->mon_part_n >= n) return NS3D_OK;
+    if (c->mon_part) { HIPCALL(c, hipFree(c->mon_part)); c->mon_part = nullptr; c->mon_part_n = 0; }
+    HIPCALL(c, hipMalloc((void **)&c->mon_part, (size_t)n * sizeof(double)));
+    c->mon_part_n = n;
+    return NS3D_OK;
+}
+
+int ensure_fft(ns3d_ctx *c)
+{
+    if (c->fft_ready) return NS3D_OK;
+    const size_t n = (size_t)c->dp.nx * c->dp.ny * c->dp.nz;
+    const size_t nc = (size_t)(c->dp.nx / 2 + 1) * c->dp.ny * c->dp.nz;
+    HIPCALL(c, hipMalloc((void **)&c->fft_in, n * sizeof(double)));
+    HIPCALL(c, hipMalloc(&c->fft_out, nc * 2 * sizeof(double)));
+    if (hipfftPlan3d(&c->fft, c->dp.nz, c->dp.ny, c->dp.nx, HIPFFT_D2Z) != HIPFFT_SUCCESS) {
+        c->err = "hipfftPlan3d failed";
+        return NS3D_ERR_HIP;
+    }
+    c->fft_ready = true;
+    if (hipfftSetStream(c->fft, c->stream) != HIPFFT_SUCCESS) {
+        c->err = "hipfftSetStream failed";
+        return NS3D_ERR_HIP;
+    }
+    return NS3D_OK;
+}
+
+// device pointers of rho, u, v, w, p, T in d_ptrs[0..5]: the primitives
+// (FP64 mode) or C2P of U into the gradient scratch (mixed mode)
+int prims_on_device(ns3d_ctx *c)
+{
+    double *const *src = c->f.W;
+    if (c->mixed) {
+        HIPCALL(c, hipMemcpyAsync(c->d_ptrs, c->f.G, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(c, launch_m_to_prims(c->stream, c->dp, c->m, c->d_ptrs));
+        src = c->f.G;
+    }
+    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, src, 6 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+    return NS3D_OK;
+}
+
+// shell spectrum into Ek[0..nbins) (device arrays from prims_on_device)
+int spectrum(ns3d_ctx *c, double *Ek, int nbins)
+{
+    int st = ensure_fft(c);
+    if (!st) st = ensure_mon_part(c, (long long)monitor_blocks() * nbins);
+    if (st) return st;
+    double *host_ptrs[6];
+    HIPCALL(c, hipMemcpyAsync(host_ptrs, c->d_ptrs, 6 * sizeof(double *), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    const double N = (double)c->dp.nx * c->dp.ny * c->dp.nz;
+    for (int q = 1; q <= 3; ++q) {
+        TIMED(c, "spectrum", launch_pack(c->stream, c->dp, host_ptrs[q], c->fft_in));
+        if (hipfftExecD2Z(c->fft, (hipfftDoubleReal *)c->fft_in, (hipfftDoubleComplex *)c->fft_out) !=
+            HIPFFT_SUCCESS) {
+            c->err = "hipfftExecD2Z failed";
+            return NS3D_ERR_HIP;
+        }
+        TIMED(c, "spectrum", launch_spec_bins(c->stream, c->dp, c->fft_out, c->mon_part, nbins, 1.0 / (N * N),
+                                              q > 1));
+    }
+    std::vector<double> h((size_t)monitor_blocks() * nbins);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->mon_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    for (int b = 0; b < nbins; ++b) {
+        std::vector<double> col((size_t)monitor_blocks());
+        for (int q = 0; q < monitor_blocks(); ++q) col[q] = h[(size_t)q * nbins + b];
+        Ek[b] = compensated_sum(col);
+    }
+    return NS3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -555,6 +641,10 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->adj_part) (void)hipFree(c->adj_part);
     if (c->adj_sums) (void)hipFree(c->adj_sums);
     if (c->ck_slab) (void)hipFree(c->ck_slab);
+    if (c->fft_ready) (void)hipfftDestroy(c->fft);
+    if (c->fft_in) (void)hipFree(c->fft_in);
+    if (c->fft_out) (void)hipFree(c->fft_out);
+    if (c->mon_part) (void)hipFree(c->mon_part);
     for (auto &q : c->pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -790,6 +880,89 @@ int ns3d_compute_vis_flux(ns3d_ctx *c)
     HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[0], 0ull, (int)(5 * c->ncell)));
     HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0ull));
     return stream_sync(c);
+}
+
+int ns3d_energy_spectrum(ns3d_ctx *c, double *Ek, int nbins)
+{
+    if (!c || !Ek || nbins < 1) return NS3D_ERR_INVALID_ARG;
+    int st = prims_on_device(c);
+    if (st) return st;
+    return spectrum(c, Ek, nbins);
+}
+
+int ns3d_turbulence_statistics(ns3d_ctx *c, double *out, int n)
+{
+    if (!c || !out || n < 13) return NS3D_ERR_INVALID_ARG;
+    int st = prims_on_device(c);
+    if (st) return st;
+    const int nbins = 2 * c->dp.nx + 1;      // Kmax = 2 NX (EnergySpectrum.cpp:108)
+    std::vector<double> ek((size_t)nbins);
+    st = spectrum(c, ek.data(), nbins);
+    if (!st) st = ensure_mon_part(c, (long long)monitor_blocks() * 12);
+    if (st) return st;
+    TIMED(c, "statistics", launch_stats(c->stream, c->dp, c->d_ptrs, c->mon_part));
+    std::vector<double> h((size_t)monitor_blocks() * 12);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->mon_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    double sum[12];
+    for (int q = 0; q < 12; ++q) {
+        std::vector<double> col((size_t)monitor_blocks());
+        for (int b = 0; b < monitor_blocks(); ++b) col[b] = h[(size_t)b * 12 + q];
+        sum[q] = compensated_sum(col);
+    }
+    // ana_isotropic.cpp:272-315
+    const double N = (double)c->dp.nx * c->dp.ny * c->dp.nz;
+    const double mean_mu = sum[10] / N, mean_rho = sum[11] / N;
+    const double nu = mean_mu / mean_rho;
+    const double ke = sum[0] / N;
+    const double u_rms = std::sqrt(sum[1] / (3.0 * N));
+    const double mu_ = sum[2] / N, mv = sum[3] / N, mw = sum[4] / N;
+    const double tke = 0.5 * ((sum[5] / N - mu_ * mu_) + (sum[6] / N - mv * mv) + (sum[7] / N - mw * mw));
+    const double taylor_li = u_rms / std::sqrt(sum[8] / N);
+    const double mach_t = std::sqrt(ke * 2.0) / (sum[9] / N);
+    const double re_lambda = u_rms * taylor_li / nu;
+    double eps = 0.0;
+    for (int k = 0; k < nbins; ++k) eps += 2.0 * nu * (double)k * (double)k * ek[(size_t)k];
+    out[0] = ke;
+    out[1] = tke;
+    out[2] = u_rms;
+    out[3] = eps;
+    out[4] = std::sqrt(15.0 * nu * u_rms * u_rms / eps);
+    out[5] = std::pow(nu * nu * nu / eps, 0.25);
+    out[6] = std::pow(nu * eps, 0.25);
+    out[7] = std::pow(nu / eps, 0.5);
+    out[8] = taylor_li;
+    out[9] = re_lambda;
+    out[10] = mach_t;
+    out[11] = mean_mu;
+    out[12] = mean_rho;
+    return NS3D_OK;
+}
+
+int ns3d_residuals(ns3d_ctx *c, double out[5])
+{
+    if (!c || !out) return NS3D_ERR_INVALID_ARG;
+    int st = ensure_mon_part(c, (long long)monitor_blocks() * 10);
+    if (st) return st;
+    double *ptrs[10];
+    for (int m = 0; m < 5; ++m) { ptrs[m] = c->f.W[m]; ptrs[5 + m] = c->f.U0[m]; }
+    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, ptrs, 10 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+    TIMED(c, "residuals", launch_resid(c->stream, c->dp, c->d_ptrs, c->d_ptrs + 5, c->mixed ? 1 : 0, c->mon_part));
+    std::vector<double> h((size_t)monitor_blocks() * 10);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->mon_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st = stream_sync(c);
+    if (st) return st;
+    for (int m = 0; m < 5; ++m) {
+        std::vector<double> col((size_t)monitor_blocks());
+        double mx = 0.0;
+        for (int b = 0; b < monitor_blocks(); ++b) {
+            col[b] = h[(size_t)b * 10 + m];
+            mx = std::max(mx, h[(size_t)b * 10 + 5 + m]);
+        }
+        out[m] = std::sqrt(compensated_sum(col)) / mx;   // postprocess.cpp:89-93
+    }
+    return NS3D_OK;
 }
 
 int ns3d_compute_rhs(ns3d_ctx *c)

@@ -98,4 +98,13 @@ bool adj_xrow(const DevParams &P);
 hipError_t launch_dissip(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 hipError_t launch_sum_partials(hipStream_t s, const double *partial, int n, double *out);
 
+// monitors (ns3d_monitors.hip)
+int monitor_blocks();
+hipError_t launch_pack(hipStream_t s, const DevParams &P, const double *src, double *dst);
+hipError_t launch_spec_bins(hipStream_t s, const DevParams &P, const void *U, double *partial, int nbins,
+                            double scale, int accumulate);
+hipError_t launch_stats(hipStream_t s, const DevParams &P, const double *const *W, double *partial);
+hipError_t launch_resid(hipStream_t s, const DevParams &P, const double *const *Wc, const double *const *U0,
+                        int conserved, double *partial);
+
 }  // namespace ns3d

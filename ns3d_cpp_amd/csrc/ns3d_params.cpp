@@ -24,6 +24,12 @@ std::string lower(std::string s)
 }
 
 // parse_bc of initialization.cpp:288-294 (anything else is Periodic)
+// parse_bool (initialization.cpp:28-32), on an already lower-cased value
+int parse_bool(const std::string &lv)
+{
+    return lv == "true" || lv == "yes" || lv == "1" || lv == "on";
+}
+
 int parse_bc(const std::string &v)
 {
     if (v == "inflow") return NS3D_BC_INFLOW;
@@ -63,6 +69,10 @@ extern "C" void ns3d_params_default(ns3d_params *p, ns3d_grid *g)
         p->mu_power = 0.76;
         p->sw_eps = 0.0;
         p->precision = NS3D_PREC_FP64;
+        p->post_basicfield = 1;        // field_structures.h:160-165
+        p->isotropic_analyse = 1;
+        p->monitor_res = 1;
+        p->monitor_energy = 1;
         p->mu = 1.0 / p->Re;
         p->Cv = 1.0 / (p->gamma * (p->gamma - 1.0) * p->Ma * p->Ma);
         p->Cp = p->Cv * p->gamma;
@@ -172,6 +182,11 @@ extern "C" int ns3d_params_from_file(const char *path, ns3d_params *p, ns3d_grid
             else if (k == "monitor_stepfreq") p->monitor_stepfreq = std::stoi(val);
             else if (k == "output_timefreq") p->output_timefreq = std::stod(val);
             else if (k == "totaltime") p->total_time = std::stod(val);
+            // initialization.cpp:304-307 with parse_bool (:28-32)
+            else if (k == "post_basicfield") p->post_basicfield = parse_bool(v);
+            else if (k == "isotropicanalyse") p->isotropic_analyse = parse_bool(v);
+            else if (k == "monitor_res") p->monitor_res = parse_bool(v);
+            else if (k == "monitor_energy") p->monitor_energy = parse_bool(v);
             // keys absent from the reference's parser (mu law, SW epsilon, precision)
             else if (k == "mu_law") {
                 if (v == "constant") p->mu_law = NS3D_MU_CONSTANT;
