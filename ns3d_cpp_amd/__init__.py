@@ -7,6 +7,7 @@ host wrapper over that ABI; it never computes anything on the CPU, and it
 raises if the library is missing.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -67,6 +68,16 @@ def make_params(n, L=2.0 * np.pi, **kw):
     st = L_.ns3d_params_finalize(C.byref(p), C.byref(g))
     if st:
         raise NS3DError(st, "ns3d_params_finalize")
+    return p, g
+
+
+def read_params(path):
+    """(Params, Grid) of a solver.in file (read_solver_params_from_file,
+    initialization.cpp:199-380), finalized."""
+    p, g = Params(), Grid()
+    st = lib().ns3d_params_from_file(os.fsencode(path), C.byref(p), C.byref(g))
+    if st:
+        raise NS3DError(st, f"ns3d_params_from_file({path})")
     return p, g
 
 

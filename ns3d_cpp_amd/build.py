@@ -65,7 +65,7 @@ def build(verbose=True):
     drv_src = os.path.join(HOST, "ns3d_gpu_run.cpp")
     if os.path.exists(drv_src):
         _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", HOST,
-              drv_src, os.path.join(HOST, "ns3d_field.cpp"), "-o", DRIVER,
+              drv_src, os.path.join(HOST, "ns3d_field.cpp"), os.path.join(HOST, "ns3d_io.cpp"), "-o", DRIVER,
               f"-L{HERE}", "-lns3d_gpu", f"-Wl,-rpath,{HERE}"], verbose)
     return LIB
 

@@ -1,8 +1,10 @@
 // ns3d_field.cpp -- the reference call surface over the C ABI (see header).
 #include "ns3d_field.hpp"
+#include "ns3d_io.hpp"
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <iomanip>
 #include <iostream>
 
@@ -98,6 +100,78 @@ void compute_total_energy(Field3D &F, const GridDesc &, const SolverParams &)
     check(ns3d_total_energy(F.ctx, &F.global_Etot), "total_energy", &F);
 }
 
+namespace {
+
+ZoneDesc zone_of(const Field3D &F, const GridDesc &G)
+{
+    return ZoneDesc{F.nx, F.ny, F.nz, F.ng, F.sx, F.sy, G.x0, G.y0, G.z0, G.dx, G.dy, G.dz};
+}
+
+}  // namespace
+
+void compute_diagnostics(Field3D &F, const SolverParams &, const GridDesc &)
+{
+    check(ns3d_residuals(F.ctx, F.global_res), "residuals", &F);
+}
+
+void write_residuals_tecplot(const Field3D &F, int step, const std::string &filename)
+{
+    write_residual_line("output/" + filename, step, F.global_res, F.global_Etot);
+}
+
+void compute_energy_spectrum(Field3D &F, const GridDesc &G, const std::string &filename)
+{
+    std::vector<double> Ek((size_t)(2 * G.nx + 1));   // Kmax = 2 NX (EnergySpectrum.cpp:108)
+    check(ns3d_energy_spectrum(F.ctx, Ek.data(), (int)Ek.size()), "energy_spectrum", &F);
+    write_spectrum(filename, Ek);
+    std::cout << "[Spectrum] Energy spectrum computed.\n";
+}
+
+void compute_turbulence_statistics(Field3D &F, const GridDesc &, const SolverParams &, double current_time)
+{
+    double o[13];
+    check(ns3d_turbulence_statistics(F.ctx, o, 13), "turbulence_statistics", &F);
+    TurbStats s{};
+    s.kinetic_energy = o[0]; s.tke = o[1]; s.u_rms = o[2];
+    s.taylor_Li = o[8]; s.Re_lambda = o[9]; s.Mach_t = o[10]; s.mean_mu = o[11]; s.mean_rho = o[12];
+    std::vector<double> k, Ek;
+    read_spectrum(time_prefix(current_time) + "_spectrum.dat", k, Ek);
+    dissipation_scales(s, k, Ek);
+    std::cout << "[Post] mean_mu = " << s.mean_mu << ", mean_rho = " << s.mean_rho << ", u_rms = " << s.u_rms
+              << ", Taylor_Li = " << s.taylor_Li << ", Mach_t = " << s.Mach_t << "\n";
+    append_turbulence_stats("output/turbulence_stats.dat", current_time, s);
+}
+
+void isotropic_post_process(Field3D &F, const GridDesc &G, const SolverParams &P, double current_time)
+{
+    compute_energy_spectrum(F, G, time_prefix(current_time) + "_spectrum.dat");
+    compute_turbulence_statistics(F, G, P, current_time);
+}
+
+void write_tecplot_field(Field3D &F, const GridDesc &G, const SolverParams &, double time)
+{
+    F.sync_to_host();
+    const std::string path = tecplot_field_path(time, 0);
+    if (write_tecplot(path, time, 0, zone_of(F, G), F.rho.data(), F.u.data(), F.v.data(), F.w.data(), F.E.data(),
+                      F.p.data(), F.T.data())) {
+        char t[64];
+        std::snprintf(t, sizeof(t), "%.8e", time);
+        std::cerr << "Wrote Tecplot file: " << path << " (rank 0, time=" << t << ")\n";
+    }
+}
+
+bool initialize_from_tecplot(Field3D &F, const GridDesc &G, const SolverParams &, const std::string &filename)
+{
+    return read_tecplot(filename, zone_of(F, G), F.rho.data(), F.u.data(), F.v.data(), F.w.data(), F.p.data());
+}
+
+bool initialize_from_tecplot_downsample(Field3D &F, const GridDesc &G, const SolverParams &,
+                                        const std::string &filename, int src_nx, int src_ny, int src_nz)
+{
+    return read_tecplot_downsample(filename, zone_of(F, G), src_nx, src_ny, src_nz, F.rho.data(), F.u.data(),
+                                   F.v.data(), F.w.data(), F.p.data());
+}
+
 void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
 {
     using clk = std::chrono::steady_clock;
@@ -111,27 +185,44 @@ void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
         const double eps = 1e-12;
         if (std::abs(TotalTime - current_time) <= eps) break;
         if (P.dt_fixed < 0.0) dt = compute_timestep(F, G, P);
-        if (current_time >= output_Timefreq - eps) output_Timefreq += P.output_timefreq;
+        bool if_output = false;
+        if (current_time >= output_Timefreq - eps) {
+            if_output = true;
+            output_Timefreq += P.output_timefreq;
+        }
         if (current_time + dt > TotalTime) dt = TotalTime - current_time;
         if (current_time + dt > output_Timefreq) {
             dt = output_Timefreq - current_time;
             output_Timefreq += P.output_timefreq;
+            if_output = true;
         }
         if (dt <= eps) {
             std::cerr << "dt too small or zero; aborting to avoid stall at Time=" << current_time << "\n";
             break;
         }
         current_time += dt;
+        // U0 := U is taken inside the step (recordConservedTo0, maincycle.cpp:21)
         runge_kutta_3(F, G, P, dt);
+        std::cout << "Completed step " << step << " / " << P.max_steps << ", Time = " << current_time << " / "
+                  << TotalTime << "\n";
         if (step % P.monitor_stepfreq == 0 || step == 1) {
-            compute_total_energy(F, G, P);
+            if (P.monitor_res) compute_diagnostics(F, P, G);
+            if (P.monitor_energy) compute_total_energy(F, G, P);
             const auto t_now = clk::now();
             const double t_elapsed = std::chrono::duration<double>(t_now - t_start).count();
             const double t_step = std::chrono::duration<double>(t_now - t_last).count();
             t_last = t_now;
             std::cout << std::fixed << std::setprecision(6) << "[Step " << step << "] "
                       << "dt=" << dt << "  Time/step=" << t_step << "s  Elapsed=" << t_elapsed << "s"
-                      << "  Time=" << current_time << "/" << TotalTime << "  Ek_avg=" << F.global_Etot << "\n";
+                      << "  Time=" << current_time << "/" << TotalTime;
+            if (P.monitor_energy) std::cout << "  Ek_avg=" << F.global_Etot;
+            std::cout << "\n";
+            if (P.monitor_res) write_residuals_tecplot(F, step, "output_residuals.dat");
+            if (P.isotropic_analyse) isotropic_post_process(F, G, P, current_time);
+        }
+        if (if_output) {
+            if (P.post_basicfield) write_tecplot_field(F, G, P, current_time);
+            if (P.isotropic_analyse) isotropic_post_process(F, G, P, current_time);
         }
         if (current_time >= TotalTime) break;
     }

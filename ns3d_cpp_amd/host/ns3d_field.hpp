@@ -34,6 +34,8 @@ struct Field3D {
     std::vector<double> rho, u, v, w, p, T;
     std::vector<double> rhou, rhov, rhow, E;
     double global_Etot = 0.0;
+    // compute_diagnostics results: rho, rho u, rho v, rho w, E
+    double global_res[5] = {0, 0, 0, 0, 0};
 
     Field3D() = default;
     Field3D(const Field3D &) = delete;
@@ -60,8 +62,25 @@ void compute_rhs(Field3D &F, GridDesc &G, SolverParams &P);                // :9
 void runge_kutta_3(Field3D &F, GridDesc &G, SolverParams &P, double dt);   // :102
 double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P);  // timestep.cpp:5
 void compute_total_energy(Field3D &F, const GridDesc &G, const SolverParams &P); // postprocess.cpp:103
+void compute_diagnostics(Field3D &F, const SolverParams &P, const GridDesc &G);   // postprocess.cpp:7
+void write_residuals_tecplot(const Field3D &F, int step, const std::string &filename);   // output.cpp:96
+// spectrum of the device state written as "k Ek" lines (EnergySpectrum.cpp:159)
+void compute_energy_spectrum(Field3D &F, const GridDesc &G, const std::string &filename);
+// statistics appended to output/turbulence_stats.dat; like the reference,
+// the dissipation is taken from the spectrum file that
+// isotropic_post_process wrote for this time (ana_isotropic.cpp:207-219)
+void compute_turbulence_statistics(Field3D &F, const GridDesc &G, const SolverParams &P, double current_time);
+void isotropic_post_process(Field3D &F, const GridDesc &G, const SolverParams &P, double current_time);  // postprocess.cpp:133
+void write_tecplot_field(Field3D &F, const GridDesc &G, const SolverParams &P, double time);   // output.cpp:21
+// initial field from a Tecplot file of the grid's size, or resampled from one
+// of src sizes (initialization.cpp:35, :103); fills the host mirrors of rho,
+// u, v, w, p -- upload_primitive() then applies the boundary and P2C
+bool initialize_from_tecplot(Field3D &F, const GridDesc &G, const SolverParams &P, const std::string &filename);
+bool initialize_from_tecplot_downsample(Field3D &F, const GridDesc &G, const SolverParams &P,
+                                        const std::string &filename, int src_nx, int src_ny, int src_nz);
 // time_advance (maincycle.cpp:7-113): dt fixed or CFL, dt clipping to the
-// output time and TotalTime, monitor lines on rank 0's stdout.
+// output time and TotalTime, monitor lines on stdout, residual and
+// turbulence histories, field files at output times.
 void time_advance(Field3D &F, GridDesc &G, SolverParams &P);
 
 }  // namespace ns3d_host

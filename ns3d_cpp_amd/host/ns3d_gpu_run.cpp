@@ -1,14 +1,22 @@
 // ns3d_gpu_run.cpp -- the main.cpp counterpart (main.cpp:9-86) on one GPU:
-// read solver.in, allocate the device field, set the initial field, run
-// time_advance.
+// read solver.in, allocate the device field, set the initial field, write
+// it (and its spectrum and statistics when isotropic_analyse is on), run
+// time_advance, write the final field.
 //
-// usage: ns3d_gpu_run <solver.in> [--ic tgv|uniform|raw:<file>]
+// usage: ns3d_gpu_run <solver.in> [--ic <initial field>] [--no-field-files]
 //   tgv       Taylor-Green vortex at x_i = x0 + i*dx (rho = 1,
 //             p = 1/(gamma Ma^2) + (cos 2x + cos 2y)(cos 2z + 2)/16)
 //   uniform   initialize_uniform_field (initialization.cpp:528-546)
 //   raw:<f>   five interior arrays rho,u,v,w,p as little-endian doubles,
 //             i fastest (the interior of the idx3 layout)
+//   tecplot:<f>                  a field file of the grid's size
+//                                (initialize_from_tecplot)
+//   tecplot_ds:<f>:<nx>:<ny>:<nz> a field file of nx*ny*nz points resampled
+//                                onto the grid (initialize_from_tecplot_downsample)
+// --no-field-files skips the field files at time 0 and at the end
+// (main.cpp:76, :83); the files at output times follow post_basicfield.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -70,8 +78,16 @@ int main(int argc, char **argv)
         return 2;
     }
     std::string ic = "tgv";
-    for (int a = 2; a + 1 < argc; ++a)
-        if (std::string(argv[a]) == "--ic") ic = argv[a + 1];
+    bool field_files = true;
+    for (int a = 2; a < argc; ++a) {
+        const std::string arg = argv[a];
+        if (arg == "--ic" && a + 1 < argc) ic = argv[++a];
+        else if (arg == "--no-field-files") field_files = false;
+        else {
+            std::cerr << "unknown argument " << arg << "\n";
+            return 2;
+        }
+    }
     SolverParams P;
     GridDesc G;
     int st = ns3d_params_from_file(argv[1], &P, &G);
@@ -92,13 +108,32 @@ int main(int argc, char **argv)
                 std::cerr << "Error: cannot read " << ic.substr(4) << "\n";
                 return 1;
             }
+        } else if (ic.rfind("tecplot:", 0) == 0) {
+            if (!initialize_from_tecplot(F, G, P, ic.substr(8))) return 1;
+        } else if (ic.rfind("tecplot_ds:", 0) == 0) {
+            // tecplot_ds:<file>:<nx>:<ny>:<nz>, the sizes after the last three colons
+            std::string rest = ic.substr(11);
+            int n[3];
+            for (int q = 2; q >= 0; --q) {
+                const size_t c = rest.rfind(':');
+                if (c == std::string::npos) {
+                    std::cerr << "--ic tecplot_ds:<file>:<nx>:<ny>:<nz>\n";
+                    return 2;
+                }
+                n[q] = std::atoi(rest.c_str() + c + 1);
+                rest.resize(c);
+            }
+            if (!initialize_from_tecplot_downsample(F, G, P, rest, n[0], n[1], n[2])) return 1;
         } else {
             std::cerr << "unknown --ic " << ic << "\n";
             return 2;
         }
         F.upload_primitive();
         std::cout << "Initialization + halo exchange done\n";
+        if (field_files) write_tecplot_field(F, G, P, 0.0);
+        if (P.isotropic_analyse) isotropic_post_process(F, G, P, 0.0);
         time_advance(F, G, P);
+        if (field_files) write_tecplot_field(F, G, P, P.total_time);
     } catch (const Error &e) {
         std::cerr << "Error: " << e.what() << "\n";
         return 1;

@@ -1,0 +1,229 @@
+"""The host driver (ns3d_cpp_amd/ns3d_gpu_run, the main.cpp + time_advance
+counterpart) end to end: a TGV run with every history switch on, checked
+against the same run made through the Python binding of the C ABI (identical
+kernels, so the field files agree digit for digit), then a restart from the
+written Tecplot file and a resampled start from a finer one.
+
+The driver's time loop (maincycle.cpp:18-102) is replayed here to know which
+steps monitor and which write files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import ns3d_cpp_amd as ns
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "ns3d_cpp_amd", "ns3d_gpu_run")
+
+
+def solver_in(path, n, max_steps, dt=1e-3, monitor=3, out_freq=0.0025, total=1.0, **extra):
+    text = f"""# test case
+Re = 400
+Ma = 0.1
+gamma = 1.4
+Pr = 0.72
+cfl = 0.5
+dt_fixed = {dt!r}
+fvs_type = StagerWarming
+recon = WENO5
+vis_scheme = C8th
+global_nx = {n}
+global_ny = {n}
+global_nz = {n}
+Lx = 6.283185307179586
+Ly = 6.283185307179586
+Lz = 6.283185307179586
+max_steps = {max_steps}
+monitor_Stepfreq = {monitor}
+output_Timefreq = {out_freq!r}
+TotalTime = {total!r}
+Post_BasicField = True
+IsotropicAnalyse = True
+monitor_Res = True
+monitor_Energy = True
+"""
+    for k, v in extra.items():
+        text += f"{k} = {v}\n"
+    path.write_text(text)
+    return str(path)
+
+
+def schedule(P):
+    """(step, dt, time, if_output) of time_advance (maincycle.cpp:18-56)."""
+    t, of, out = 0.0, P.output_timefreq, []
+    for step in range(1, P.max_steps + 1):
+        dt, eps = P.dt_fixed, 1e-12
+        if abs(P.total_time - t) <= eps:
+            break
+        iff = False
+        if t >= of - eps:
+            iff = True
+            of += P.output_timefreq
+        if t + dt > P.total_time:
+            dt = P.total_time - t
+        if t + dt > of:
+            dt = of - t
+            of += P.output_timefreq
+            iff = True
+        if dt <= eps:
+            break
+        t += dt
+        out.append((step, dt, t, iff))
+        if t >= P.total_time:
+            break
+    return out
+
+
+def run_driver(cwd, inp, *args):
+    print("driver", inp, *args, flush=True)
+    r = subprocess.run([DRIVER, inp, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def field_lines(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    z = next(n for n, line in enumerate(lines) if line.startswith("ZONE"))
+    return lines[z + 1:]
+
+
+def expected_lines(s, g):
+    """Tecplot point lines of a context's current state (output.cpp:55-82)."""
+    pr, U = s.primitive(), s.conserved()
+    it = s.interior
+    cols = [it(pr["rho"]), it(pr["u"]), it(pr["v"]), it(pr["w"]), it(U[4]), it(pr["p"]), it(pr["T"])]
+    n = cols[0].shape
+    out = []
+    for k in range(n[0]):
+        for j in range(n[1]):
+            for i in range(n[2]):
+                xyz = (g.x0 + i * g.dx, g.y0 + j * g.dy, g.z0 + k * g.dz)
+                out.append(" ".join("%.8e" % v for v in (*xyz, *(c[k, j, i] for c in cols))))
+    return out
+
+
+def test_driver_histories_and_field_files(gpu, tmp_path):
+    n = 16
+    inp = solver_in(tmp_path / "solver.in", n, max_steps=7)
+    P, G = ns.read_params(inp)
+    x = G.x0 + np.arange(n) * G.dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    st = [np.ones_like(X), np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X,
+          1.0 / (P.gamma * P.Ma ** 2) + (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0) / 16.0]
+    with open(tmp_path / "ic.raw", "wb") as f:
+        for a in st:
+            f.write(np.ascontiguousarray(a).tobytes())
+    log = run_driver(tmp_path, inp, "--ic", "raw:ic.raw")
+    assert "Simulation complete" in log
+
+    s = ns.Solver(P, G)
+    s.upload_primitive(*st)
+    out = tmp_path / "output"
+    assert field_lines(out / "time_0" / "field_rank0.dat") == expected_lines(s, G)
+
+    sched = schedule(P)
+    assert [e[0] for e in sched] == list(range(1, 8)) and sum(e[3] for e in sched) == 2
+    post = []            # (time, spectrum, stats) per isotropic_post_process call
+
+    def post_process(t):
+        post.append((t, s.energy_spectrum()[1], s.turbulence_statistics()))
+    post_process(0.0)
+    resid = []
+    for step, dt, t, iff in sched:
+        s.rk3_step(dt)
+        if step % P.monitor_stepfreq == 0 or step == 1:
+            resid.append((step, s.residuals(), s.total_energy()))
+            post_process(t)
+        if iff:
+            lines = field_lines(out / f"time_{int(t * 100000)}" / "field_rank0.dat")
+            assert lines == expected_lines(s, G), t
+            post_process(t)
+    # the final file carries TotalTime in its name (main.cpp:83)
+    assert field_lines(out / f"time_{int(P.total_time * 100000)}" / "field_rank0.dat") == expected_lines(s, G)
+
+    # residual history (output.cpp:93-128)
+    rl = (out / "output_residuals.dat").read_text().splitlines()
+    assert rl[:2] == ['TITLE = "Residuals vs Step"',
+                      'VARIABLES = "Step" "Res_rho" "Res_rhou" "Res_rhov" "Res_rhow" "Res_E" "Etot"']
+    assert len(rl) == 2 + len(resid)
+    for line, (step, res, etot) in zip(rl[2:], resid):
+        v = line.split()
+        assert int(v[0]) == step
+        np.testing.assert_allclose([float(a) for a in v[1:]], [*res, etot], rtol=1e-8, atol=0)
+
+    # spectrum files: the last call at each time wins; six significant digits
+    for t, ek, _ in post:
+        path = out / ("output_time_%.5f_spectrum.dat" % t)
+        got = np.loadtxt(path)
+        assert np.array_equal(got[:, 0], np.arange(2 * n + 1))
+        want = [t2 for t2 in post if t2[0] == t][-1][1]
+        # bins at round-off level differ with the order of the device's atomic adds
+        np.testing.assert_allclose(got[:, 1], want, rtol=6e-6, atol=1e-14 * np.max(want))
+
+    # turbulence statistics: one line per call, the dissipation and its
+    # scales from the six-digit spectrum of the file
+    tl = (out / "turbulence_stats.dat").read_text().splitlines()
+    assert tl[0].startswith("current_time Kinetic Energy TKE")
+    assert len(tl) == 1 + len(post)
+    order = ["kinetic_energy", "tke", "u_rms", "dissipation", "taylor", "eta", "u_eta", "tau_eta", "taylor_Li",
+             "Re_lambda", "Mach_t"]
+    for line, (t, _, stats) in zip(tl[1:], post):
+        v = [float(a) for a in line.split()]
+        assert v[0] == pytest.approx(t, rel=1e-8, abs=1e-300)
+        for name, val in zip(order, v[1:]):
+            tol = 2e-5 if name in ("dissipation", "taylor", "eta", "u_eta", "tau_eta") else 1e-8
+            assert val == pytest.approx(stats[name], rel=tol), name
+    s.close()
+
+
+def test_driver_restart_and_resample_from_tecplot(gpu, tmp_path):
+    # a 32^3 file at eight digits, read back on 32^3 (restart) and resampled
+    # onto 16^3: target cell i takes source llround(2 i) = 2 i
+    rng = np.random.default_rng(9)
+    src = tmp_path / "src.dat"
+    m = 32
+    Lx = 6.283185307179586
+    vals = np.empty((m, m, m, 10))
+    vals[..., 0:3] = 0.0
+    vals[..., 3] = 1.0 + 0.05 * rng.standard_normal((m, m, m))
+    vals[..., 4:7] = 0.1 * rng.standard_normal((m, m, m, 3))
+    vals[..., 7] = 0.0
+    vals[..., 8] = 1.0 / (1.4 * 0.01) + 0.1 * rng.standard_normal((m, m, m))
+    vals[..., 9] = 0.0
+    for k in range(m):
+        for j in range(m):
+            for i in range(m):
+                vals[k, j, i, :3] = (i * (Lx / m), j * (Lx / m), k * (Lx / m))   # x0 + i dx
+    text = ["TITLE = \"src\"", "VARIABLES = \"X\" \"Y\" \"Z\" \"rho\" \"u\" \"v\" \"w\" \"E\" \"p\" \"T\"",
+            f"ZONE T=\"src\" I={m} J={m} K={m} DATAPACKING=POINT"]
+    text += [" ".join("%.8e" % a for a in row) for row in vals.reshape(-1, 10)]
+    src.write_text("\n".join(text) + "\n")
+    src_lines = text[3:]
+
+    def cols(line, idx):
+        f = line.split()
+        return [f[q] for q in idx]
+    keep = [0, 1, 2, 3, 4, 5, 6, 8]          # X Y Z rho u v w p (E and T are recomputed)
+
+    d1 = tmp_path / "restart"
+    d1.mkdir()
+    inp = solver_in(d1 / "solver.in", m, max_steps=0)
+    run_driver(d1, inp, "--ic", f"tecplot:{src}")
+    got = field_lines(d1 / "output" / "time_0" / "field_rank0.dat")
+    assert [cols(a, keep) for a in got] == [cols(b, keep) for b in src_lines]
+
+    d2 = tmp_path / "resample"
+    d2.mkdir()
+    n = 16
+    inp = solver_in(d2 / "solver.in", n, max_steps=0)
+    run_driver(d2, inp, "--ic", f"tecplot_ds:{src}:{m}:{m}:{m}")
+    got = field_lines(d2 / "output" / "time_0" / "field_rank0.dat")
+    src_grid = np.array(src_lines, dtype=object).reshape(m, m, m)
+    want = [src_grid[2 * k, 2 * j, 2 * i] for k in range(n) for j in range(n) for i in range(n)]
+    data = [3, 4, 5, 6, 8]
+    assert [cols(a, data) for a in got] == [cols(b, data) for b in want]
