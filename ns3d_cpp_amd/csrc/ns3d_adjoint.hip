@@ -688,8 +688,9 @@ int ad_blocks(const DevParams &P) { return ((P.nx + 63) / 64) * ((P.ny + 3) / 4)
 
 bool ad_supported(const DevParams &P)
 {
-    return P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_UD7 ||
-           P.recon == R_UD7L;
+    return (P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_UD7 ||
+            P.recon == R_UD7L) &&
+           P.periodic[0] && P.periodic[1] && P.periodic[2];
 }
 
 template <int RECON, int FVS>

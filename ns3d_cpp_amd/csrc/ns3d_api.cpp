@@ -172,14 +172,19 @@ int pick_seg(int nd, long long ncol)
 int check_supported(const ns3d_params &p, const ns3d_grid &g)
 {
     for (int f = 0; f < 6; ++f)
-        if (p.bc[f] != NS3D_BC_PERIODIC) return NS3D_ERR_UNSUPPORTED;
+        if (p.bc[f] < NS3D_BC_PERIODIC || p.bc[f] > NS3D_BC_INFLOW) return NS3D_ERR_CONFIG;
+    // a periodic side facing a non-periodic one leaves its ghosts unset in
+    // the reference (no neighbour, no physical BC); refused here
+    for (int a = 0; a < 3; ++a)
+        if ((p.bc[2 * a] == NS3D_BC_PERIODIC) != (p.bc[2 * a + 1] == NS3D_BC_PERIODIC)) return NS3D_ERR_CONFIG;
     if (p.recon == NS3D_RECON_WCNS) return NS3D_ERR_CONFIG;
     if (p.precision != NS3D_PREC_FP64 && p.precision != NS3D_PREC_MIXED) return NS3D_ERR_INVALID_ARG;
     // the mixed path is component-wise with full-row x blocks
     if (p.precision == NS3D_PREC_MIXED && p.char_recon) return NS3D_ERR_UNSUPPORTED;
     if (p.fvs_type != NS3D_FVS_STEGER_WARMING && p.fvs_type != NS3D_FVS_LAX_FRIEDRICHS)
         return NS3D_ERR_CONFIG;
-    // periodic images of the ghost layers must exist inside the interior
+    // periodic images and wall mirrors of the ghost layers must lie inside
+    // the interior
     if (g.nx < g.ng || g.ny < g.ng || g.nz < g.ng) return NS3D_ERR_GRID_TOO_SMALL;
     return NS3D_OK;
 }
@@ -556,7 +561,9 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.fvs = p.fvs_type; d.recon = p.recon; d.vis = p.vis_scheme; d.mu_law = p.mu_law;
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ng = g.ng;
     d.sx = (int)sx; d.sy = (int)sy; d.sz = (int)sz;
-    d.periodic[0] = d.periodic[1] = d.periodic[2] = 1;
+    for (int a = 0; a < 3; ++a)
+        d.periodic[a] = p.bc[2 * a] == NS3D_BC_PERIODIC && p.bc[2 * a + 1] == NS3D_BC_PERIODIC;
+    for (int f = 0; f < 6; ++f) d.bc[f] = p.bc[f];
 
     e = hipMalloc((void **)&c->slab, (size_t)ncell * kCellArrays * sizeof(double));
     if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc fields"); ns3d_destroy(c); return r; }

@@ -16,7 +16,8 @@ struct DevParams {
     double dx, dy, dz, idx, idy, idz;
     int fvs, recon, vis, mu_law;
     int nx, ny, nz, ng, sx, sy, sz;
-    int periodic[3];
+    int periodic[3];         // axis has both sides periodic (initialization.cpp:315-323)
+    int bc[6];               // ns3d_bc_type of xmin, xmax, ymin, ymax, zmin, zmax
     double rho_ref, p_ref;   // reference state of the mixed-precision shadow
     int adj_lines;           // 1: adjoint sweeps may hold whole lines in LDS
 };
@@ -346,5 +347,86 @@ __device__ __forceinline__ double diff_window(const double *v, double h)
         return (-v[2] + 8.0 * v[1] - 8.0 * v[-1] + v[-2]) / (12.0 * h);
     }
 }
+
+// ---------------------------------------- closures on non-periodic axes
+// choose_scheme_C4/C6/C8 (ViscousFlux.cpp:43-118) on an axis without a
+// neighbour, for a cell dlo cells above the first and dhi cells below the
+// last interior cell: min(dlo, dhi) = 3, 2, 1 take the 6th, 4th and 2nd
+// order central formulas (ViscousFlux.cpp:19-29), the end cells the one-sided
+// second-order formula of ViscousFlux.cpp:15-17 with flag +1 (low end) and
+// -1 (high end).  With flag -1 that formula, (-3 f0 + 4 f[-1] - f[-2])/(2h),
+// equals minus the derivative; it is the reference's expression and is kept.
+// Only interior values are read.  Periodic axes pass dlo = dhi = R.
+__device__ __forceinline__ double diff_closure(const double *v, double h, int dlo, int dhi)
+{
+    if (dlo == 0) return (-3.0 * v[0] + 4.0 * v[1] - v[2]) / (2.0 * h);
+    if (dhi == 0) return (-3.0 * v[0] + 4.0 * v[-1] - v[-2]) / (2.0 * h);
+    const int d = dlo < dhi ? dlo : dhi;
+    if (d == 1) return (v[1] - v[-1]) / (2.0 * h);
+    if (d == 2) return diff_window<VIS_C4>(v, h);
+    return diff_window<VIS_C6>(v, h);
+}
+
+template <int VIS>
+__device__ __forceinline__ double diff_window_bc(const double *v, double h, int dlo, int dhi)
+{
+    constexpr int R = VisReach<VIS>::R;
+    if (dlo >= R && dhi >= R) return diff_window<VIS>(v, h);
+    return diff_closure(v, h, dlo, dhi);
+}
+
+// diff_central with the closures; the window is gathered from memory only
+// near a non-periodic end (the ghost layers keep every read in bounds)
+template <int VIS>
+__device__ __forceinline__ double diff_central_bc(const double *__restrict__ c, long st, double h, int dlo, int dhi)
+{
+    constexpr int R = VisReach<VIS>::R;
+    if (dlo >= R && dhi >= R) return diff_central<VIS>(c, st, h);
+    double v[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; ++o) v[o + R] = c[o * st];
+    return diff_closure(v + R, h, dlo, dhi);
+}
+
+// distances of interior index q (padded numbering) to the ends of an axis of
+// n cells; R, R on periodic axes
+__device__ __forceinline__ void axis_dist(int periodic, int q, int ng, int n, int R, int &dlo, int &dhi)
+{
+    if (periodic) { dlo = R; dhi = R; }
+    else { dlo = q - ng; dhi = ng + n - 1 - q; }
+}
+
+// BC kinds of the ghost fill (ns3d_bc_type values)
+enum { BCK_PERIODIC = 0, BCK_WALL = 1, BCK_SYMMETRY = 2, BCK_OUTFLOW = 3, BCK_INFLOW = 4 };
+
+// One ghost cell of face `face` (0..5 = xmin..zmax) over the full cross
+// ranges, t in [0, ng * cross1 * cross2): the cell id, the id it copies
+// (mirror for wall/symmetry, the adjacent interior cell for outflow) and the
+// normal-velocity component (1..3).  boundary.cpp:58-182.
+__device__ __forceinline__ void bc_cell(const DevParams &P, int face, int kind, int t, int &dst, int &src, int &normal)
+{
+    const int axis = face >> 1, hi = face & 1, ng = P.ng;
+    const int na = axis == 0 ? P.sy : P.sx;     // fastest cross axis
+    const int g = t % ng;
+    const int rest = t / ng;
+    const int a = rest % na;
+    const int b = rest / na;
+    const int s = axis == 0 ? P.sx : axis == 1 ? P.sy : P.sz;
+    int q, qs;
+    if (!hi) {
+        q = g;
+        qs = kind == BCK_OUTFLOW ? ng : 2 * ng - 1 - g;
+    } else {
+        q = s - ng + g;
+        qs = kind == BCK_OUTFLOW ? s - ng - 1 : 2 * (s - ng) - 1 - q;
+    }
+    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
+    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
+    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+    normal = 1 + axis;
+}
+
+// apply_inflow_bc's hard-wired state (boundary.cpp:188): rho, u, v, w, p
+__device__ __forceinline__ void inflow_state(double s[5]) { s[0] = 1.0; s[1] = 1.0; s[2] = 0.0; s[3] = 0.0; s[4] = 1.0; }
 
 }  // namespace ns3d

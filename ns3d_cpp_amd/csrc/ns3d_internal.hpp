@@ -66,8 +66,6 @@ hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, cons
 hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &ff);
 hipError_t launch_grad(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_vflux(hipStream_t s, const DevParams &P, const Fields &F);
-hipError_t launch_vdiv(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt,
-                       unsigned long long *flag, unsigned long long key);
 hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsigned long long *out);
 hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n);
 int ke_blocks(const DevParams &P);

@@ -100,6 +100,36 @@ __global__ void k_fill_periodic(DevParams P, Fields F)
     for (int m = 0; m < 6; ++m) F.W[m][dst] = F.W[m][src];
 }
 
+// apply_wall_bc / apply_symmetry_bc / apply_outflow_bc / apply_inflow_bc
+// (boundary.cpp:58-207) on W for one face over its full cross ranges, then T
+// of the written cells as primitiveToConserved forms it, with its density
+// clamp (field_structures.h:436-448).  Launched face by face in the order of
+// apply_boundary (boundary.cpp:24-53), after the periodic fill.
+__global__ void k_fill_bc(DevParams P, Fields F, int face, int kind)
+{
+    const int axis = face >> 1;
+    const int na = axis == 0 ? P.sy : P.sx;
+    const int nb = axis == 2 ? P.sy : P.sz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.ng * na * nb) return;
+    int dst, src, normal;
+    bc_cell(P, face, kind, t, dst, src, normal);
+    double s[5];
+    if (kind == BCK_INFLOW) {
+        inflow_state(s);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) s[m] = F.W[m][src];
+        if (kind != BCK_OUTFLOW) s[normal] = -s[normal];
+    }
+    double rr = s[0];
+    if (rr <= 0.0) rr = 1e-12;
+    F.W[0][dst] = rr;
+#pragma unroll
+    for (int m = 1; m < 5; ++m) F.W[m][dst] = s[m];
+    F.W[5][dst] = s[4] / (rr * P.Rgas);
+}
+
 // --------------------------------------------------------- fused sweeps
 //
 // One thread marches one column of cells along the sweep axis DIR over a
@@ -251,7 +281,7 @@ __global__ __launch_bounds__(1024) void k_sweep_xrow(DevParams P, Fields F)
 // Writes only the viscous-flux arrays (9 distinct: the stress is symmetric and
 // the context aliases VF[4], VF[8], VF[9] to VF[1], VF[2], VF[6]).  Same
 // arithmetic and order as k_grad + k_vflux.
-template <int VIS, int MINB>
+template <int VIS, int MINB, int BC>
 __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, int seg_len)
 {
     constexpr int R = VisReach<VIS>::R;
@@ -271,8 +301,14 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
 #pragma unroll
         for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
     const double Pr = P.Pr, Cp = P.Cp;
+    int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
+    if constexpr (BC) {
+        axis_dist(P.periodic[0], i, P.ng, P.nx, R, dxl, dxh);
+        axis_dist(P.periodic[1], j, P.ng, P.ny, R, dyl, dyh);
+    }
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
+        if constexpr (BC) axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
 #pragma unroll
@@ -283,9 +319,9 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double *c = src[q] + id;
-            g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
-            g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
-            g[3 * q + 2] = diff_window<VIS>(&win[q][R], P.dz);
+            g[3 * q + 0] = diff_central_bc<VIS>(c, 1, P.dx, dxl, dxh);
+            g[3 * q + 1] = diff_central_bc<VIS>(c, P.sx, P.dy, dyl, dyh);
+            g[3 * q + 2] = diff_window_bc<VIS>(&win[q][R], P.dz, dzl, dzh);
         }
         double mu = get_mu(P, win[3][R]);
         double du_dx = g[0], du_dy = g[1], du_dz = g[2];
@@ -455,23 +491,29 @@ __global__ void k_face_diff(DevParams P, Fields F, FaceArrays ff)
 
 // ------------------------------------------------------------- viscous
 
-// compute_gradients (ViscousFlux.cpp:240-279), periodic axes: full order.
+// compute_gradients (ViscousFlux.cpp:240-279): full order on periodic axes,
+// the closures of choose_scheme near the ends of the others.
 template <int VIS>
 __global__ __launch_bounds__(256) void k_grad(DevParams P, Fields F)
 {
+    constexpr int R = VisReach<VIS>::R;
     const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
     const int k = P.ng + blockIdx.z;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int id = (k * P.sy + j) * P.sx + i;
     const int sxy = P.sx * P.sy;
+    int dxl, dxh, dyl, dyh, dzl, dzh;
+    axis_dist(P.periodic[0], i, P.ng, P.nx, R, dxl, dxh);
+    axis_dist(P.periodic[1], j, P.ng, P.ny, R, dyl, dyh);
+    axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
     const double *src[4] = {F.W[1], F.W[2], F.W[3], F.W[5]};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const double *c = src[q] + id;
-        F.G[3 * q + 0][id] = diff_central<VIS>(c, 1, P.dx);
-        F.G[3 * q + 1][id] = diff_central<VIS>(c, P.sx, P.dy);
-        F.G[3 * q + 2][id] = diff_central<VIS>(c, sxy, P.dz);
+        F.G[3 * q + 0][id] = diff_central_bc<VIS>(c, 1, P.dx, dxl, dxh);
+        F.G[3 * q + 1][id] = diff_central_bc<VIS>(c, P.sx, P.dy, dyl, dyh);
+        F.G[3 * q + 2][id] = diff_central_bc<VIS>(c, sxy, P.dz, dzl, dzh);
     }
 }
 
@@ -517,97 +559,14 @@ __global__ __launch_bounds__(256) void k_vflux(DevParams P, Fields F)
 // compute_vis_flux (ViscousFlux.cpp:341-386) + the RHS combine of
 // range_kutta.cpp:61-97, and for MODE 1..3 the RK3 stage update of
 // range_kutta.cpp:109-147 with the C2P of field_structures.h:405-427.
-// Viscous-flux ghosts are read as periodic images (index wrap), which is
-// what exchange_halos_viscous_flux stores there on one periodic rank.
+// One thread per (i, j) column marching along z, the z derivative of Gv
+// taken from a register window of the R planes above and below.
+// Viscous-flux ghosts on periodic axes are read as periodic images (index
+// wrap), which is what exchange_halos_viscous_flux stores there on one
+// periodic rank; on the other axes the closures read interior values only.
 // MODE 0: A <- rhs.  MODE s = 1,2,3: W <- C2P(stage s update), U0 written at
 // stage 1; stage 3 flags non-physical cells (range_kutta.cpp:150-161).
-template <int VIS, int MODE>
-__global__ __launch_bounds__(256) void k_vdiv(DevParams P, Fields F, double dt,
-                                              unsigned long long *flag, unsigned long long step_key)
-{
-    constexpr int R = VisReach<VIS>::R;
-    const int ng = P.ng;
-    const int i = ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = ng + blockIdx.z;
-    if (i >= ng + P.nx || j >= ng + P.ny) return;
-    const int id = (k * P.sy + j) * P.sx + i;
-    int ox[2 * R + 1], oy[2 * R + 1], oz[2 * R + 1];
-#pragma unroll
-    for (int o = -R; o <= R; ++o) {
-        ox[o + R] = (k * P.sy + j) * P.sx + wrapi(i + o, ng, P.nx);
-        oy[o + R] = (k * P.sy + wrapi(j + o, ng, P.ny)) * P.sx + i;
-        oz[o + R] = (wrapi(k + o, ng, P.nz) * P.sy + j) * P.sx + i;
-    }
-    double r[5];
-    r[0] = 0.0;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        double win[2 * R + 1];
-        double acc = 0.0;
-#pragma unroll
-        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[m][ox[o]];
-        acc += diff_window<VIS>(win + R, P.dx);
-#pragma unroll
-        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[4 + m][oy[o]];
-        acc += diff_window<VIS>(win + R, P.dy);
-#pragma unroll
-        for (int o = 0; o < 2 * R + 1; ++o) win[o] = F.VF[8 + m][oz[o]];
-        acc += diff_window<VIS>(win + R, P.dz);
-        r[m + 1] = acc;
-    }
-#pragma unroll
-    for (int m = 0; m < 5; ++m) r[m] += -(F.A[m][id]);
-    if constexpr (MODE == 0) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) F.A[m][id] = r[m];
-    } else {
-        const double gamma = P.gamma;
-        double rr = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
-        double U[5];
-        U[0] = rr;
-        U[1] = rr * u;
-        U[2] = rr * v;
-        U[3] = rr * w;
-        {
-            double kinetic = 0.5 * (u * u + v * v + w * w);
-            double e_internal = p / ((gamma - 1.0) * rr);
-            U[4] = rr * (e_internal + kinetic);
-        }
-        double Un[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            if constexpr (MODE == 1) {
-                F.U0[m][id] = U[m];
-                Un[m] = U[m] + dt * r[m];
-            } else if constexpr (MODE == 2) {
-                Un[m] = 0.75 * F.U0[m][id] + 0.25 * (U[m] + dt * r[m]);
-            } else {
-                Un[m] = (1.0 / 3.0) * F.U0[m][id] + (2.0 / 3.0) * (U[m] + dt * r[m]);
-            }
-        }
-        double rn = Un[0];
-        if (rn <= 0.0) rn = 1e-12;
-        double un = Un[1] / rn;
-        double vn = Un[2] / rn;
-        double wn = Un[3] / rn;
-        double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
-        double e_internal = Un[4] / rn - kinetic;
-        double pn = (gamma - 1.0) * rn * e_internal;
-        double Tn = pn / (rn * P.Rgas);
-        F.W[0][id] = rn; F.W[1][id] = un; F.W[2][id] = vn; F.W[3][id] = wn; F.W[4][id] = pn;
-        F.W[5][id] = Tn;
-        if constexpr (MODE == 3) {
-            if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                atomicMin(flag, step_key | (unsigned long long)id);
-        }
-    }
-}
-
-// k_vdiv with the z derivative of Gv taken from a register window of the
-// R planes above and below (one thread per (i, j) column marching along z;
-// periodic wrap in z as in k_vdiv).  Same arithmetic and order as k_vdiv.
-template <int VIS, int MODE>
+template <int VIS, int MODE, int BC>
 __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt, int seg_len,
                                                 unsigned long long *flag, unsigned long long step_key)
 {
@@ -634,9 +593,15 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
 #pragma unroll
         for (int o = 1; o < 2 * R + 1; ++o)
             win[m][o] = F.VF[8 + m][wrapi(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
+    if constexpr (BC) {
+        axis_dist(P.periodic[0], i, ng, P.nx, R, dxl, dxh);
+        axis_dist(P.periodic[1], j, ng, P.ny, R, dyl, dyh);
+    }
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         const int kn = wrapi(k + R, ng, P.nz) * sxy + col;
+        if constexpr (BC) axis_dist(P.periodic[2], k, ng, P.nz, R, dzl, dzh);
         double r[5];
         r[0] = 0.0;
 #pragma unroll
@@ -651,9 +616,9 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
                 wy[o] = F.VF[4 + m][k * sxy + oyc[o]];
             }
             double acc = 0.0;
-            acc += diff_window<VIS>(wx + R, P.dx);
-            acc += diff_window<VIS>(wy + R, P.dy);
-            acc += diff_window<VIS>(&win[m][R], P.dz);
+            acc += diff_window_bc<VIS>(wx + R, P.dx, dxl, dxh);
+            acc += diff_window_bc<VIS>(wy + R, P.dy, dyl, dyh);
+            acc += diff_window_bc<VIS>(&win[m][R], P.dz, dzl, dzh);
             r[m + 1] = acc;
         }
 #pragma unroll
@@ -831,6 +796,12 @@ hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F
     if (P.nx * P.ny > m) m = P.nx * P.ny;
     int total = 2 * P.ng * m;
     hipLaunchKernelGGL(k_fill_periodic, dim3((total + 255) / 256, 3), dim3(256), 0, s, P, F);
+    for (int face = 0; face < 6; ++face) {
+        const int axis = face >> 1;
+        if (P.periodic[axis] || P.bc[face] == BCK_PERIODIC) continue;
+        const int n = P.ng * (axis == 0 ? P.sy : P.sx) * (axis == 2 ? P.sy : P.sz);
+        hipLaunchKernelGGL(k_fill_bc, dim3((n + 255) / 256), dim3(256), 0, s, P, F, face, P.bc[face]);
+    }
     return hipGetLastError();
 }
 
@@ -913,50 +884,59 @@ hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F)
     return hipGetLastError();
 }
 
-hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb)
+template <int MINB, int BC>
+static hipError_t gradflux_vis(hipStream_t s, const DevParams &P, const Fields &F, int seg_len)
 {
     dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg_len - 1) / seg_len);
-    if (minb == 2) {
-        switch (P.vis) {
-        case VIS_C8: hipLaunchKernelGGL((k_gradflux<VIS_C8, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        case VIS_C6: hipLaunchKernelGGL((k_gradflux<VIS_C6, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        case VIS_C4: hipLaunchKernelGGL((k_gradflux<VIS_C4, 2>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        default: return hipErrorInvalidValue;
-        }
-    } else {
-        switch (P.vis) {
-        case VIS_C8: hipLaunchKernelGGL((k_gradflux<VIS_C8, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        case VIS_C6: hipLaunchKernelGGL((k_gradflux<VIS_C6, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        case VIS_C4: hipLaunchKernelGGL((k_gradflux<VIS_C4, 1>), g, kCellBlock, 0, s, P, F, seg_len); break;
-        default: return hipErrorInvalidValue;
-        }
+    switch (P.vis) {
+    case VIS_C8: hipLaunchKernelGGL((k_gradflux<VIS_C8, MINB, BC>), g, kCellBlock, 0, s, P, F, seg_len); break;
+    case VIS_C6: hipLaunchKernelGGL((k_gradflux<VIS_C6, MINB, BC>), g, kCellBlock, 0, s, P, F, seg_len); break;
+    case VIS_C4: hipLaunchKernelGGL((k_gradflux<VIS_C4, MINB, BC>), g, kCellBlock, 0, s, P, F, seg_len); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int VIS>
+// the closure-aware variant only where an axis is not periodic
+hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb)
+{
+    const bool bc = !(P.periodic[0] && P.periodic[1] && P.periodic[2]);
+    if (minb == 2) return bc ? gradflux_vis<2, 1>(s, P, F, seg_len) : gradflux_vis<2, 0>(s, P, F, seg_len);
+    return bc ? gradflux_vis<1, 1>(s, P, F, seg_len) : gradflux_vis<1, 0>(s, P, F, seg_len);
+}
+
+template <int VIS, int BC>
 static void vdivz_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
                        unsigned long long *flag, unsigned long long key)
 {
     dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_vdiv_z<VIS, 0>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    case 1: hipLaunchKernelGGL((k_vdiv_z<VIS, 1>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    case 2: hipLaunchKernelGGL((k_vdiv_z<VIS, 2>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    default: hipLaunchKernelGGL((k_vdiv_z<VIS, 3>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 0: hipLaunchKernelGGL((k_vdiv_z<VIS, 0, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 1: hipLaunchKernelGGL((k_vdiv_z<VIS, 1, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 2: hipLaunchKernelGGL((k_vdiv_z<VIS, 2, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    default: hipLaunchKernelGGL((k_vdiv_z<VIS, 3, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
     }
 }
 
-hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key)
+template <int BC>
+static hipError_t vdivz_vis(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                            unsigned long long *flag, unsigned long long key)
 {
     switch (P.vis) {
-    case VIS_C8: vdivz_mode<VIS_C8>(s, P, F, mode, dt, seg, flag, key); break;
-    case VIS_C6: vdivz_mode<VIS_C6>(s, P, F, mode, dt, seg, flag, key); break;
-    case VIS_C4: vdivz_mode<VIS_C4>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C8: vdivz_mode<VIS_C8, BC>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C6: vdivz_mode<VIS_C6, BC>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C4: vdivz_mode<VIS_C4, BC>(s, P, F, mode, dt, seg, flag, key); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// the closure-aware variant only where an axis is not periodic
+hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                         unsigned long long *flag, unsigned long long key)
+{
+    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, key);
+    return vdivz_vis<1>(s, P, F, mode, dt, seg, flag, key);
 }
 
 using FaceFn = void (*)(DevParams, Fields, FaceArrays);
@@ -1017,30 +997,6 @@ hipError_t launch_grad(hipStream_t s, const DevParams &P, const Fields &F)
 hipError_t launch_vflux(hipStream_t s, const DevParams &P, const Fields &F)
 {
     hipLaunchKernelGGL(k_vflux, cell_grid(P), kCellBlock, 0, s, P, F);
-    return hipGetLastError();
-}
-
-template <int VIS>
-static void vdiv_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt,
-                      unsigned long long *flag, unsigned long long key)
-{
-    switch (mode) {
-    case 0: hipLaunchKernelGGL((k_vdiv<VIS, 0>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
-    case 1: hipLaunchKernelGGL((k_vdiv<VIS, 1>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
-    case 2: hipLaunchKernelGGL((k_vdiv<VIS, 2>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
-    default: hipLaunchKernelGGL((k_vdiv<VIS, 3>), cell_grid(P), kCellBlock, 0, s, P, F, dt, flag, key); break;
-    }
-}
-
-hipError_t launch_vdiv(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt,
-                       unsigned long long *flag, unsigned long long key)
-{
-    switch (P.vis) {
-    case VIS_C8: vdiv_mode<VIS_C8>(s, P, F, mode, dt, flag, key); break;
-    case VIS_C6: vdiv_mode<VIS_C6>(s, P, F, mode, dt, flag, key); break;
-    case VIS_C4: vdiv_mode<VIS_C4>(s, P, F, mode, dt, flag, key); break;
-    default: return hipErrorInvalidValue;
-    }
     return hipGetLastError();
 }
 

@@ -123,16 +123,19 @@ __device__ __forceinline__ void shadow_prims(const DevParams &P, const MFields &
 
 // ---------------------------------------------------------------- sweeps
 
-// x sweep with FP64 split values in LDS, one block per full periodic x-row
-// (as k_sweep_xrow); used for MDCD_HYBRID, which reconstructs in FP64.
-template <int RECON, int FVS>
+// x sweep with FP64 split values in LDS, one block per full x-row (as
+// k_sweep_xrow); used for MDCD_HYBRID, which reconstructs in FP64.  PER = 0
+// (x not periodic) computes the left face of the first cell as well.
+template <int RECON, int FVS, int PER>
 __global__ __launch_bounds__(1024) void km_sweep_xrow64(DevParams P, MFields M)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
+    constexpr int SH = PER ? 0 : 1;
     extern __shared__ double lds[];
     const int TX = P.nx;
-    const int NSLOT = TX + S - 1;
+    const int NSLOT = TX + S - 1 + SH;
+    const int NF = TX + SH;                 // faces, the first right of cell ng - SH
     double *fpl = lds;
     double *fml = lds + 5 * NSLOT;
     double *ffl = lds + 10 * NSLOT;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow64(DevParams P, MFields M)
     const int j = P.ng + blockIdx.x;
     const int k = P.ng + blockIdx.y;
     const int row = (k * P.sy + j) * P.sx;
-    const int i0 = P.ng - MID;
+    const int i0 = P.ng - MID - SH;
     for (int q = t; q < NSLOT; q += blockDim.x) {
         double rho, u, v, w, p, Fp[5], Fm[5];
         shadow_prims(P, M, row + i0 + q, rho, u, v, w, p);
@@ -149,21 +152,22 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow64(DevParams P, MFields M)
         for (int m = 0; m < 5; ++m) { fpl[m * NSLOT + q] = Fp[m]; fml[m * NSLOT + q] = Fm[m]; }
     }
     __syncthreads();
-    if (t < TX) {
+    for (int f = t; f < NF; f += blockDim.x) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double sp[S], sm[S];
 #pragma unroll
-            for (int s2 = 0; s2 < S; ++s2) { sp[s2] = fpl[m * NSLOT + t + s2]; sm[s2] = fml[m * NSLOT + t + s2]; }
-            ffl[m * TX + t] = face_mixed<RECON, S>(P, sp, sm);
+            for (int s2 = 0; s2 < S; ++s2) { sp[s2] = fpl[m * NSLOT + f + s2]; sm[s2] = fml[m * NSLOT + f + s2]; }
+            ffl[m * NF + f] = face_mixed<RECON, S>(P, sp, sm);
         }
     }
     __syncthreads();
     if (t < TX) {
-        const int tl = t == 0 ? TX - 1 : t - 1;
+        const int fr = t + SH;
+        const int fl = PER ? (t == 0 ? TX - 1 : t - 1) : t;
         const int id = row + P.ng + t;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) M.A[m][id] = (float)((ffl[m * TX + t] - ffl[m * TX + tl]) * P.idx);
+        for (int m = 0; m < 5; ++m) M.A[m][id] = (float)((ffl[m * NF + fr] - ffl[m * NF + fl]) * P.idx);
     }
 }
 
@@ -393,7 +397,7 @@ __device__ __forceinline__ void split_fast(const DevParams &P, double rho, doubl
 // slots, its first lane only supplying the left neighbour of the second.  LDS
 // holds the FP32 differences of the row and its ghost cells, and rel per
 // face.  Launched with 64 * ceil((nx + S) / 63) threads.
-template <int RECON, int FVS>
+template <int RECON, int FVS, int PER>
 __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
 {
     constexpr int S = Stencil<RECON>::S;
@@ -402,7 +406,9 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
     const int TX = P.nx;
     const int NSLOT = TX + S;                 // cells ng - MID - 1 .. ng + TX + S - MID - 2
     float *dl = ldsf;                         // [10][NSLOT] differences (slot 0 unused)
-    float *rl = dl + 10 * NSLOT;              // [5][TX] rel per face
+    float *rl = dl + 10 * NSLOT;              // [5][NF] rel per face
+    constexpr int SH = PER ? 0 : 1;
+    const int NF = TX + SH;                   // faces, the first right of cell ng - SH
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int j = P.ng + blockIdx.x;
@@ -424,28 +430,31 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
     __syncthreads();
     float e[S];
     diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);   // unused for WENO5
-    if (t < TX) {
-        // face right of cell ng + t: stencil cells ng + t - MID + s = slots t + 1 + s
+    if (t < NF) {
+        // face right of cell ng + c, c = t - SH: stencil cells ng + c - MID + s
+        // = slots c + 1 + s (c = -1 reads slots 1 .. S - 1)
+        const int c = t - SH;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             float D[S], Dm[S];
 #pragma unroll
             for (int tt = 1; tt < S; ++tt) {
-                D[tt] = dl[m * NSLOT + t + 1 + tt];
-                Dm[tt] = -dl[(5 + m) * NSLOT + t + 1 + S - tt];
+                D[tt] = dl[m * NSLOT + c + 1 + tt];
+                Dm[tt] = -dl[(5 + m) * NSLOT + c + 1 + S - tt];
             }
-            rl[m * TX + t] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
+            rl[m * NF + t] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
         }
     }
     __syncthreads();
     if (t < TX) {
-        const int tl = t == 0 ? TX - 1 : t - 1;
+        const int fr = t + SH;
+        const int fl = PER ? (t == 0 ? TX - 1 : t - 1) : t;
         const int id = row + P.ng + t;
         const float ix = (float)P.idx;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             const float dF = dl[m * NSLOT + t + MID + 1] + dl[(5 + m) * NSLOT + t + MID + 2] +
-                             (rl[m * TX + t] - rl[m * TX + tl]);
+                             (rl[m * NF + fr] - rl[m * NF + fl]);
             M.A[m][id] = dF * ix;
         }
     }
@@ -575,11 +584,31 @@ __device__ __forceinline__ float diff32(const float *v, float h_inv)
     }
 }
 
+// the closures of choose_scheme on non-periodic axes (as diff_closure) in
+// FP32; periodic axes pass dlo = dhi = R
+__device__ __forceinline__ float diff32_closure(const float *v, float h_inv, int dlo, int dhi)
+{
+    if (dlo == 0) return (-3.0f * v[0] + 4.0f * v[1] - v[2]) * (0.5f * h_inv);
+    if (dhi == 0) return (-3.0f * v[0] + 4.0f * v[-1] - v[-2]) * (0.5f * h_inv);
+    const int d = dlo < dhi ? dlo : dhi;
+    if (d == 1) return (v[1] - v[-1]) * (0.5f * h_inv);
+    if (d == 2) return diff32<VIS_C4>(v, h_inv);
+    return diff32<VIS_C6>(v, h_inv);
+}
+
+template <int VIS>
+__device__ __forceinline__ float diff32_bc(const float *v, float h_inv, int dlo, int dhi)
+{
+    constexpr int R = VisReach<VIS>::R;
+    if (dlo >= R && dhi >= R) return diff32<VIS>(v, h_inv);
+    return diff32_closure(v, h_inv, dlo, dhi);
+}
+
 // gradients + viscous flux in FP32 from the shadow (u, v, w, T-1),
 // z-marching register window as k_gradflux.  The stress tensor is symmetric:
 // VF[4], VF[8], VF[9] (txy, txz, tyz of the y and z fluxes) alias VF[1],
 // VF[2], VF[6], so 9 arrays are written.
-template <int VIS>
+template <int VIS, int BC>
 __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
 {
     constexpr int R = VisReach<VIS>::R;
@@ -600,8 +629,14 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
 #pragma unroll
         for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
     const float cq = (float)(P.Cp / P.Pr);
+    int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
+    if constexpr (BC) {
+        axis_dist(P.periodic[0], i, P.ng, P.nx, R, dxl, dxh);
+        axis_dist(P.periodic[1], j, P.ng, P.ny, R, dyl, dyh);
+    }
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
+        if constexpr (BC) axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
         float g[12];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -614,9 +649,9 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
                 wx[o + R] = src[q][id + o];
                 wy[o + R] = src[q][id + o * P.sx];
             }
-            g[3 * q + 0] = diff32<VIS>(wx + R, idx);
-            g[3 * q + 1] = diff32<VIS>(wy + R, idy);
-            g[3 * q + 2] = diff32<VIS>(&win[q][R], idz);
+            g[3 * q + 0] = diff32_bc<VIS>(wx + R, idx, dxl, dxh);
+            g[3 * q + 1] = diff32_bc<VIS>(wy + R, idy, dyl, dyh);
+            g[3 * q + 2] = diff32_bc<VIS>(&win[q][R], idz, dzl, dzh);
         }
         const float mu = get_mu32(P, 1.0f + win[3][R]);
         const float divU = g[0] + g[4] + g[8];
@@ -641,10 +676,10 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
     }
 }
 
-// divergence of the FP32 viscous fluxes (periodic wrap), the RHS in FP64,
-// and the stage update + C2P in FP64 (MODE 1..3) or the RHS into `rhs_out`
-// (MODE 0).
-template <int VIS, int MODE>
+// divergence of the FP32 viscous fluxes (periodic wrap on periodic axes,
+// the closures near the ends of the others), the RHS in FP64, and the stage
+// update + C2P in FP64 (MODE 1..3) or the RHS into `rhs_out` (MODE 0).
+template <int VIS, int MODE, int BC>
 __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
                                                  unsigned long long *flag, unsigned long long step_key)
 {
@@ -672,9 +707,15 @@ __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double 
 #pragma unroll
         for (int o = 1; o < 2 * R + 1; ++o)
             win[m][o] = M.VF[8 + m][wrapi_m(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
+    if constexpr (BC) {
+        axis_dist(P.periodic[0], i, ng, P.nx, R, dxl, dxh);
+        axis_dist(P.periodic[1], j, ng, P.ny, R, dyl, dyh);
+    }
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         const int kn = wrapi_m(k + R, ng, P.nz) * sxy + col;
+        if constexpr (BC) axis_dist(P.periodic[2], k, ng, P.nz, R, dzl, dzh);
         double r[5];
         r[0] = -(double)M.A[0][id];
 #pragma unroll
@@ -688,7 +729,8 @@ __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double 
                 wx[o] = M.VF[m][k * sxy + oxc[o]];
                 wy[o] = M.VF[4 + m][k * sxy + oyc[o]];
             }
-            const float vd = diff32<VIS>(wx + R, idx) + diff32<VIS>(wy + R, idy) + diff32<VIS>(&win[m][R], idz);
+            const float vd = diff32_bc<VIS>(wx + R, idx, dxl, dxh) + diff32_bc<VIS>(wy + R, idy, dyl, dyh) +
+                             diff32_bc<VIS>(&win[m][R], idz, dzl, dzh);
             r[m + 1] = (double)vd - (double)M.A[m + 1][id];
         }
         if constexpr (MODE == 0) {
@@ -802,6 +844,38 @@ __global__ void km_fill_periodic(DevParams P, MFields M)
     for (int m = 0; m < 6; ++m) M.S[m][dst] = M.S[m][src];
 }
 
+// the boundary.cpp:58-207 ghost fills of k_fill_bc on the FP32 shadow
+// (deviations: the wall/symmetry mirror negates the normal velocity, the
+// inflow state is converted as km_from_prims converts a cell)
+__global__ void km_fill_bc(DevParams P, MFields M, int face, int kind)
+{
+    const int axis = face >> 1;
+    const int na = axis == 0 ? P.sy : P.sx;
+    const int nb = axis == 2 ? P.sy : P.sz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.ng * na * nb) return;
+    int dst, src, normal;
+    bc_cell(P, face, kind, t, dst, src, normal);
+    float v[6];
+    if (kind == BCK_INFLOW) {
+        double st[5];
+        inflow_state(st);
+        const double T = st[4] / (st[0] * P.Rgas);
+        v[0] = (float)(st[0] - P.rho_ref);
+        v[1] = (float)st[1];
+        v[2] = (float)st[2];
+        v[3] = (float)st[3];
+        v[4] = (float)(st[4] - P.p_ref);
+        v[5] = (float)(T - 1.0);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v[m] = M.S[m][src];
+        if (kind != BCK_OUTFLOW) v[normal] = -v[normal];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) M.S[m][dst] = v[m];
+}
+
 // CFL minimum (timestep.cpp:5-33) from the shadow
 __global__ __launch_bounds__(256) void km_cfl(DevParams P, MFields M, unsigned long long *out)
 {
@@ -854,6 +928,12 @@ hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields
     if (P.nx * P.ny > m) m = P.nx * P.ny;
     const int total = 2 * P.ng * m;
     hipLaunchKernelGGL(km_fill_periodic, dim3((total + 255) / 256, 3), dim3(256), 0, s, P, M);
+    for (int face = 0; face < 6; ++face) {
+        const int axis = face >> 1;
+        if (P.periodic[axis] || P.bc[face] == BCK_PERIODIC) continue;
+        const int n = P.ng * (axis == 0 ? P.sy : P.sx) * (axis == 2 ? P.sy : P.sz);
+        hipLaunchKernelGGL(km_fill_bc, dim3((n + 255) / 256), dim3(256), 0, s, P, M, face, P.bc[face]);
+    }
     return hipGetLastError();
 }
 
@@ -869,16 +949,16 @@ hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, uns
 using MXrowFn = void (*)(DevParams, MFields);
 using MSweepFn = void (*)(DevParams, MFields, int);
 
-template <int FVS>
+template <int FVS, int PER>
 static MXrowFn m_pick_xrow(int recon)
 {
     switch (recon) {
-    case R_WENO5: return km_sweep_xrow<R_WENO5, FVS>;
-    case R_LINEAR: return km_sweep_xrow<R_LINEAR, FVS>;
-    case R_MDCD: return km_sweep_xrow<R_MDCD, FVS>;
-    case R_MDCD_HYBRID: return km_sweep_xrow64<R_MDCD_HYBRID, FVS>;
-    case R_UD7: return km_sweep_xrow<R_UD7, FVS>;
-    case R_UD7L: return km_sweep_xrow<R_UD7L, FVS>;
+    case R_WENO5: return km_sweep_xrow<R_WENO5, FVS, PER>;
+    case R_LINEAR: return km_sweep_xrow<R_LINEAR, FVS, PER>;
+    case R_MDCD: return km_sweep_xrow<R_MDCD, FVS, PER>;
+    case R_MDCD_HYBRID: return km_sweep_xrow64<R_MDCD_HYBRID, FVS, PER>;
+    case R_UD7: return km_sweep_xrow<R_UD7, FVS, PER>;
+    case R_UD7L: return km_sweep_xrow<R_UD7L, FVS, PER>;
     default: return nullptr;
     }
 }
@@ -897,28 +977,40 @@ static MSweepFn m_pick_sweep(int recon)
     }
 }
 
-// the mixed x sweep holds whole rows (1024 threads, 64 KiB of LDS at most)
-bool m_xrow_supported(const DevParams &P)
+// x-row launch shape of the mixed path: threads and LDS bytes
+static void m_xrow_shape(const DevParams &P, int &threads, size_t &lds)
 {
     const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
                   : P.recon == R_LINEAR ? 2 : 8;
-    if (P.recon == R_MDCD_HYBRID) return xrow_supported(P);
-    const size_t lds = (size_t)(10 * (P.nx + S) + 5 * P.nx) * sizeof(float);
-    return P.periodic[0] && 64 * ((P.nx + S + 62) / 63) <= 1024 && lds <= 65536;
+    const int sh = P.periodic[0] ? 0 : 1;
+    if (P.recon == R_MDCD_HYBRID) {
+        threads = ((P.nx + 63) / 64) * 64;
+        lds = (size_t)(10 * (P.nx + S - 1 + sh) + 5 * (P.nx + sh)) * sizeof(double);
+    } else {
+        threads = 64 * ((P.nx + S + 62) / 63);
+        lds = (size_t)(10 * (P.nx + S) + 5 * (P.nx + sh)) * sizeof(float);
+    }
+}
+
+// the mixed x sweep holds whole rows (1024 threads, 64 KiB of LDS at most)
+bool m_xrow_supported(const DevParams &P)
+{
+    int threads;
+    size_t lds;
+    m_xrow_shape(P, threads, lds);
+    return threads <= 1024 && lds <= 65536;
 }
 
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len)
 {
     if (dir == 0) {
-        MXrowFn fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW>(P.recon) : m_pick_xrow<FVS_LF>(P.recon);
-        if (!fn || !m_xrow_supported(P)) return hipErrorInvalidValue;
-        const int S = P.recon == R_WENO5 || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ? 6
-                      : P.recon == R_LINEAR ? 2 : 8;
-        const bool hyb = P.recon == R_MDCD_HYBRID;
-        const size_t lds = hyb ? (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double)
-                               : (size_t)(10 * (P.nx + S) + 5 * P.nx) * sizeof(float);
-        const int threads = hyb ? ((P.nx + 63) / 64) * 64 : 64 * ((P.nx + S + 62) / 63);
-        if (threads > 1024 || lds > 65536) return hipErrorInvalidValue;
+        MXrowFn fn;
+        if (P.periodic[0]) fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW, 1>(P.recon) : m_pick_xrow<FVS_LF, 1>(P.recon);
+        else fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW, 0>(P.recon) : m_pick_xrow<FVS_LF, 0>(P.recon);
+        int threads;
+        size_t lds;
+        m_xrow_shape(P, threads, lds);
+        if (!fn || threads > 1024 || lds > 65536) return hipErrorInvalidValue;
         hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, M);
         return hipGetLastError();
     }
@@ -932,44 +1024,59 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
     return hipGetLastError();
 }
 
+template <int BC>
+static void m_gradflux_vis(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, dim3 g, dim3 b)
+{
+    switch (P.vis) {
+    case VIS_C8: hipLaunchKernelGGL((km_gradflux<VIS_C8, BC>), g, b, 0, s, P, M, seg_len); break;
+    case VIS_C6: hipLaunchKernelGGL((km_gradflux<VIS_C6, BC>), g, b, 0, s, P, M, seg_len); break;
+    default: hipLaunchKernelGGL((km_gradflux<VIS_C4, BC>), g, b, 0, s, P, M, seg_len); break;
+    }
+}
+
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
+    if (P.vis != VIS_C8 && P.vis != VIS_C6 && P.vis != VIS_C4) return hipErrorInvalidValue;
     dim3 g((P.nx + bx - 1) / bx, (P.ny + by - 1) / by, (P.nz + seg_len - 1) / seg_len);
     dim3 b(bx, by, 1);
-    switch (P.vis) {
-    case VIS_C8: hipLaunchKernelGGL(km_gradflux<VIS_C8>, g, b, 0, s, P, M, seg_len); break;
-    case VIS_C6: hipLaunchKernelGGL(km_gradflux<VIS_C6>, g, b, 0, s, P, M, seg_len); break;
-    case VIS_C4: hipLaunchKernelGGL(km_gradflux<VIS_C4>, g, b, 0, s, P, M, seg_len); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) m_gradflux_vis<0>(s, P, M, seg_len, g, b);
+    else m_gradflux_vis<1>(s, P, M, seg_len, g, b);
     return hipGetLastError();
 }
 
-template <int VIS>
+template <int VIS, int BC>
 static void m_vdiv_mode(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                         unsigned long long *flag, unsigned long long key)
 {
     dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
     dim3 b(64, 4, 1);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    case 2: hipLaunchKernelGGL((km_vdiv_z<VIS, 2>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    default: hipLaunchKernelGGL((km_vdiv_z<VIS, 3>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_z<VIS, 2, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    default: hipLaunchKernelGGL((km_vdiv_z<VIS, 3, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
     }
+}
+
+template <int BC>
+static hipError_t m_vdiv_vis(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                             unsigned long long *flag, unsigned long long key)
+{
+    switch (P.vis) {
+    case VIS_C8: m_vdiv_mode<VIS_C8, BC>(s, P, M, mode, dt, seg, flag, key); break;
+    case VIS_C6: m_vdiv_mode<VIS_C6, BC>(s, P, M, mode, dt, seg, flag, key); break;
+    case VIS_C4: m_vdiv_mode<VIS_C4, BC>(s, P, M, mode, dt, seg, flag, key); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, unsigned long long key)
 {
-    switch (P.vis) {
-    case VIS_C8: m_vdiv_mode<VIS_C8>(s, P, M, mode, dt, seg, flag, key); break;
-    case VIS_C6: m_vdiv_mode<VIS_C6>(s, P, M, mode, dt, seg, flag, key); break;
-    case VIS_C4: m_vdiv_mode<VIS_C4>(s, P, M, mode, dt, seg, flag, key); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, key);
+    return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, key);
 }
 
 }  // namespace ns3d

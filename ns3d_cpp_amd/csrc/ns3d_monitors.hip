@@ -64,7 +64,9 @@ __global__ __launch_bounds__(256) void km_spec_bins(DevParams P, const double2 *
 //   0 sum 0.5|u|^2   1 sum |u|^2   2 sum u   3 sum v   4 sum w
 //   5 sum u^2   6 sum v^2   7 sum w^2   8 sum (du/dx)^2
 //   9 sum sqrt(gamma R T)   10 sum mu(T)   11 sum rho
-// du/dx with the viscous central scheme on periodic images (index wrap).
+// du/dx with the viscous scheme of compute_gradients_dudx
+// (ViscousFlux.cpp:389-411): periodic images (index wrap) on a periodic x
+// axis, the closures near the ends otherwise.
 template <int VIS>
 __global__ __launch_bounds__(256) void km_stats(DevParams P, const double *const *W, double *partial)
 {
@@ -85,7 +87,9 @@ __global__ __launch_bounds__(256) void km_stats(DevParams P, const double *const
         double win[2 * R + 1];
 #pragma unroll
         for (int o = -R; o <= R; ++o) win[o + R] = W[1][row + mwrap(i + o, P.ng, P.nx)];
-        const double dudx = diff_window<VIS>(win + R, P.dx);
+        int dlo, dhi;
+        axis_dist(P.periodic[0], i, P.ng, P.nx, R, dlo, dhi);
+        const double dudx = diff_window_bc<VIS>(win + R, P.dx, dlo, dhi);
         const double q2 = u * u + v * v + w * w;
         s[0] += 0.5 * q2;
         s[1] += q2;

@@ -32,10 +32,10 @@ def kind_of(name):
         return ["sweep_x", "sweep_y", "sweep_z"][int(m.group(1))]
     if "gradflux" in n:
         return "gradflux"
-    m = re.search(r"km?_vdiv(_z)?<\d+, (\d)>", n)
+    m = re.search(r"km?_vdiv(_z)?<\d+, (\d)(?:, \d)?>", n)
     if m:
         return "vdiv_rhs" if m.group(2) == "0" else "vdiv_stage"
-    if "fill_periodic" in n:
+    if "fill_periodic" in n or "fill_bc" in n:
         return "fill_ghosts"
     if "k_cfl" in n or "km_cfl" in n:
         return "cfl"

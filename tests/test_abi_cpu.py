@@ -106,8 +106,9 @@ def test_create_argument_errors():
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_INVALID_ARG
     q, h = ns.make_params(16, recon="WCNS")
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_CONFIG
-    q, h = ns.make_params(16, bc=["Wall"] * 6)
-    assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_UNSUPPORTED
+    # a periodic side facing a non-periodic one (validated before any device call)
+    q, h = ns.make_params(16, bc=["Periodic", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])
+    assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_CONFIG
     q, h = ns.make_params(2, recon="WENO5", vis_scheme="C8th")     # n < ng
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_GRID_TOO_SMALL
     for s in range(0, 9):
