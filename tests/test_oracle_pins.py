@@ -460,6 +460,65 @@ def test_periodic_fill_identity():
     assert np.all(o.arr(10)[:ng, :ng, :ng] == 0.0)
 
 
+def test_physical_bc_fills_closed_forms():
+    """apply_inflow_bc / apply_outflow_bc / apply_wall_bc / apply_symmetry_bc
+    (boundary.cpp:58-207) on the oracle: the hard-wired inflow state, copies
+    of the last interior cell, mirrors with the normal velocity negated."""
+    nx, ny, nz = 10, 8, 6
+    p, g = ns.make_params((nx, ny, nz), recon="WENO5", vis_scheme="C8th",
+                          bc=["Inflow", "Outflow", "Wall", "Wall", "Symmetry", "Outflow"])
+    o = Oracle(p, g)
+    rng = np.random.default_rng(8)
+    st = [rng.uniform(0.5, 1.5, (nz, ny, nx)) for _ in range(5)]
+    o.set_primitive(*st)
+    ng = o.ng
+    pr = o.primitive()
+    I = slice(ng, -ng)
+    for k, want in zip(("rho", "u", "v", "w", "p"), (1.0, 1.0, 0.0, 0.0, 1.0)):
+        assert np.all(pr[k][I, I, :ng] == want), k
+    # E of the inflow ghosts after primitiveToConserved: p/(gamma-1) + rho|u|^2/2
+    assert np.allclose(o.arr(4)[I, I, :ng], 1.0 / 0.4 + 0.5, rtol=1e-15)
+    for k in ("rho", "u", "v", "w", "p"):
+        assert np.array_equal(pr[k][I, I, -ng:], np.repeat(pr[k][I, I, -ng - 1:-ng], ng, axis=2)), k
+        assert np.array_equal(pr[k][-ng:, I, I], np.repeat(pr[k][-ng - 1:-ng, I, I], ng, axis=0)), k
+    for k, sign in (("rho", 1), ("u", 1), ("v", -1), ("w", 1), ("p", 1)):
+        assert np.array_equal(pr[k][I, ng - 1::-1, I], sign * pr[k][I, ng:2 * ng, I]), k
+        assert np.array_equal(pr[k][I, -ng:, I], sign * pr[k][I, -ng - 1:-2 * ng - 1:-1, I]), k
+    for k, sign in (("rho", 1), ("u", 1), ("v", 1), ("w", -1), ("p", 1)):
+        assert np.array_equal(pr[k][ng - 1::-1, I, I], sign * pr[k][ng:2 * ng, I, I]), k
+
+
+@pytest.mark.parametrize("vis,orders", [("C8th", [1, 2, 4, 6, 8]), ("C6th", [1, 2, 4, 6]), ("C4th", [1, 2, 4])])
+def test_gradient_closures_on_a_cubic(vis, orders):
+    """compute_gradients along a non-periodic x axis for u = a x^3 + b x^2 + c x:
+    the 4th/6th/8th-order central formulas are exact, the 2nd-order central
+    one is off by a h^2, the one-sided formula at the low end by -2 a h^2,
+    and at the high end it returns -u' + 2 a h^2 (the reference's flag = -1
+    expression, minus the derivative)."""
+    nx, ny, nz = 16, 6, 6
+    p, g = ns.make_params((nx, ny, nz), recon="WENO5", vis_scheme=vis,
+                          bc=["Outflow", "Outflow", "Periodic", "Periodic", "Periodic", "Periodic"])
+    o = Oracle(p, g)
+    h = g.dx
+    a, b, c = 0.3, -0.7, 0.4
+    x = np.arange(nx) * h
+    u1 = a * x ** 3 + b * x ** 2 + c * x
+    u = np.broadcast_to(u1, (nz, ny, nx)).copy()
+    st = [np.ones_like(u), u, np.zeros_like(u), np.zeros_like(u), np.full_like(u, 50.0)]
+    o.set_primitive(*st)
+    dudx = o.interior(o.compute_gradients()[0])[2, 3]
+    exact = 3 * a * x ** 2 + 2 * b * x + c
+    R = orders[-1] // 2
+    want = exact.copy()
+    want[0] = exact[0] - 2 * a * h * h
+    want[1] = exact[1] + a * h * h
+    want[-2] = exact[-2] + a * h * h
+    want[-1] = -exact[-1] + 2 * a * h * h
+    assert np.allclose(dudx, want, rtol=0, atol=1e-11), (dudx - want)
+    # the interior beyond R of the ends uses the full-order formula: exact
+    assert np.allclose(dudx[R:nx - R], exact[R:nx - R], rtol=0, atol=1e-11)
+
+
 def test_timestep_and_energy_closed_forms():
     n = 12
     rho, u, v, w, p = smooth_field(n, 6)
