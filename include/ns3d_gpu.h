@@ -135,7 +135,9 @@ int ns3d_download_rhs(ns3d_ctx *ctx, double *r0, double *r1, double *r2,
                       double *r3, double *r4);
 
 /* apply_boundary (boundary.cpp:14-54): periodic ghost fill of rho,u,v,w,p
- * (faces only, like exchange_halos_physical field_structures.h:795-853). */
+ * (faces only, like exchange_halos_physical field_structures.h:795-853), then
+ * the wall / symmetry / outflow / inflow fills of the non-periodic sides
+ * (boundary.cpp:58-207) in the order xmin, xmax, ..., zmax. */
 int ns3d_apply_boundary(ns3d_ctx *ctx);
 
 /* du_inviscous (InviscousFlux_new.cpp:4-163): face fluxes into device face
@@ -157,7 +159,8 @@ int ns3d_download_viscous_flux(ns3d_ctx *ctx, double *const flux[12]);
 /* compute_vis_flux (ViscousFlux.cpp:341-386) from the fluxes stored by
  * ns3d_compute_viscous_flux, on rhs = 0: the 5 RHS arrays (download with
  * ns3d_download_rhs) receive ((0 + Dx Fv) + Dy Hv) + Dz Gv, flux ghosts read
- * as periodic images (exchange_halos_viscous_flux, field_structures.h:736-791). */
+ * as periodic images on periodic axes (exchange_halos_viscous_flux,
+ * field_structures.h:736-791), the choose_scheme closures on the others. */
 int ns3d_compute_vis_flux(ns3d_ctx *ctx);
 /* compute_rhs (range_kutta.cpp:11-99) with the fused sweep kernels:
  * rhs = viscous divergence - inviscid flux difference, interior cells. */
