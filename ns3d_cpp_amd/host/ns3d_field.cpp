@@ -172,15 +172,52 @@ bool initialize_from_tecplot_downsample(Field3D &F, const GridDesc &G, const Sol
                                    F.v.data(), F.w.data(), F.p.data());
 }
 
+bool write_restart_file(Field3D &F, const GridDesc &G, const SolverParams &P, const std::string &path)
+{
+    const int kind = P.precision == NS3D_PREC_MIXED ? 1 : 0;
+    if (kind == 1) {
+        check(ns3d_download_conserved(F.ctx, F.rho.data(), F.rhou.data(), F.rhov.data(), F.rhow.data(), F.E.data()),
+              "download_conserved", &F);
+    } else {
+        check(ns3d_download_primitive(F.ctx, F.rho.data(), F.u.data(), F.v.data(), F.w.data(), F.p.data(),
+                                      F.T.data()),
+              "download_primitive", &F);
+    }
+    const double *a[5] = {F.rho.data(), kind ? F.rhou.data() : F.u.data(), kind ? F.rhov.data() : F.v.data(),
+                          kind ? F.rhow.data() : F.w.data(), kind ? F.E.data() : F.p.data()};
+    return write_restart(path, RestartInfo{F.nx, F.ny, F.nz, kind, F.step, F.time}, zone_of(F, G), a);
+}
+
+bool read_restart_file(Field3D &F, const GridDesc &G, const std::string &path)
+{
+    RestartInfo info{};
+    std::vector<double> b[5];
+    for (auto &x : b) x.assign(F.rho.size(), 0.0);
+    double *a[5] = {b[0].data(), b[1].data(), b[2].data(), b[3].data(), b[4].data()};
+    if (!read_restart(path, info, zone_of(F, G), a)) return false;
+    if (info.kind == 1)
+        check(ns3d_upload_conserved(F.ctx, a[0], a[1], a[2], a[3], a[4]), "upload_conserved", &F);
+    else
+        check(ns3d_upload_primitive(F.ctx, a[0], a[1], a[2], a[3], a[4]), "upload_primitive", &F);
+    F.step = info.step;
+    F.time = info.time;
+    return true;
+}
+
 void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
 {
     using clk = std::chrono::steady_clock;
     const auto t_start = clk::now();
     auto t_last = t_start;
-    double current_time = 0.0;
+    double current_time = F.time;
     double output_Timefreq = P.output_timefreq;
     const double TotalTime = P.total_time;
-    for (int step = 1; step <= P.max_steps; ++step) {
+    // a restarted run resumes the output schedule by the same repeated
+    // additions an unbroken run makes
+    while (F.time > 0.0 && output_Timefreq <= current_time + 1e-12 && P.output_timefreq > 0.0)
+        output_Timefreq += P.output_timefreq;
+    int step = (int)F.step + 1;
+    for (; step <= P.max_steps; ++step) {
         double dt = P.dt_fixed;
         const double eps = 1e-12;
         if (std::abs(TotalTime - current_time) <= eps) break;
@@ -224,6 +261,8 @@ void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
             if (P.post_basicfield) write_tecplot_field(F, G, P, current_time);
             if (P.isotropic_analyse) isotropic_post_process(F, G, P, current_time);
         }
+        F.step = step;
+        F.time = current_time;
         if (current_time >= TotalTime) break;
     }
     const double total = std::chrono::duration<double>(clk::now() - t_start).count();

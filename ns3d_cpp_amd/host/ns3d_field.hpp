@@ -36,6 +36,10 @@ struct Field3D {
     double global_Etot = 0.0;
     // compute_diagnostics results: rho, rho u, rho v, rho w, E
     double global_res[5] = {0, 0, 0, 0, 0};
+    // step index and time the state belongs to: time_advance starts after
+    // them and leaves its last step and time here (0, 0 for a fresh field)
+    long long step = 0;
+    double time = 0.0;
 
     Field3D() = default;
     Field3D(const Field3D &) = delete;
@@ -78,6 +82,11 @@ void write_tecplot_field(Field3D &F, const GridDesc &G, const SolverParams &P, d
 bool initialize_from_tecplot(Field3D &F, const GridDesc &G, const SolverParams &P, const std::string &filename);
 bool initialize_from_tecplot_downsample(Field3D &F, const GridDesc &G, const SolverParams &P,
                                         const std::string &filename, int src_nx, int src_ny, int src_nz);
+// Binary restart of the context's canonical state (primitives in FP64 mode,
+// conserved variables in mixed mode) with F.step and F.time; reading uploads
+// the state (boundary fill included) and sets F.step and F.time.
+bool write_restart_file(Field3D &F, const GridDesc &G, const SolverParams &P, const std::string &path);
+bool read_restart_file(Field3D &F, const GridDesc &G, const std::string &path);
 // time_advance (maincycle.cpp:7-113): dt fixed or CFL, dt clipping to the
 // output time and TotalTime, monitor lines on stdout, residual and
 // turbulence histories, field files at output times.

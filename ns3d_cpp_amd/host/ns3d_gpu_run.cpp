@@ -15,6 +15,8 @@
 //                                onto the grid (initialize_from_tecplot_downsample)
 // --no-field-files skips the field files at time 0 and at the end
 // (main.cpp:76, :83); the files at output times follow post_basicfield.
+// --restart-in <f> starts from a binary restart file (its step and time
+// included) instead of --ic; --restart-out <f> writes one after the last step.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -77,11 +79,13 @@ int main(int argc, char **argv)
         std::cerr << "usage: " << argv[0] << " <solver.in> [--ic tgv|uniform|raw:<file>]\n";
         return 2;
     }
-    std::string ic = "tgv";
+    std::string ic = "tgv", restart_in, restart_out;
     bool field_files = true;
     for (int a = 2; a < argc; ++a) {
         const std::string arg = argv[a];
         if (arg == "--ic" && a + 1 < argc) ic = argv[++a];
+        else if (arg == "--restart-in" && a + 1 < argc) restart_in = argv[++a];
+        else if (arg == "--restart-out" && a + 1 < argc) restart_out = argv[++a];
         else if (arg == "--no-field-files") field_files = false;
         else {
             std::cerr << "unknown argument " << arg << "\n";
@@ -101,7 +105,10 @@ int main(int argc, char **argv)
     try {
         Field3D F;
         F.allocate(P, G);
-        if (ic == "tgv") set_tgv(F, G, P);
+        if (!restart_in.empty()) {
+            if (!read_restart_file(F, G, restart_in)) return 1;
+            std::cout << "Restart from " << restart_in << " at step " << F.step << ", time " << F.time << "\n";
+        } else if (ic == "tgv") set_tgv(F, G, P);
         else if (ic == "uniform") set_uniform(F);
         else if (ic.rfind("raw:", 0) == 0) {
             if (!set_raw(F, ic.substr(4))) {
@@ -128,12 +135,13 @@ int main(int argc, char **argv)
             std::cerr << "unknown --ic " << ic << "\n";
             return 2;
         }
-        F.upload_primitive();
+        if (restart_in.empty()) F.upload_primitive();
         std::cout << "Initialization + halo exchange done\n";
-        if (field_files) write_tecplot_field(F, G, P, 0.0);
-        if (P.isotropic_analyse) isotropic_post_process(F, G, P, 0.0);
+        if (field_files) write_tecplot_field(F, G, P, F.time);
+        if (P.isotropic_analyse) isotropic_post_process(F, G, P, F.time);
         time_advance(F, G, P);
         if (field_files) write_tecplot_field(F, G, P, P.total_time);
+        if (!restart_out.empty() && !write_restart_file(F, G, P, restart_out)) return 1;
     } catch (const Error &e) {
         std::cerr << "Error: " << e.what() << "\n";
         return 1;

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cctype>
@@ -289,6 +290,64 @@ bool write_residual_line(const std::string &path, int step, const double res[5],
         std::fprintf(f.get(), "VARIABLES = \"Step\" \"Res_rho\" \"Res_rhou\" \"Res_rhov\" \"Res_rhow\" \"Res_E\" \"Etot\"\n");
     }
     std::fprintf(f.get(), "%d %.8e %.8e %.8e %.8e %.8e %.8e\n", step, res[0], res[1], res[2], res[3], res[4], Etot);
+    return true;
+}
+
+namespace {
+const char kRestartMagic[8] = {'N', 'S', '3', 'D', 'R', 'S', 'T', '1'};
+}
+
+bool write_restart(const std::string &path, const RestartInfo &info, const ZoneDesc &Z, const double *const a[5])
+{
+    ensure_parent(path);
+    File f(std::fopen(path.c_str(), "wb"));
+    if (!f) {
+        std::cerr << "Failed to open restart file " << path << " for writing\n";
+        return false;
+    }
+    const int32_t dims[4] = {Z.nx, Z.ny, Z.nz, info.kind};
+    const int64_t step = info.step;
+    bool ok = std::fwrite(kRestartMagic, 1, 8, f.get()) == 8 && std::fwrite(dims, sizeof(int32_t), 4, f.get()) == 4 &&
+              std::fwrite(&step, sizeof(step), 1, f.get()) == 1 &&
+              std::fwrite(&info.time, sizeof(double), 1, f.get()) == 1;
+    for (int q = 0; q < 5 && ok; ++q)
+        for (int k = 0; k < Z.nz && ok; ++k)
+            for (int j = 0; j < Z.ny && ok; ++j)
+                ok = std::fwrite(a[q] + cell(Z, 0, j, k), sizeof(double), (size_t)Z.nx, f.get()) == (size_t)Z.nx;
+    if (!ok) std::cerr << "Failed writing " << path << "\n";
+    return ok;
+}
+
+bool read_restart(const std::string &path, RestartInfo &info, const ZoneDesc &Z, double *const a[5])
+{
+    File f(std::fopen(path.c_str(), "rb"));
+    if (!f) {
+        std::cerr << "Error: cannot open restart file: " << path << "\n";
+        return false;
+    }
+    char magic[8];
+    int32_t dims[4];
+    int64_t step;
+    double time;
+    if (std::fread(magic, 1, 8, f.get()) != 8 || std::memcmp(magic, kRestartMagic, 8) != 0 ||
+        std::fread(dims, sizeof(int32_t), 4, f.get()) != 4 || std::fread(&step, sizeof(step), 1, f.get()) != 1 ||
+        std::fread(&time, sizeof(double), 1, f.get()) != 1) {
+        std::cerr << "Error: " << path << " is not a restart file\n";
+        return false;
+    }
+    if (dims[0] != Z.nx || dims[1] != Z.ny || dims[2] != Z.nz || (dims[3] != 0 && dims[3] != 1)) {
+        std::cerr << "Error: restart file " << path << " holds a " << dims[0] << "x" << dims[1] << "x" << dims[2]
+                  << " grid (kind " << dims[3] << "), the solver has " << Z.nx << "x" << Z.ny << "x" << Z.nz << "\n";
+        return false;
+    }
+    for (int q = 0; q < 5; ++q)
+        for (int k = 0; k < Z.nz; ++k)
+            for (int j = 0; j < Z.ny; ++j)
+                if (std::fread(a[q] + cell(Z, 0, j, k), sizeof(double), (size_t)Z.nx, f.get()) != (size_t)Z.nx) {
+                    std::cerr << "Error: restart file " << path << " is truncated\n";
+                    return false;
+                }
+    info = RestartInfo{dims[0], dims[1], dims[2], dims[3], (long long)step, time};
     return true;
 }
 

@@ -67,4 +67,20 @@ bool append_turbulence_stats(const std::string &path, double time, const TurbSta
 // step 1 truncates and writes the header, later steps append (output.cpp:93-128)
 bool write_residual_line(const std::string &path, int step, const double res[5], double Etot);
 
+// Binary restart file (no counterpart in the reference, whose only restart
+// path is the 8-digit Tecplot reader).  Little-endian: the 8-byte magic
+// "NS3DRST1", int32 nx, ny, nz, kind, int64 step, double time, then five
+// interior arrays (x fastest) of doubles: kind 0 = rho, u, v, w, p (the
+// canonical state of the FP64 mode), kind 1 = rho, rho u, rho v, rho w, E
+// (that of the mixed mode), so a run continues bit for bit.
+struct RestartInfo {
+    int nx, ny, nz, kind;
+    long long step;
+    double time;
+};
+bool write_restart(const std::string &path, const RestartInfo &info, const ZoneDesc &Z, const double *const a[5]);
+// reads into the interior of padded arrays; false (with a message) on a
+// missing file, a wrong magic or sizes that differ from the zone
+bool read_restart(const std::string &path, RestartInfo &info, const ZoneDesc &Z, double *const a[5]);
+
 }  // namespace ns3d_host

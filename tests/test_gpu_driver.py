@@ -227,3 +227,40 @@ def test_driver_restart_and_resample_from_tecplot(gpu, tmp_path):
     want = [src_grid[2 * k, 2 * j, 2 * i] for k in range(n) for j in range(n) for i in range(n)]
     data = [3, 4, 5, 6, 8]
     assert [cols(a, data) for a in got] == [cols(b, data) for b in want]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("split", [3, 4])
+def test_driver_restart_continues_bitwise(gpu, tmp_path, precision, split):
+    """Seven steps in one run against `split` steps, a binary restart file,
+    and the rest in a second run: the final restart files are identical
+    (step 3 ends at an output time, so the resumed output schedule is
+    exercised too)."""
+    n = 16
+    x = np.arange(n) * (6.283185307179586 / n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    st = [1.0 + 0.1 * np.sin(X + 2 * Z), np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z),
+          0.2 * np.sin(Y + Z), 1.0 / (1.4 * 0.01) + 0.5 * np.cos(X + Y)]
+    (tmp_path / "ic.raw").write_bytes(b"".join(np.ascontiguousarray(a).tobytes() for a in st))
+    runs = {}
+    for name, steps in (("whole", 7), ("first", split), ("second", 7)):
+        d = tmp_path / name
+        d.mkdir()
+        inp = solver_in(d / "solver.in", n, max_steps=steps, precision=precision)
+        args = ["--no-field-files", "--restart-out", str(d / "r.bin")]
+        if name == "second":
+            args += ["--restart-in", str(tmp_path / "first" / "r.bin")]
+        else:
+            args += ["--ic", f"raw:{tmp_path / 'ic.raw'}"]
+        runs[name] = run_driver(d, inp, *args)
+    a = (tmp_path / "whole" / "r.bin").read_bytes()
+    b = (tmp_path / "second" / "r.bin").read_bytes()
+    assert a[:8] == b"NS3DRST1" and len(a) == 40 + 5 * n ** 3 * 8
+    assert len(a) == len(b)
+    diff = np.flatnonzero(np.frombuffer(a[40:], np.uint8) != np.frombuffer(b[40:], np.uint8))
+    assert a[:40] == b[:40] and diff.size == 0, diff[:10]
+    # the histories of the second run continue those of the whole run
+    whole = (tmp_path / "whole" / "output" / "output_residuals.dat").read_text().splitlines()
+    second = (tmp_path / "second" / "output" / "output_residuals.dat").read_text().splitlines()
+    tail = [line for line in whole[2:] if int(line.split()[0]) > split]
+    assert [line for line in second if line and line[0].isdigit()] == tail

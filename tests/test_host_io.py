@@ -168,3 +168,35 @@ def test_residual_file(tool, tmp_path):
                      "10 1.00000000e-03 2.00000000e-03 3.00000000e-03 4.00000000e-03 5.00000000e-03 1.25000000e-01"]
     tool("resid", p, 1, 0, 0, 0, 0, 0, 0)          # step 1 starts the file again
     assert len(p.read_text().splitlines()) == 3
+
+
+def test_restart_round_trip(tool, tmp_path):
+    nx, ny, nz, ng = 6, 5, 4, 2
+    path = tmp_path / "r.bin"
+    tool("rwrite", path, nx, ny, nz, ng, 1, 123456789012, 0.1 + 0.2)
+    raw = path.read_bytes()
+    assert raw[:8] == b"NS3DRST1" and len(raw) == 8 + 16 + 8 + 8 + 5 * nx * ny * nz * 8
+    head = tool("rread", path, nx, ny, nz, ng, tmp_path / "out.raw").split()
+    assert head[:4] == [str(nx), str(ny), str(nz), "1"] and int(head[4]) == 123456789012
+    assert float(head[5]) == 0.1 + 0.2
+    a = load_raw(tmp_path / "out.raw", nx, ny, nz, ng)
+    inner = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    for q in range(5):
+        assert np.array_equal(a[q][inner], pattern(q, i, j, k) + 1e-13 * q)
+        ghost = a[q].copy()
+        ghost[inner] = np.nan
+        assert np.all(np.isnan(ghost))
+
+
+def test_restart_rejects_wrong_size_and_truncation(tool, tmp_path):
+    path = tmp_path / "r.bin"
+    tool("rwrite", path, 6, 5, 4, 2, 0, 3, 1.5)
+    with pytest.raises(AssertionError, match="holds a 6x5x4 grid"):
+        tool("rread", path, 6, 5, 5, 2, tmp_path / "o.raw")
+    path.write_bytes(path.read_bytes()[:-8])
+    with pytest.raises(AssertionError, match="truncated"):
+        tool("rread", path, 6, 5, 4, 2, tmp_path / "o.raw")
+    (tmp_path / "x.bin").write_bytes(b"NOTARST1" + bytes(64))
+    with pytest.raises(AssertionError, match="not a restart file"):
+        tool("rread", tmp_path / "x.bin", 6, 5, 4, 2, tmp_path / "o.raw")
