@@ -179,8 +179,6 @@ int check_supported(const ns3d_params &p, const ns3d_grid &g)
         if ((p.bc[2 * a] == NS3D_BC_PERIODIC) != (p.bc[2 * a + 1] == NS3D_BC_PERIODIC)) return NS3D_ERR_CONFIG;
     if (p.recon == NS3D_RECON_WCNS) return NS3D_ERR_CONFIG;
     if (p.precision != NS3D_PREC_FP64 && p.precision != NS3D_PREC_MIXED) return NS3D_ERR_INVALID_ARG;
-    // the mixed path is component-wise with full-row x blocks
-    if (p.precision == NS3D_PREC_MIXED && p.char_recon) return NS3D_ERR_UNSUPPORTED;
     if (p.fvs_type != NS3D_FVS_STEGER_WARMING && p.fvs_type != NS3D_FVS_LAX_FRIEDRICHS)
         return NS3D_ERR_CONFIG;
     // periodic images and wall mirrors of the ghost layers must lie inside
@@ -239,7 +237,22 @@ int inviscid(ns3d_ctx *c)
 int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
 {
     static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
-    for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
+    if (c->P.char_recon) {
+        // characteristic-wise faces in FP64 from the widened shadow (in the
+        // gradient arrays G[6..11], unused in mixed mode)
+        int st = ensure_faces(c);
+        if (st) return st;
+        Prims64 w;
+        for (int m = 0; m < 6; ++m) w.p[m] = c->f.G[6 + m];
+        TIMED(c, "shadow_to_prims", launch_m_shadow_to_prims64(c->stream, c->dp, c->m, w));
+        Fields fw = c->f;
+        for (int m = 0; m < 6; ++m) fw.W[m] = w.p[m];
+        static const char *fname[3] = {"faces_char_x", "faces_char_y", "faces_char_z"};
+        for (int d = 0; d < 3; ++d) TIMED(c, fname[d], launch_faces(c->stream, c->dp, fw, c->faces, d, 1));
+        TIMED(c, "face_diff", launch_m_face_diff(c->stream, c->dp, c->m, c->faces));
+    } else {
+        for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
+    }
     TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1]));
     const unsigned long long key = (unsigned long long)c->steps_done << 40;
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
@@ -584,7 +597,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.p_ref = p.Rgas;   // p at rho = 1, T = 1: 1 / (gamma Ma^2)
     c->mixed = p.precision == NS3D_PREC_MIXED;
     if (c->mixed) {
-        if (!m_xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
+        if (!p.char_recon && !m_xrow_supported(d)) { ns3d_destroy(c); return NS3D_ERR_UNSUPPORTED; }
         const int st2 = alloc_mixed(c);
         if (st2) { ns3d_destroy(c); return st2; }
     }
@@ -860,7 +873,7 @@ int ns3d_set_precision(ns3d_ctx *c, int precision)
     if (to_mixed == c->mixed) return NS3D_OK;
     c->fw_valid = false;
     if (to_mixed) {
-        if (c->P.char_recon || !m_xrow_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
+        if (!c->P.char_recon && !m_xrow_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
         int st = alloc_mixed(c);
         if (st) return st;
         // conserved U (in place over the primitives) and the FP32 shadow

@@ -50,6 +50,11 @@ struct FaceArrays {
     double *f[15];
 };
 
+// six FP64 cell arrays (rho, u, v, w, p, T) passed by value
+struct Prims64 {
+    double *p[6];
+};
+
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
 hipError_t launch_prim_setup(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_c2p_all(hipStream_t s, const DevParams &P, const Fields &F, const double *const *src);
@@ -74,6 +79,8 @@ hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields 
 hipError_t launch_m_to_prims(hipStream_t s, const DevParams &P, const MFields &M, double *const *dst);
 hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields &M);
 hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out);
+hipError_t launch_m_shadow_to_prims64(hipStream_t s, const DevParams &P, const MFields &M, const Prims64 &out);
+hipError_t launch_m_face_diff(hipStream_t s, const DevParams &P, const MFields &M, const FaceArrays &ff);
 bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by);

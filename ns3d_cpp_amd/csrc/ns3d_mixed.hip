@@ -876,6 +876,44 @@ __global__ void km_fill_bc(DevParams P, MFields M, int face, int kind)
     for (int m = 0; m < 6; ++m) M.S[m][dst] = v[m];
 }
 
+// Characteristic-wise reconstruction (du_inviscous_character,
+// InviscousFlux_new.cpp:165-437) in mixed mode: the shadow widened to FP64
+// primitives (every cell) for the FP64 face kernels, then the face
+// differences into the FP32 accumulator.
+__global__ void km_shadow_to_prims64(DevParams P, MFields M, Prims64 out, int ncell)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    double rho, u, v, w, p;
+    shadow_prims(P, M, id, rho, u, v, w, p);
+    out.p[0][id] = rho;
+    out.p[1][id] = u;
+    out.p[2][id] = v;
+    out.p[3][id] = w;
+    out.p[4][id] = p;
+    out.p[5][id] = 1.0 + (double)M.S[5][id];
+}
+
+// (F(f) - F(f-1)) / d per direction, summed as k_face_diff does, into M.A
+__global__ void km_face_diff(DevParams P, MFields M, FaceArrays ff)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    const int fxr = (k * P.sy + j) * (P.sx - 1) + i;
+    const int fyt = (k * (P.sy - 1) + j) * P.sx + i;
+    const int fzf = (k * P.sy + j) * P.sx + i;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double fx = ff.f[m][fxr] - ff.f[m][fxr - 1];
+        const double fy = ff.f[5 + m][fyt] - ff.f[5 + m][fyt - P.sx];
+        const double fz = ff.f[10 + m][fzf] - ff.f[10 + m][fzf - P.sx * P.sy];
+        M.A[m][id] = (float)(fx * P.idx + fy * P.idy + fz * P.idz);
+    }
+}
+
 // CFL minimum (timestep.cpp:5-33) from the shadow
 __global__ __launch_bounds__(256) void km_cfl(DevParams P, MFields M, unsigned long long *out)
 {
@@ -934,6 +972,19 @@ hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields
         const int n = P.ng * (axis == 0 ? P.sy : P.sx) * (axis == 2 ? P.sy : P.sz);
         hipLaunchKernelGGL(km_fill_bc, dim3((n + 255) / 256), dim3(256), 0, s, P, M, face, P.bc[face]);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_m_shadow_to_prims64(hipStream_t s, const DevParams &P, const MFields &M, const Prims64 &out)
+{
+    const int n = P.sx * P.sy * P.sz;
+    hipLaunchKernelGGL(km_shadow_to_prims64, dim3((n + 255) / 256), dim3(256), 0, s, P, M, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_face_diff(hipStream_t s, const DevParams &P, const MFields &M, const FaceArrays &ff)
+{
+    hipLaunchKernelGGL(km_face_diff, dim3((P.nx + 63) / 64, (P.ny + 3) / 4, P.nz), dim3(64, 4, 1), 0, s, P, M, ff);
     return hipGetLastError();
 }
 

@@ -118,3 +118,43 @@ def test_mixed_rejects_fp64_only_entry_points(gpu):
     with pytest.raises(ns.NS3DError) as e:
         s.du_inviscous()
     assert e.value.status == ns._lib.NS3D_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("recon,fvs", [("WENO5", "StagerWarming"), ("MDCD", "LaxFriedrichs"),
+                                       ("UPWIND_7TH", "StagerWarming")])
+def test_mixed_characteristic_rhs_vs_oracle(gpu, recon, fvs):
+    """char_recon in mixed mode (du_inviscous_character, InviscousFlux_new.cpp:
+    165-437, in FP64 on the widened shadow): one RHS of a 32^3 isotropic
+    state within 2e-5 of the FP64 oracle's."""
+    n = 32
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=19)
+    cfg = dict(CFG, recon=recon, fvs_type=fvs, char_recon=1)
+    p, g = ns.make_params(n, precision="mixed", Ma=Ma, **cfg)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*st)
+    pr, gr = ns.make_params(n, Ma=Ma, **cfg)
+    o = Oracle(pr, gr)
+    o.set_primitive(*st)
+    for m, (a, b) in enumerate(zip(s.compute_rhs(), o.compute_rhs())):
+        a, b = s.interior(a), o.interior(b)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err < 2e-5, (m, err)
+
+
+def test_mixed_characteristic_trajectory(gpu):
+    """30 RK3 steps of char_recon WENO5 at 32^3, mixed against FP64 with the
+    FP64 path's dt: rel-L2 of U <= 1e-5; a non-periodic variant (walls in y)
+    as well."""
+    n = 32
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=23)
+    for bc in (None, ["Periodic", "Periodic", "Wall", "Wall", "Periodic", "Periodic"]):
+        cfg = dict(CFG, char_recon=1, Ma=Ma)
+        if bc:
+            cfg["bc"] = bc
+        d, m = solvers(n, st, **cfg)
+        for _ in range(30):
+            dt = d.timestep()
+            d.rk3_step(dt)
+            m.rk3_step(dt)
+        errs = [rel_l2(m.interior(a), d.interior(b)) for a, b in zip(m.conserved(), d.conserved())]
+        assert max(errs) < 1e-5, (bc, errs)
