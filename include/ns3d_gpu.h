@@ -143,6 +143,9 @@ int ns3d_apply_boundary(ns3d_ctx *ctx);
 /* du_inviscous (InviscousFlux_new.cpp:4-163): face fluxes into device face
  * arrays laid out as idx_fx/idx_fy/idx_fz (field_structures.h:72-81). */
 int ns3d_du_inviscous(ns3d_ctx *ctx);
+/* du_inviscous_character (InviscousFlux_new.cpp:165-437, ns3d_func.h:155)
+ * into the same face arrays, whatever p->char_recon says. */
+int ns3d_du_inviscous_character(ns3d_ctx *ctx);
 /* Download the 5 face-flux arrays of direction dir (0=x,1=y,2=z); each has
  * (sx-1)*sy*sz, sx*(sy-1)*sz or sx*sy*(sz-1) doubles. */
 int ns3d_download_face_flux(ns3d_ctx *ctx, int dir, double *f0, double *f1,
@@ -193,6 +196,24 @@ int ns3d_turbulence_statistics(ns3d_ctx *ctx, double *out, int n);
  * U0_m)^2 dx^3) / max |U_m| for rho, rho u, rho v, rho w, E, with U0 the
  * state at the start of the last RK3 step. */
 int ns3d_residuals(ns3d_ctx *ctx, double out[5]);
+
+/* ---- point functions of the reference surface, batched on the device.
+ * Host arrays in and out; no context; the call returns when the results are
+ * in place.  The arithmetic is the FP64 path's (bit-identical to the
+ * reference's expressions). */
+/* split_StagerWarming / split_LaxFriedrichs (InviscousFlux_new.cpp:439-530,
+ * ns3d_func.h:136-145) of n states W = rho, u, v, w, p (5 arrays of n), with
+ * E and c formed as primitiveToConserved forms them (field_structures.h:
+ * 445-449); p->fvs_type selects the split, p->sw_eps the SW variant.
+ * (n1, n2, n3) must be a unit axis.  Fp, Fm: 5 arrays of n each. */
+int ns3d_split_batch(const ns3d_params *p, long long n, const double *const W[5], int n1, int n2, int n3,
+                     double *const Fp[5], double *const Fm[5]);
+/* reconstruct_select (reconstruction.cpp:13-85, ns3d_func.h:96) of n
+ * stencils of `width` values each (row-major), reversed for flag < 0.
+ * NS3D_ERR_CONFIG when width is not the scheme's stencil (the reference
+ * throws std::runtime_error); other schemes take its averaging fallback. */
+int ns3d_reconstruct_batch(const ns3d_params *p, long long n, int width, const double *stencils, double flag,
+                           double *out);
 
 /* Switch the arithmetic of an existing context between NS3D_PREC_FP64 and
  * NS3D_PREC_MIXED, keeping its state (SolverParams has no counterpart: the

@@ -71,6 +71,33 @@ def make_params(n, L=2.0 * np.pi, **kw):
     return p, g
 
 
+def split_batch(params, W, axis):
+    """split_StagerWarming / split_LaxFriedrichs of n states on the device:
+    W = (rho, u, v, w, p) arrays of n; returns (Fp, Fm), 5 arrays each."""
+    W = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in W]
+    n = W[0].size
+    Fp = [np.empty(n) for _ in range(5)]
+    Fm = [np.empty(n) for _ in range(5)]
+    arr = C.c_void_p * 5
+    nvec = [1 if axis == d else 0 for d in range(3)]
+    st = lib().ns3d_split_batch(C.byref(params), n, arr(*[a.ctypes.data for a in W]), *nvec,
+                                arr(*[a.ctypes.data for a in Fp]), arr(*[a.ctypes.data for a in Fm]))
+    if st:
+        raise NS3DError(st, "ns3d_split_batch")
+    return Fp, Fm
+
+
+def reconstruct_batch(params, stencils, flag):
+    """reconstruct_select of n stencils (an (n, width) array) on the device."""
+    st_ = np.ascontiguousarray(stencils, dtype=np.float64)
+    n, width = st_.shape
+    out = np.empty(n)
+    st = lib().ns3d_reconstruct_batch(C.byref(params), n, width, st_.ctypes.data, float(flag), out.ctypes.data)
+    if st:
+        raise NS3DError(st, "ns3d_reconstruct_batch")
+    return out
+
+
 def read_params(path):
     """(Params, Grid) of a solver.in file (read_solver_params_from_file,
     initialization.cpp:199-380), finalized."""
@@ -161,6 +188,9 @@ class Solver:
 
     def du_inviscous(self):
         self._check(self._L.ns3d_du_inviscous(self.ctx), "du_inviscous")
+
+    def du_inviscous_character(self):
+        self._check(self._L.ns3d_du_inviscous_character(self.ctx), "du_inviscous_character")
 
     def face_flux(self, d):
         sz, sy, sx = self.shape

@@ -72,6 +72,11 @@ SIGNATURES = {
     "ns3d_download_rhs": (C.c_int, [C.c_void_p] + [DP] * 5),
     "ns3d_apply_boundary": (C.c_int, [C.c_void_p]),
     "ns3d_du_inviscous": (C.c_int, [C.c_void_p]),
+    "ns3d_du_inviscous_character": (C.c_int, [C.c_void_p]),
+    "ns3d_split_batch": (C.c_int, [C.POINTER(Params), C.c_longlong, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_void_p]),
+    "ns3d_reconstruct_batch": (C.c_int, [C.POINTER(Params), C.c_longlong, C.c_int, C.c_void_p, C.c_double,
+                                         C.c_void_p]),
     "ns3d_download_face_flux": (C.c_int, [C.c_void_p, C.c_int] + [DP] * 5),
     "ns3d_compute_gradients": (C.c_int, [C.c_void_p]),
     "ns3d_download_gradients": (C.c_int, [C.c_void_p, DP * 12]),

@@ -815,6 +815,16 @@ int ns3d_du_inviscous(ns3d_ctx *c)
     return stream_sync(c);
 }
 
+int ns3d_du_inviscous_character(ns3d_ctx *c)
+{
+    if (!c) return NS3D_ERR_INVALID_ARG;
+    if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
+    int st = ensure_faces(c);
+    if (st) return st;
+    for (int d = 0; d < 3; ++d) HIPCALL(c, launch_faces(c->stream, c->dp, c->f, c->faces, d, 1));
+    return stream_sync(c);
+}
+
 int ns3d_download_face_flux(ns3d_ctx *c, int dir, double *f0, double *f1, double *f2, double *f3, double *f4)
 {
     if (!c || dir < 0 || dir > 2 || !c->face_slab) return NS3D_ERR_INVALID_ARG;
@@ -1319,6 +1329,87 @@ int ns3d_adj_memory(const ns3d_ctx *c, long long *bytes)
     b += c->ck_states * 6 * c->ncell * (long long)sizeof(double);
     b += c->adj_nsums * (long long)sizeof(double);
     *bytes = b;
+    return NS3D_OK;
+}
+
+// ------------------------------------------------------ point functions
+
+}  // extern "C"
+
+namespace {
+
+// the scalars the point functions read
+DevParams point_params(const ns3d_params &p)
+{
+    DevParams d{};
+    d.gamma = p.gamma; d.Rgas = p.Rgas; d.sw_eps = p.sw_eps;
+    d.mdcd_disp = p.mdcd_disp; d.mdcd_diss = p.mdcd_diss; d.ud7l_alpha = p.ud7l_alpha;
+    d.fvs = p.fvs_type; d.recon = p.recon;
+    return d;
+}
+
+// device buffer freed on scope exit
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ns3d_split_batch(const ns3d_params *p, long long n, const double *const W[5], int n1, int n2, int n3,
+                     double *const Fp[5], double *const Fm[5])
+{
+    if (!p || n < 0 || !W || !Fp || !Fm) return NS3D_ERR_INVALID_ARG;
+    for (int m = 0; m < 5; ++m)
+        if (!W[m] || !Fp[m] || !Fm[m]) return NS3D_ERR_INVALID_ARG;
+    if (p->fvs_type != NS3D_FVS_STEGER_WARMING && p->fvs_type != NS3D_FVS_LAX_FRIEDRICHS) return NS3D_ERR_CONFIG;
+    int dir;
+    if (n1 == 1 && n2 == 0 && n3 == 0) dir = 0;
+    else if (n1 == 0 && n2 == 1 && n3 == 0) dir = 1;
+    else if (n1 == 0 && n2 == 0 && n3 == 1) dir = 2;
+    else return NS3D_ERR_INVALID_ARG;
+    if (n == 0) return NS3D_OK;
+    DevBuf buf;
+    const size_t bytes = (size_t)n * sizeof(double);
+    if (hipMalloc(&buf.p, 15 * bytes) != hipSuccess) return NS3D_ERR_OOM;
+    double *q = (double *)buf.p;
+    SplitBatch b;
+    for (int m = 0; m < 5; ++m) {
+        b.w[m] = q + (size_t)m * n;
+        b.fp[m] = q + (size_t)(5 + m) * n;
+        b.fm[m] = q + (size_t)(10 + m) * n;
+        if (hipMemcpy((void *)b.w[m], W[m], bytes, hipMemcpyHostToDevice) != hipSuccess) return NS3D_ERR_HIP;
+    }
+    if (launch_split_batch(nullptr, point_params(*p), n, b, dir) != hipSuccess) return NS3D_ERR_HIP;
+    for (int m = 0; m < 5; ++m) {
+        if (hipMemcpy(Fp[m], b.fp[m], bytes, hipMemcpyDeviceToHost) != hipSuccess) return NS3D_ERR_HIP;
+        if (hipMemcpy(Fm[m], b.fm[m], bytes, hipMemcpyDeviceToHost) != hipSuccess) return NS3D_ERR_HIP;
+    }
+    return NS3D_OK;
+}
+
+int ns3d_reconstruct_batch(const ns3d_params *p, long long n, int width, const double *stencils, double flag,
+                           double *out)
+{
+    if (!p || n < 0 || width < 0 || (n > 0 && (!out || (width > 0 && !stencils)))) return NS3D_ERR_INVALID_ARG;
+    int need = 0;
+    switch (p->recon) {
+    case NS3D_RECON_WENO5: case NS3D_RECON_MDCD: case NS3D_RECON_MDCD_HYBRID: need = 6; break;
+    case NS3D_RECON_LINEAR: need = 2; break;
+    case NS3D_RECON_UPWIND_7TH: case NS3D_RECON_UD7L: need = 8; break;
+    default: break;
+    }
+    if (need && width != need) return NS3D_ERR_CONFIG;
+    if (n == 0) return NS3D_OK;
+    DevBuf buf;
+    const size_t sb = (size_t)n * width * sizeof(double), ob = (size_t)n * sizeof(double);
+    if (hipMalloc(&buf.p, sb + ob) != hipSuccess) return NS3D_ERR_OOM;
+    double *dst = (double *)buf.p, *dout = dst + (size_t)n * width;
+    if (sb && hipMemcpy(dst, stencils, sb, hipMemcpyHostToDevice) != hipSuccess) return NS3D_ERR_HIP;
+    if (launch_recon_batch(nullptr, point_params(*p), n, width, dst, flag, dout) != hipSuccess) return NS3D_ERR_HIP;
+    if (hipMemcpy(out, dout, ob, hipMemcpyDeviceToHost) != hipSuccess) return NS3D_ERR_HIP;
     return NS3D_OK;
 }
 

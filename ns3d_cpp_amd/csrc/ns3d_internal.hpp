@@ -50,6 +50,13 @@ struct FaceArrays {
     double *f[15];
 };
 
+// arrays of the batched split: states in, split fluxes out (n each)
+struct SplitBatch {
+    const double *w[5];
+    double *fp[5];
+    double *fm[5];
+};
+
 // six FP64 cell arrays (rho, u, v, w, p, T) passed by value
 struct Prims64 {
     double *p[6];
@@ -72,6 +79,9 @@ hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, 
 hipError_t launch_grad(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_vflux(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsigned long long *out);
+hipError_t launch_split_batch(hipStream_t s, const DevParams &P, long long n, const SplitBatch &b, int dir);
+hipError_t launch_recon_batch(hipStream_t s, const DevParams &P, long long n, int width, const double *st,
+                              double flag, double *out);
 hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n);
 int ke_blocks(const DevParams &P);
 hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields &M, const double *const *src,

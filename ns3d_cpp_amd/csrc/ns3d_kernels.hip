@@ -670,6 +670,49 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
     }
 }
 
+// ------------------------------------------------------ point functions
+
+// split_state of n states (one per thread) for the batched point API
+template <int FVS, int DIR>
+__global__ __launch_bounds__(256) void k_split_batch(DevParams P, long long n, SplitBatch b)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double Fp[5], Fm[5];
+    split_state<FVS, DIR>(P, b.w[0][t], b.w[1][t], b.w[2][t], b.w[3][t], b.w[4][t], Fp, Fm);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { b.fp[m][t] = Fp[m]; b.fm[m][t] = Fm[m]; }
+}
+
+// reconstruct_select on n stencils of S values (reversed for flag < 0)
+template <int RECON>
+__global__ __launch_bounds__(256) void k_recon_batch(DevParams P, long long n, const double *st, double flag,
+                                                     double *out)
+{
+    constexpr int S = Stencil<RECON>::S;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) s[q] = st[t * S + (flag < 0.0 ? S - 1 - q : q)];
+    out[t] = recon<RECON>(P, s);
+}
+
+// the averaging fallback of reconstruction.cpp:77-84 for other schemes
+__global__ __launch_bounds__(256) void k_recon_fallback(long long n, int width, const double *st, double flag,
+                                                        double *out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double *v = st + t * width;
+    if (width >= 2) {
+        const int mid = width / 2;
+        out[t] = flag >= 0.0 ? 0.5 * (v[mid] + v[mid - 1]) : 0.5 * (v[mid] + v[mid + 1]);
+    } else {
+        out[t] = width == 0 ? 0.0 : v[0];
+    }
+}
+
 // ----------------------------------------------------------- reductions
 
 __device__ __forceinline__ double wave_min(double v)
@@ -997,6 +1040,38 @@ hipError_t launch_grad(hipStream_t s, const DevParams &P, const Fields &F)
 hipError_t launch_vflux(hipStream_t s, const DevParams &P, const Fields &F)
 {
     hipLaunchKernelGGL(k_vflux, cell_grid(P), kCellBlock, 0, s, P, F);
+    return hipGetLastError();
+}
+
+template <int FVS>
+static void split_batch_dir(hipStream_t s, const DevParams &P, long long n, const SplitBatch &b, int dir)
+{
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (dir == 0) hipLaunchKernelGGL((k_split_batch<FVS, 0>), g, dim3(256), 0, s, P, n, b);
+    else if (dir == 1) hipLaunchKernelGGL((k_split_batch<FVS, 1>), g, dim3(256), 0, s, P, n, b);
+    else hipLaunchKernelGGL((k_split_batch<FVS, 2>), g, dim3(256), 0, s, P, n, b);
+}
+
+hipError_t launch_split_batch(hipStream_t s, const DevParams &P, long long n, const SplitBatch &b, int dir)
+{
+    if (P.fvs == FVS_SW) split_batch_dir<FVS_SW>(s, P, n, b, dir);
+    else split_batch_dir<FVS_LF>(s, P, n, b, dir);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon_batch(hipStream_t s, const DevParams &P, long long n, int width, const double *st,
+                              double flag, double *out)
+{
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    switch (P.recon) {
+    case R_WENO5: hipLaunchKernelGGL(k_recon_batch<R_WENO5>, g, b, 0, s, P, n, st, flag, out); break;
+    case R_LINEAR: hipLaunchKernelGGL(k_recon_batch<R_LINEAR>, g, b, 0, s, P, n, st, flag, out); break;
+    case R_MDCD: hipLaunchKernelGGL(k_recon_batch<R_MDCD>, g, b, 0, s, P, n, st, flag, out); break;
+    case R_MDCD_HYBRID: hipLaunchKernelGGL(k_recon_batch<R_MDCD_HYBRID>, g, b, 0, s, P, n, st, flag, out); break;
+    case R_UD7: hipLaunchKernelGGL(k_recon_batch<R_UD7>, g, b, 0, s, P, n, st, flag, out); break;
+    case R_UD7L: hipLaunchKernelGGL(k_recon_batch<R_UD7L>, g, b, 0, s, P, n, st, flag, out); break;
+    default: hipLaunchKernelGGL(k_recon_fallback, g, b, 0, s, n, width, st, flag, out); break;
+    }
     return hipGetLastError();
 }
 

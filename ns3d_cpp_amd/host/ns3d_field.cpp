@@ -66,6 +66,56 @@ void du_inviscous(Field3D &F, const SolverParams &)
     check(ns3d_du_inviscous(F.ctx), "du_inviscous", &F);
 }
 
+void du_inviscous_character(Field3D &F, const SolverParams &)
+{
+    check(ns3d_du_inviscous_character(F.ctx), "du_inviscous_character", &F);
+}
+
+void compute_vis_flux(Field3D &F, const GridDesc &, const SolverParams &)
+{
+    check(ns3d_compute_vis_flux(F.ctx), "compute_vis_flux", &F);
+}
+
+namespace {
+
+void split_point(const Field3D &F, std::vector<double> &Fp, std::vector<double> &Fm, int id, int n1, int n2, int n3,
+                 SolverParams P, int fvs)
+{
+    P.fvs_type = fvs;
+    const double *W[5] = {&F.rho[(size_t)id], &F.u[(size_t)id], &F.v[(size_t)id], &F.w[(size_t)id],
+                          &F.p[(size_t)id]};
+    Fp.resize(5);
+    Fm.resize(5);
+    double *fp[5], *fm[5];
+    for (int m = 0; m < 5; ++m) { fp[m] = &Fp[(size_t)m]; fm[m] = &Fm[(size_t)m]; }
+    check(ns3d_split_batch(&P, 1, W, n1, n2, n3, fp, fm), "split");
+}
+
+}  // namespace
+
+void split_StagerWarming(const Field3D &F, std::vector<double> &F_plus, std::vector<double> &F_minus, int id, int n1,
+                         int n2, int n3, const SolverParams &P)
+{
+    split_point(F, F_plus, F_minus, id, n1, n2, n3, P, NS3D_FVS_STEGER_WARMING);
+}
+
+void split_LaxFriedrichs(const Field3D &F, std::vector<double> &F_plus, std::vector<double> &F_minus, int id, int n1,
+                         int n2, int n3, const SolverParams &P)
+{
+    split_point(F, F_plus, F_minus, id, n1, n2, n3, P, NS3D_FVS_LAX_FRIEDRICHS);
+}
+
+double reconstruct_select(const std::vector<double> &vstencil, double flag, const SolverParams P)
+{
+    double out = 0.0;
+    const int st = ns3d_reconstruct_batch(&P, 1, (int)vstencil.size(), vstencil.data(), flag, &out);
+    if (st == NS3D_ERR_CONFIG)
+        throw Error(st, "reconstruct_select: stencil of " + std::to_string(vstencil.size()) +
+                            " values does not match the reconstruction");
+    check(st, "reconstruct_select");
+    return out;
+}
+
 void compute_gradients(Field3D &F, const GridDesc &, const SolverParams &)
 {
     check(ns3d_compute_gradients(F.ctx), "compute_gradients", &F);

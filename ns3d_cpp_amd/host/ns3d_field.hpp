@@ -60,6 +60,17 @@ void check(int st, const char *what, const Field3D *F = nullptr);
 // ns3d_func.h call surface
 void apply_boundary(Field3D &F, GridDesc &G, const SolverParams &P);     // :56
 void du_inviscous(Field3D &F, const SolverParams &P);                      // :154
+void du_inviscous_character(Field3D &F, const SolverParams &P);            // :155
+void compute_vis_flux(Field3D &F, const GridDesc &G, const SolverParams &P);   // :163
+// the per-point functions (:96, :136-145), on the host mirrors of F (valid
+// after sync_to_host) through the batched device entry points
+void split_StagerWarming(const Field3D &F, std::vector<double> &F_plus, std::vector<double> &F_minus, int id, int n1,
+                         int n2, int n3, const SolverParams &P);
+void split_LaxFriedrichs(const Field3D &F, std::vector<double> &F_plus, std::vector<double> &F_minus, int id, int n1,
+                         int n2, int n3, const SolverParams &P);
+// throws Error (a std::runtime_error) on a stencil of the wrong size, as the
+// reference throws
+double reconstruct_select(const std::vector<double> &vstencil, double flag, const SolverParams P);
 void compute_gradients(Field3D &F, const GridDesc &G, const SolverParams &P);  // :157
 void compute_viscous_flux(Field3D &F, const SolverParams &P);              // :160
 void compute_rhs(Field3D &F, GridDesc &G, SolverParams &P);                // :99
