@@ -251,11 +251,78 @@ __device__ __forceinline__ void linear_coeffs(const DevParams &P, double *c)
     }
 }
 
+// mdcd_hybrid_reconstruction (reconstruction.cpp:151-225, as the oracle and
+// recon_mdcd_hybrid restate it) for a generic scalar: the shock sensor
+// chooses the branch on values (its derivative is zero where it is
+// defined), |x| is differentiated as sign(x)
+template <class T>
+__device__ __forceinline__ T recon_mdcd_hybrid_t(const DevParams &P, const T *s)
+{
+    const double disp = P.mdcd_disp, diss = P.mdcd_diss;
+    const double eps_small = 1e-40;
+    const double eps = 0.9 * 0.4 / (1.0 - 0.9 * 0.4) * 1e-4;
+    const T f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4], f5 = s[5];
+    {
+        const double g0 = value_of(f0), g1 = value_of(f1), g2 = value_of(f2);
+        const double g3 = value_of(f3), g4 = value_of(f4), g5 = value_of(f5);
+        const double a1 = fabs(g2 - g1) + fabs(g2 - 2 * g1 + g0);
+        const double b1 = fabs(g2 - g3) + fabs(g2 - 2 * g3 + g4);
+        const double a2 = fabs(g3 - g2) + fabs(g3 - 2 * g2 + g1);
+        const double b2 = fabs(g3 - g4) + fabs(g3 - 2 * g4 + g5);
+        const double sai1 = (2 * a1 * b1 + eps) / (a1 * a1 + b1 * b1 + eps);
+        const double sai2 = (2 * a2 * b2 + eps) / (a2 * a2 + b2 * b2 + eps);
+        const double sai = (sai2 < sai1) ? sai2 : sai1;
+        if (sai > 0.4) {
+            double c[6];
+            c[0] = 1.0 / 2.0 * disp + 1.0 / 2.0 * diss;
+            c[1] = -3.0 / 2.0 * disp - 5.0 / 2.0 * diss - 1.0 / 12.0;
+            c[2] = 7.0 / 12.0 + disp + 5.0 * diss;
+            c[3] = 7.0 / 12.0 + disp - 5.0 * diss;
+            c[4] = -3.0 / 2.0 * disp + 5.0 / 2.0 * diss - 1.0 / 12.0;
+            c[5] = 1.0 / 2.0 * disp - 1.0 / 2.0 * diss;
+            T r = s[0] * c[0];
+#pragma unroll
+            for (int q = 1; q < 6; ++q) r = r + s[q] * c[q];
+            return r;
+        }
+    }
+    const T q0 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) / 6.0;
+    const T q1 = (-f1 + 5.0 * f2 + 2.0 * f3) / 6.0;
+    const T q2 = (2.0 * f2 + 5.0 * f3 - f4) / 6.0;
+    const T q3 = (11.0 * f3 - 7.0 * f4 + 2.0 * f5) / 6.0;
+    const double d0 = 1.5 * (disp + diss);
+    const double d1 = 0.5 - 1.5 * (disp - 3 * diss);
+    const double d2 = 0.5 - 1.5 * (disp + 3 * diss);
+    const double d3 = 1.5 * (disp - diss);
+    const T t0 = f0 - 2.0 * f1 + f2, t0b = f0 - 4.0 * f1 + 3.0 * f2;
+    const T t1 = f1 - 2.0 * f2 + f3, t1b = f1 - f3;
+    const T t2 = f2 - 2.0 * f3 + f4, t2b = 3.0 * f2 - 4.0 * f3 + f4;
+    const T beta0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (t0b * t0b);
+    const T beta1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (t1b * t1b);
+    const T beta2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (t2b * t2b);
+    const T beta3 = (271799.0 * (f0 * f0) +
+                     f0 * (-2380800.0 * f1 + 4086352.0 * f2 - 3462252.0 * f3 + 1458762.0 * f4 - 245620.0 * f5) +
+                     f1 * (5653317.0 * f1 - 20427884.0 * f2 + 17905032.0 * f3 - 7727988.0 * f4 + 1325006.0 * f5) +
+                     f2 * (19510972.0 * f2 - 35817664.0 * f3 + 15929912.0 * f4 - 2792660.0 * f5) +
+                     f3 * (17195652.0 * f3 - 15880404.0 * f4 + 2863984.0 * f5) +
+                     f4 * (3824847.0 * f4 - 1429976.0 * f5) + 139633.0 * (f5 * f5)) /
+                    120960.0;
+    const T tau6 = fabs_t(beta3 - (beta0 + 4.0 * beta1 + beta2) / 6.0);
+    const T g0 = 20.0 + tau6 / (beta0 + eps_small);
+    const T g1 = 20.0 + tau6 / (beta1 + eps_small);
+    const T g2 = 20.0 + tau6 / (beta2 + eps_small);
+    const T g3 = 20.0 + tau6 / (beta3 + eps_small);
+    const T aa0 = d0 * (g0 * g0), aa1 = d1 * (g1 * g1), aa2 = d2 * (g2 * g2), aa3 = d3 * (g3 * g3);
+    return (aa0 * q0 + aa1 * q1 + aa2 * q2 + aa3 * q3) / (aa0 + aa1 + aa2 + aa3);
+}
+
 template <int RECON, class T>
 __device__ __forceinline__ T recon_t(const DevParams &P, const T *s)
 {
     if constexpr (RECON == R_WENO5) {
         return recon_weno5_t(s);
+    } else if constexpr (RECON == R_MDCD_HYBRID) {
+        return recon_mdcd_hybrid_t(P, s);
     } else {
         constexpr int S = Stencil<RECON>::S;
         double c[S];
@@ -308,6 +375,18 @@ __device__ __forceinline__ void recon_vjp(const DevParams &P, const double *s, d
         sbar[3] = 2.0 * g1 + 5.0 * g2 + gd1 - ge1 - 2.0 * gd2 - 4.0 * ge2;
         sbar[4] = -g2 + gd2 + ge2;
         sbar[5] = 0.0;
+    } else if constexpr (RECON == R_MDCD_HYBRID) {
+        // the local gradient of the 6-point formula by forward mode, seeded
+        // with the identity, then scaled by gbar
+        Dual<6> d[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            d[q] = dconst<6>(s[q]);
+            d[q].d[q] = 1.0;
+        }
+        const Dual<6> r = recon_mdcd_hybrid_t(P, d);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sbar[q] = gbar * r.d[q];
     } else {
         double c[S];
         linear_coeffs<RECON>(P, c);

@@ -688,8 +688,8 @@ int ad_blocks(const DevParams &P) { return ((P.nx + 63) / 64) * ((P.ny + 3) / 4)
 
 bool ad_supported(const DevParams &P)
 {
-    return (P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_UD7 ||
-            P.recon == R_UD7L) &&
+    return (P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ||
+            P.recon == R_UD7 || P.recon == R_UD7L) &&
            P.periodic[0] && P.periodic[1] && P.periodic[2];
 }
 
@@ -708,6 +708,7 @@ static void tlm_faces_f(hipStream_t s, const DevParams &P, const Fields &F, cons
     case R_WENO5: tlm_faces_r<R_WENO5, FVS>(s, P, F, A, dir); break;
     case R_LINEAR: tlm_faces_r<R_LINEAR, FVS>(s, P, F, A, dir); break;
     case R_MDCD: tlm_faces_r<R_MDCD, FVS>(s, P, F, A, dir); break;
+    case R_MDCD_HYBRID: tlm_faces_r<R_MDCD_HYBRID, FVS>(s, P, F, A, dir); break;
     case R_UD7: tlm_faces_r<R_UD7, FVS>(s, P, F, A, dir); break;
     default: tlm_faces_r<R_UD7L, FVS>(s, P, F, A, dir); break;
     }
@@ -794,6 +795,7 @@ static void adj_sweep_f(hipStream_t s, const DevParams &P, const Fields &F, cons
     case R_WENO5: adj_sweep_r<R_WENO5, FVS>(s, P, F, A, dir, seg); break;
     case R_LINEAR: adj_sweep_r<R_LINEAR, FVS>(s, P, F, A, dir, seg); break;
     case R_MDCD: adj_sweep_r<R_MDCD, FVS>(s, P, F, A, dir, seg); break;
+    case R_MDCD_HYBRID: adj_sweep_r<R_MDCD_HYBRID, FVS>(s, P, F, A, dir, seg); break;
     case R_UD7: adj_sweep_r<R_UD7, FVS>(s, P, F, A, dir, seg); break;
     default: adj_sweep_r<R_UD7L, FVS>(s, P, F, A, dir, seg); break;
     }

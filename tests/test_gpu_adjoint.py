@@ -87,7 +87,18 @@ DOT_CASES = [
     dict(recon="MDCD", fvs_type="LaxFriedrichs", vis_scheme="C8th"),
     dict(recon="UPWIND_7TH", fvs_type="StagerWarming", vis_scheme="C8th"),
     dict(recon="UD7L", fvs_type="StagerWarming", vis_scheme="C6th", ud7l_alpha=0.3),
+    dict(recon="MDCD_HYBRID", fvs_type="StagerWarming", vis_scheme="C8th"),
 ]
+
+
+def front_state(n):
+    """state() with density and pressure fronts about one cell wide, so that
+    the MDCD-hybrid sensor selects its nonlinear branch near them."""
+    prims, Ma = state(n)
+    rho, u, v, w, p = prims
+    x = np.arange(n) * 2 * np.pi / n
+    f = 0.25 * np.tanh(4.0 * np.sin(x) * n / (2 * np.pi))[None, None, :]
+    return (rho * (1.0 + f), u, v, w, p * (1.0 + 0.5 * f)), Ma
 
 
 @pytest.mark.parametrize("cfg", DOT_CASES, ids=lambda c: f"{c['recon']}-{c['fvs_type']}-{c['vis_scheme']}")
@@ -108,7 +119,8 @@ def test_dot_product_identity(gpu, cfg):
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
-@pytest.mark.parametrize("cfg", DOT_CASES[:2] + DOT_CASES[5:6], ids=["weno5-sw", "weno5-lf", "ud7"])
+@pytest.mark.parametrize("cfg", DOT_CASES[:2] + DOT_CASES[5:6] + DOT_CASES[7:8],
+                         ids=["weno5-sw", "weno5-lf", "ud7", "mdcd-hybrid"])
 def test_dot_product_marching_sweeps(gpu, cfg, monkeypatch):
     """The column-marching adjoint sweeps (used for lines too long for LDS)
     against the TLM, and against the whole-line sweeps."""
@@ -158,6 +170,27 @@ def test_tlm_against_finite_differences(gpu, cfg):
     den = np.sqrt(sum(np.sum(s.interior(b) ** 2) for b in fd))
     print("TLM vs FD rel", num / den)
     assert num / den < 1e-7
+
+
+def test_tlm_mdcd_hybrid_fronts(gpu):
+    """MDCD_HYBRID on a state with fronts (both sensor branches in use): the
+    TLM against central differences, and the dot-product identity."""
+    n = 16
+    cfg = dict(recon="MDCD_HYBRID", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = front_state(n)
+    s = solver(n, prims, Ma, **cfg)
+    dU = [s._full(a) for a in smooth_dir(n, 7, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    tl = s.tlm_rhs(dU)
+    fd = _fd_rhs(n, prims, Ma, cfg, dU, 1e-6)
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(tl, fd)))
+    den = np.sqrt(sum(np.sum(s.interior(b) ** 2) for b in fd))
+    print("hybrid TLM vs FD rel", num / den)
+    assert num / den < 1e-6
+    rng = np.random.default_rng(13)
+    y = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    RTy, _ = s.adj_rhs(y, 0.0)
+    lhs, rhs = interior_dot(s, y, tl), interior_dot(s, RTy, dU)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
 def _sum_phi(s):
@@ -331,7 +364,8 @@ def test_gradient_against_oracle_finite_differences(gpu):
 def test_unsupported_settings(gpu):
     n = 16
     prims, Ma = state(n)
-    for cfg in (dict(recon="MDCD_HYBRID"), dict(precision="mixed"), dict(char_recon=1)):
+    for cfg in (dict(precision="mixed"), dict(char_recon=1),
+                dict(bc=["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])):
         s = solver(n, prims, Ma, **cfg)
         with pytest.raises(ns.NS3DError):
             s.tlm_rhs([np.zeros((n, n, n))] * 5)
