@@ -60,7 +60,9 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"], verbose)
+    # --no-undefined: a declaration without a definition fails here, not at load time
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs, "-lhipfft"],
+         verbose)
     os.replace(tmp, LIB)
     drv_src = os.path.join(HOST, "ns3d_gpu_run.cpp")
     if os.path.exists(drv_src):

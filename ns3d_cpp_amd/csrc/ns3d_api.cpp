@@ -44,6 +44,7 @@ struct ns3d_ctx {
     int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
     int sweep_minw = 2;                      // waves-per-EU floor of k_sweep
     int m_gf_block[2] = {64, 4};             // workgroup shape of km_gradflux
+    int m_vd_block[2] = {64, 4};             // workgroup shape of km_vdiv_z
     unsigned long long bad_key = ~0ull;
     long long steps_done = 0;
     std::string err;
@@ -256,7 +257,8 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
     TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1]));
     const unsigned long long key = (unsigned long long)c->steps_done << 40;
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
-          launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, key));
+          launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, key, c->m_vd_block[0],
+                        c->m_vd_block[1]));
     return NS3D_OK;
 }
 
@@ -630,6 +632,13 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         if (std::sscanf(env, "%dx%d", &bx, &by) == 2 && bx > 0 && by > 0 && bx % 32 == 0 && bx * by <= 256) {
             c->m_gf_block[0] = bx;
             c->m_gf_block[1] = by;
+        }
+    }
+    if (const char *env = std::getenv("NS3D_M_VDIV_BLOCK")) {   // "BXxBY"
+        int bx = 0, by = 0;
+        if (std::sscanf(env, "%dx%d", &bx, &by) == 2 && bx > 0 && by > 0 && bx % 32 == 0 && bx * by <= 256) {
+            c->m_vd_block[0] = bx;
+            c->m_vd_block[1] = by;
         }
     }
     if (const char *env = std::getenv("NS3D_SEG_Z")) {

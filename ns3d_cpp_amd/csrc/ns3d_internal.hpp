@@ -95,7 +95,7 @@ bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key);
+                         unsigned long long *flag, unsigned long long key, int bx = 64, int by = 4);
 hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 
 // tangent-linear / adjoint RHS (ns3d_adjoint.hip)

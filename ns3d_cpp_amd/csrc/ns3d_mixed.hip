@@ -1098,10 +1098,10 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
 
 template <int VIS, int BC>
 static void m_vdiv_mode(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                        unsigned long long *flag, unsigned long long key)
+                        unsigned long long *flag, unsigned long long key, int bx, int by)
 {
-    dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
-    dim3 b(64, 4, 1);
+    dim3 g((P.nx + bx - 1) / bx, (P.ny + by - 1) / by, (P.nz + seg - 1) / seg);
+    dim3 b(bx, by, 1);
     switch (mode) {
     case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
     case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
@@ -1112,22 +1112,24 @@ static void m_vdiv_mode(hipStream_t s, const DevParams &P, const MFields &M, int
 
 template <int BC>
 static hipError_t m_vdiv_vis(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                             unsigned long long *flag, unsigned long long key)
+                             unsigned long long *flag, unsigned long long key, int bx, int by)
 {
     switch (P.vis) {
-    case VIS_C8: m_vdiv_mode<VIS_C8, BC>(s, P, M, mode, dt, seg, flag, key); break;
-    case VIS_C6: m_vdiv_mode<VIS_C6, BC>(s, P, M, mode, dt, seg, flag, key); break;
-    case VIS_C4: m_vdiv_mode<VIS_C4, BC>(s, P, M, mode, dt, seg, flag, key); break;
+    case VIS_C8: m_vdiv_mode<VIS_C8, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
+    case VIS_C6: m_vdiv_mode<VIS_C6, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
+    case VIS_C4: m_vdiv_mode<VIS_C4, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key)
+                         unsigned long long *flag, unsigned long long key, int bx, int by)
 {
-    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, key);
-    return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, key);
+    if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
+    if (P.periodic[0] && P.periodic[1] && P.periodic[2])
+        return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, key, bx, by);
+    return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, key, bx, by);
 }
 
 }  // namespace ns3d
