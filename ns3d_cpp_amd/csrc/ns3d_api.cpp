@@ -916,7 +916,8 @@ int ns3d_compute_vis_flux(ns3d_ctx *c)
     if (!c) return NS3D_ERR_INVALID_ARG;
     if (c->mixed) return NS3D_ERR_UNSUPPORTED;   // FP64 parity entry point
     // rhs = 0, then the viscous divergence of the stored fluxes
-    HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[0], 0ull, (int)(5 * c->ncell)));
+    for (int m = 0; m < 5; ++m)   // one fill per array: each holds < 2^31 cells
+        HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[m], 0ull, (int)c->ncell));
     HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0ull));
     return stream_sync(c);
 }

@@ -109,6 +109,10 @@ def test_create_argument_errors():
     # a periodic side facing a non-periodic one (validated before any device call)
     q, h = ns.make_params(16, bc=["Periodic", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_CONFIG
+    # 32-bit cell indices: a padded array of 2^31 cells or more is refused
+    # (validated before any device call)
+    q, h = ns.make_params(1291, recon="WENO5", vis_scheme="C8th")
+    assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_INVALID_ARG
     q, h = ns.make_params(2, recon="WENO5", vis_scheme="C8th")     # n < ng
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_GRID_TOO_SMALL
     for s in range(0, 9):
