@@ -682,6 +682,18 @@ static void eigen_matrices(const double u[5], double nx, double ny, double nz, d
     Lm[4][3] = chi * nz - K1 * w0; Lm[4][4] = K1;
 }
 
+/* the Roe average and eigen-matrices above, exported for their tests:
+ * ul, ur = rho, u, v, w, p; L and R row-major 5x5 */
+void orc_char_matrices(const double ul[5], const double ur[5], int n1, int n2, int n3, double gamma,
+                       double L[25], double R[25])
+{
+    double uref[5], Lm[5][5], Rm[5][5];
+    roe_average(uref, ul, ur, gamma);
+    eigen_matrices(uref, n1, n2, n3, gamma, Lm, Rm);
+    for (int r = 0; r < 5; ++r)
+        for (int q = 0; q < 5; ++q) { L[5 * r + q] = Lm[r][q]; R[5 * r + q] = Rm[r][q]; }
+}
+
 static void pencil_faces_char(const orc_field *F, int dir, int a, int b,
                               double (*Fp)[5], double (*Fm)[5], double *out[5])
 {

@@ -61,6 +61,11 @@ void orc_split_state(int fvs, double gamma, double eps,
  * if the stencil size does not match the scheme (the reference throws). */
 double orc_reconstruct(const double *st, int n, double flag,
                        const ns3d_params *p, int *err);
+/* computeRoeAveragedState_new + build_eigen_matrices_new
+ * (InviscousFlux_new.cpp:532-659) for the face between states ul and ur
+ * (rho, u, v, w, p), normal (n1, n2, n3): L and R row-major 5x5. */
+void orc_char_matrices(const double ul[5], const double ur[5], int n1, int n2, int n3, double gamma,
+                       double L[25], double R[25]);
 /* the central / one-sided derivative of ViscousFlux.cpp:15-40 on the 9-point
  * window f[0..8] centred at f[4]; order in {1,-1,2,4,6,8} */
 double orc_diff(const double *f9, int order, double dx);

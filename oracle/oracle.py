@@ -53,6 +53,7 @@ def load():
                             + [C.c_int] * 3 + [DP, DP]),
         "orc_reconstruct": (C.c_double, [DP, C.c_int, C.c_double, PP, C.POINTER(C.c_int)]),
         "orc_diff": (C.c_double, [DP, C.c_int, C.c_double]),
+        "orc_char_matrices": (None, [DP, DP, C.c_int, C.c_int, C.c_int, C.c_double, DP, DP]),
         "orc_choose_scheme": (C.c_int, [C.c_int] * 5),
         "orc_get_mu": (C.c_double, [PP, C.c_double]),
         "orc_du_inviscous": (None, [C.c_void_p]),

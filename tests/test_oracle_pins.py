@@ -519,6 +519,70 @@ def test_gradient_closures_on_a_cubic(vis, orders):
     assert np.allclose(dudx[R:nx - R], exact[R:nx - R], rtol=0, atol=1e-11)
 
 
+def euler_flux(U, n, gamma=1.4):
+    """Inviscid flux of U = (rho, rho u, rho v, rho w, E) along the unit axis n."""
+    rho, mx, my, mz, E = U
+    u, v, w = mx / rho, my / rho, mz / rho
+    p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v + w * w))
+    un = u * n[0] + v * n[1] + w * n[2]
+    return np.array([rho * un, mx * un + p * n[0], my * un + p * n[1], mz * un + p * n[2], (E + p) * un])
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_char_matrices_diagonalise_the_flux_jacobian(axis):
+    """build_eigen_matrices_new (InviscousFlux_new.cpp:532-625) with its
+    permuted tangential basis: L R = I, and R diag(un-c, un, un, un, un+c) L
+    is the Jacobian of the Euler flux (central differences of a textbook
+    flux) at the state the matrices are built from."""
+    gamma = 1.4
+    rng = np.random.default_rng(40 + axis)
+    n = [1 if d == axis else 0 for d in range(3)]
+    for _ in range(5):
+        w = np.array([rng.uniform(0.5, 2.0), *rng.normal(size=3), rng.uniform(0.5, 3.0)])
+        Lm = np.zeros(25)
+        Rm = np.zeros(25)
+        L.orc_char_matrices(w.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)),
+                            *n, gamma, Lm.ctypes.data_as(C.POINTER(C.c_double)),
+                            Rm.ctypes.data_as(C.POINTER(C.c_double)))
+        Lm, Rm = Lm.reshape(5, 5), Rm.reshape(5, 5)
+        assert np.allclose(Lm @ Rm, np.eye(5), atol=1e-13)
+        rho, u, v, wz, p = w
+        U = np.array([rho, rho * u, rho * v, rho * wz, p / (gamma - 1) + 0.5 * rho * (u * u + v * v + wz * wz)])
+        A = np.zeros((5, 5))
+        for q in range(5):
+            h = 1e-6 * max(1.0, abs(U[q]))
+            e = np.zeros(5)
+            e[q] = h
+            A[:, q] = (euler_flux(U + e, n) - euler_flux(U - e, n)) / (2 * h)
+        un, c = w[1 + axis], np.sqrt(gamma * p / rho)
+        lam = np.diag([un - c, un, un, un, un + c])
+        assert np.allclose(Rm @ lam @ Lm, A, rtol=0, atol=1e-6 * np.abs(A).max())
+
+
+@pytest.mark.parametrize("recon", ["MDCD", "UPWIND_7TH", "UD7L", "LINEAR"])
+@pytest.mark.parametrize("fvs", ["StagerWarming", "LaxFriedrichs"])
+def test_char_recon_of_linear_schemes_equals_componentwise(recon, fvs):
+    """du_inviscous_character projects each face's stencil with L, reconstructs
+    and projects back with R (InviscousFlux_new.cpp:165-437).  A linear
+    reconstruction commutes with the per-face projections and L R = I, so its
+    characteristic faces equal the component-wise faces of du_inviscous."""
+    n = 10
+    rng = np.random.default_rng(7)
+    st = [1.0 + 0.1 * rng.standard_normal((n, n, n)), 0.3 * rng.standard_normal((n, n, n)),
+          0.3 * rng.standard_normal((n, n, n)), 0.3 * rng.standard_normal((n, n, n)),
+          5.0 + 0.5 * rng.standard_normal((n, n, n))]
+    out = {}
+    for ch in (0, 1):
+        p, g = ns.make_params(n, Ma=0.4, recon=recon, fvs_type=fvs, vis_scheme="C8th", char_recon=ch)
+        o = Oracle(p, g)
+        o.set_primitive(*st)
+        o.du_inviscous()
+        out[ch] = [o.face_flux(d) for d in range(3)]
+    for d in range(3):
+        for a, b in zip(out[1][d], out[0][d]):
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), (d, np.abs(a - b).max())
+
+
 def test_timestep_and_energy_closed_forms():
     n = 12
     rho, u, v, w, p = smooth_field(n, 6)
