@@ -519,7 +519,7 @@ def test_gradient_closures_on_a_cubic(vis, orders):
     assert np.allclose(dudx[R:nx - R], exact[R:nx - R], rtol=0, atol=1e-11)
 
 
-def euler_flux(U, n, gamma=1.4):
+def euler_flux_conserved(U, n, gamma=1.4):
     """Inviscid flux of U = (rho, rho u, rho v, rho w, E) along the unit axis n."""
     rho, mx, my, mz, E = U
     u, v, w = mx / rho, my / rho, mz / rho
@@ -553,7 +553,7 @@ def test_char_matrices_diagonalise_the_flux_jacobian(axis):
             h = 1e-6 * max(1.0, abs(U[q]))
             e = np.zeros(5)
             e[q] = h
-            A[:, q] = (euler_flux(U + e, n) - euler_flux(U - e, n)) / (2 * h)
+            A[:, q] = (euler_flux_conserved(U + e, n) - euler_flux_conserved(U - e, n)) / (2 * h)
         un, c = w[1 + axis], np.sqrt(gamma * p / rho)
         lam = np.diag([un - c, un, un, un, un + c])
         assert np.allclose(Rm @ lam @ Lm, A, rtol=0, atol=1e-6 * np.abs(A).max())
