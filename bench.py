@@ -46,7 +46,9 @@ ALG_BYTES = {
         "sweep_y": 60,        # + read A(5)
         "sweep_z": 60,
         "gradflux": 52,       # read u,v,w,T-1 (FP32); write 9 FP32 fluxes
-        "vdiv_stage": 200,    # read 9 FP32 fluxes, A32, U64, U0; write U64, shadow(6)
+        # read 9 FP32 fluxes, A32, U64 (+ dU32 in stages 2, 3); write dU32,
+        # shadow(6) (+ U64 in stage 3): 140, 160, 200 B, mean over a step
+        "vdiv_stage": 500 / 3,
         "cfl": 20,            # read the FP32 shadow
     },
 }
@@ -124,7 +126,7 @@ def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
     traffic = traffic_from_profiles(prec, dom)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
-    step_bytes = 3 * sum(alg[k] for k in alg if k != "cfl") + alg["cfl"]
+    step_bytes = round(3 * sum(alg[k] for k in alg if k != "cfl") + alg["cfl"], 1)
     return {"value": value, "ms_per_step": ms_step, "wall_s_timed": round(wall, 4), "kernels": kernels,
             "roofline": roof,
             "kernel_bytes_per_cell_update": step_bytes,

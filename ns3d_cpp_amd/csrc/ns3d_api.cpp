@@ -339,7 +339,7 @@ int alloc_mixed(ns3d_ctx *c)
     c->m.VF[9] = c->m.VF[6];
     for (int m = 0; m < 5; ++m) {
         c->m.U[m] = c->f.W[m];      // conserved state lives in the W arrays
-        c->m.U0[m] = c->f.U0[m];
+        c->m.dU[m] = reinterpret_cast<float *>(c->f.U0[m]);   // FP32 in the FP64 U0 arrays
         c->m.out[m] = c->f.G[m];    // RHS of ns3d_compute_rhs
     }
     return NS3D_OK;
@@ -985,10 +985,16 @@ int ns3d_residuals(ns3d_ctx *c, double out[5])
     if (!c || !out) return NS3D_ERR_INVALID_ARG;
     int st = ensure_mon_part(c, (long long)monitor_blocks() * 10);
     if (st) return st;
-    double *ptrs[10];
-    for (int m = 0; m < 5; ++m) { ptrs[m] = c->f.W[m]; ptrs[5 + m] = c->f.U0[m]; }
-    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, ptrs, 10 * sizeof(double *), hipMemcpyHostToDevice, c->stream));
-    TIMED(c, "residuals", launch_resid(c->stream, c->dp, c->d_ptrs, c->d_ptrs + 5, c->mixed ? 1 : 0, c->mon_part));
+    // slots 5..9: U0 (FP64 mode) or the mixed mode's FP32 step increment
+    void *ptrs[10];
+    for (int m = 0; m < 5; ++m) {
+        ptrs[m] = c->f.W[m];
+        ptrs[5 + m] = c->mixed ? static_cast<void *>(c->m.dU[m]) : static_cast<void *>(c->f.U0[m]);
+    }
+    HIPCALL(c, hipMemcpyAsync(c->d_ptrs, ptrs, 10 * sizeof(void *), hipMemcpyHostToDevice, c->stream));
+    TIMED(c, "residuals",
+          launch_resid(c->stream, c->dp, c->d_ptrs, c->d_ptrs + 5, reinterpret_cast<const float *const *>(c->d_ptrs + 5),
+                       c->mixed ? 1 : 0, c->mon_part));
     std::vector<double> h((size_t)monitor_blocks() * 10);
     HIPCALL(c, hipMemcpyAsync(h.data(), c->mon_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     st = stream_sync(c);

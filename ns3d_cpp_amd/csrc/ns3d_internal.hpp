@@ -18,8 +18,10 @@ struct Fields {
 
 // Device arrays of the mixed-precision path.
 struct MFields {
-    double *U[5];    // conserved rho, rho u, rho v, rho w, E (FP64)
-    double *U0[5];   // conserved state at the start of the step (FP64)
+    double *U[5];    // conserved rho, rho u, rho v, rho w, E (FP64) at the
+                     // start of the step; the third stage writes the new state
+    float *dU[5];    // RK3 increment of the stage state over U (FP32); after
+                     // the third stage, the whole step's increment
     double *out[5];  // RHS of ns3d_compute_rhs (FP64)
     float *S[6];     // shadow: rho - rho_ref, u, v, w, p - p_ref, T - 1
     float *A[5];     // inviscid accumulator
@@ -120,6 +122,6 @@ hipError_t launch_spec_bins(hipStream_t s, const DevParams &P, const void *U, do
                             double scale, int accumulate);
 hipError_t launch_stats(hipStream_t s, const DevParams &P, const double *const *W, double *partial);
 hipError_t launch_resid(hipStream_t s, const DevParams &P, const double *const *Wc, const double *const *U0,
-                        int conserved, double *partial);
+                        const float *const *dU, int conserved, double *partial);
 
 }  // namespace ns3d

@@ -1,9 +1,10 @@
 // ns3d_mixed.hip -- mixed-precision RHS + RK3 path (NS3D_PREC_MIXED).
 //
 // State and arithmetic split:
-//   FP64  conserved state U and U0, the FVS split, the face fluxes and their
+//   FP64  conserved state U, the FVS split, the face fluxes and their
 //         differences, the RK3 stage update, and p/T recovery from E minus
-//         the kinetic energy (C2P).
+//         the kinetic energy (C2P).  The RK3 stages hold their increment
+//         over U in FP32.
 //   FP32  a shadow of the primitives stored as deviations from the
 //         nondimensional reference state (rho-1, u, v, w, p-p_inf, T-1;
 //         p_inf = 1/(gamma Ma^2)), the reconstructions (weights and smoothness
@@ -604,10 +605,34 @@ __device__ __forceinline__ float diff32_bc(const float *v, float h_inv, int dlo,
     return diff32_closure(v, h_inv, dlo, dhi);
 }
 
+// x and y derivatives of the four shadow quantities at cell id
+template <int VIS>
+__device__ __forceinline__ void gradxy32(const float *const src[4], int id, int sx, float idx, float idy, int dxl,
+                                         int dxh, int dyl, int dyh, float g[8])
+{
+    constexpr int R = VisReach<VIS>::R;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+        for (int o = -R; o <= R; ++o) {
+            wx[o + R] = src[q][id + o];
+            wy[o + R] = src[q][id + o * sx];
+        }
+        g[2 * q + 0] = diff32_bc<VIS>(wx + R, idx, dxl, dxh);
+        g[2 * q + 1] = diff32_bc<VIS>(wy + R, idy, dyl, dyh);
+    }
+}
+
 // gradients + viscous flux in FP32 from the shadow (u, v, w, T-1),
 // z-marching register window as k_gradflux.  The stress tensor is symmetric:
 // VF[4], VF[8], VF[9] (txy, txz, tyz of the y and z fluxes) alias VF[1],
 // VF[2], VF[6], so 9 arrays are written.
+// A plane enters the z window R iterations before it is the centre plane;
+// its x / y derivatives are taken in that same iteration and held in a
+// register delay line, so each plane's neighbours are read while its lines
+// are still in L2 (taking them R planes later re-fetched the plane from HBM:
+// the XCD streams about 1 MB per plane step through its 4 MB L2).
 template <int VIS, int BC>
 __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
 {
@@ -621,36 +646,46 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
     const int k1 = min(k0 + seg_len, P.ng + P.nz);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
-    const float *src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
+    const float *const src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
     const float idx = (float)P.idx, idy = (float)P.idy, idz = (float)P.idz;
-    float win[4][2 * R + 1];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
-    const float cq = (float)(P.Cp / P.Pr);
     int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
     if constexpr (BC) {
         axis_dist(P.periodic[0], i, P.ng, P.nx, R, dxl, dxh);
         axis_dist(P.periodic[1], j, P.ng, P.ny, R, dyl, dyh);
     }
+    float win[4][2 * R + 1];
+    float gxy[R + 1][8];      // slot s: x / y derivatives of plane k + s
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
+#pragma unroll
+    for (int s = 1; s <= R; ++s)
+        if (k0 + s - 1 < k1) gradxy32<VIS>(src, (k0 + s - 1) * sxy + col, P.sx, idx, idy, dxl, dxh, dyl, dyh, gxy[s]);
+    const float cq = (float)(P.Cp / P.Pr);
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         if constexpr (BC) axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
-        float g[12];
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) gxy[s][q] = gxy[s + 1][q];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
 #pragma unroll
             for (int o = 0; o < 2 * R; ++o) win[q][o] = win[q][o + 1];
             win[q][2 * R] = src[q][id + R * sxy];
-            float wx[2 * R + 1], wy[2 * R + 1];
+        }
+        // the skip past the segment end makes the compiler serialise these
+        // loads, but keeps the kernel at 126 VGPRs (4 waves per SIMD); the
+        // unconditional form keeps them in flight at 160 (3 waves) and ran
+        // 10 % slower
+        if (k + R < k1) gradxy32<VIS>(src, id + R * sxy, P.sx, idx, idy, dxl, dxh, dyl, dyh, gxy[R]);
+        float g[12];
 #pragma unroll
-            for (int o = -R; o <= R; ++o) {
-                wx[o + R] = src[q][id + o];
-                wy[o + R] = src[q][id + o * P.sx];
-            }
-            g[3 * q + 0] = diff32_bc<VIS>(wx + R, idx, dxl, dxh);
-            g[3 * q + 1] = diff32_bc<VIS>(wy + R, idy, dyl, dyh);
+        for (int q = 0; q < 4; ++q) {
+            g[3 * q + 0] = gxy[0][2 * q];
+            g[3 * q + 1] = gxy[0][2 * q + 1];
             g[3 * q + 2] = diff32_bc<VIS>(&win[q][R], idz, dzl, dzh);
         }
         const float mu = get_mu32(P, 1.0f + win[3][R]);
@@ -737,19 +772,27 @@ __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double 
 #pragma unroll
             for (int m = 0; m < 5; ++m) M.out[m][id] = r[m];
         } else {
+            // SSP-RK3 (range_kutta.cpp:102-147) in increment form on the
+            // step's initial state U:
+            //   U1 = U + d1,  d1 = dt L(U)
+            //   U2 = 3/4 U + 1/4 (U1 + dt L(U1)) = U + d2,  d2 = (d1 + dt L(U1)) / 4
+            //   U3 = 1/3 U + 2/3 (U2 + dt L(U2)) = U + 2/3 (d2 + dt L(U2))
+            // U stays in FP64 and is read each stage but written only by the
+            // third; the increments are held in FP32 (their rounding is ~6e-8
+            // of one step's change).  The stage state is formed from the
+            // rounded increment, so the shadow matches what the next stage
+            // sees, and the third stage's increment is kept for the residual
+            // monitor (exactly U3 - U up to U3's own rounding).
             const double gamma = P.gamma;
             double Un[5];
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                const double U = M.U[m][id];
-                if constexpr (MODE == 1) {
-                    M.U0[m][id] = U;
-                    Un[m] = U + dt * r[m];
-                } else if constexpr (MODE == 2) {
-                    Un[m] = 0.75 * M.U0[m][id] + 0.25 * (U + dt * r[m]);
-                } else {
-                    Un[m] = (1.0 / 3.0) * M.U0[m][id] + (2.0 / 3.0) * (U + dt * r[m]);
-                }
+                float d;
+                if constexpr (MODE == 1) d = (float)(dt * r[m]);
+                else if constexpr (MODE == 2) d = (float)(0.25 * ((double)M.dU[m][id] + dt * r[m]));
+                else d = (float)((2.0 / 3.0) * ((double)M.dU[m][id] + dt * r[m]));
+                M.dU[m][id] = d;
+                Un[m] = M.U[m][id] + (double)d;
             }
             double rn = Un[0];
             if (rn <= 0.0) rn = 1e-12;
@@ -757,9 +800,11 @@ __global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double 
             const double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
             const double pn = (gamma - 1.0) * rn * (Un[4] / rn - kinetic);
             const double Tn = pn / (rn * P.Rgas);
-            M.U[0][id] = rn;
+            if constexpr (MODE == 3) {
+                M.U[0][id] = rn;
 #pragma unroll
-            for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
+                for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
+            }
             M.S[0][id] = (float)(rn - P.rho_ref);
             M.S[1][id] = (float)un;
             M.S[2][id] = (float)vn;

@@ -117,10 +117,11 @@ __global__ __launch_bounds__(256) void km_stats(DevParams P, const double *const
 
 // Residual sums per block (postprocess.cpp:7-96 after updateResiduals):
 //   0..4  sum (U_m - U0_m)^2 dx^3      5..9  max |U_m|
-// U comes from the conserved arrays (mixed mode) or from P2C of the
-// primitives (FP64 mode, field_structures.h:430-451).
+// FP64 mode: U from P2C of the primitives (field_structures.h:430-451) and
+// U0 the state at the start of the last step.  Mixed mode (conserved != 0):
+// U from the conserved arrays and U - U0 the last step's FP32 increment dU.
 __global__ __launch_bounds__(256) void km_resid(DevParams P, const double *const *Wc, const double *const *U0,
-                                                int conserved, double *partial)
+                                                const float *const *dU, int conserved, double *partial)
 {
     __shared__ double red[10][4];
     const long long n = (long long)P.nx * P.ny * P.nz;
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void km_resid(DevParams P, const double *const
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            const double r = U[m] - U0[m][id];
+            const double r = conserved ? (double)dU[m][id] : U[m] - U0[m][id];
             s[m] += r * r * dx3;
             s[5 + m] = fmax(s[5 + m], fabs(U[m]));
         }
@@ -201,9 +202,9 @@ hipError_t launch_stats(hipStream_t s, const DevParams &P, const double *const *
 }
 
 hipError_t launch_resid(hipStream_t s, const DevParams &P, const double *const *Wc, const double *const *U0,
-                        int conserved, double *partial)
+                        const float *const *dU, int conserved, double *partial)
 {
-    hipLaunchKernelGGL(km_resid, dim3(monitor_blocks()), dim3(256), 0, s, P, Wc, U0, conserved, partial);
+    hipLaunchKernelGGL(km_resid, dim3(monitor_blocks()), dim3(256), 0, s, P, Wc, U0, dU, conserved, partial);
     return hipGetLastError();
 }
 
