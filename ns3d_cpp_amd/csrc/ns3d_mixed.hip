@@ -465,7 +465,9 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
 // and side, the S FP32 differences of the stencil and the FP64 split value of
 // the last cell (to difference the next one); accumulates into A.
 // WENO5 fits 4 waves per SIMD without spilling; the 8-point linear schemes
-// spill there and run at 3
+// spill there and run at 3.  The sweeps are VALU-bound and need the fourth
+// wave: unrolling the march to rotate the window by renaming (no moves, 9 %
+// fewer instructions) needs 165 VGPRs and ran 25 % slower at 3 waves.
 template <int DIR, int RECON, int FVS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RECON == R_WENO5 ? 4 : 3, 8)))
 void km_sweep(DevParams P, MFields M, int seg_len)
@@ -714,8 +716,12 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
 // divergence of the FP32 viscous fluxes (periodic wrap on periodic axes,
 // the closures near the ends of the others), the RHS in FP64, and the stage
 // update + C2P in FP64 (MODE 1..3) or the RHS into `rhs_out` (MODE 0).
+// Occupancy 2: with the register budget of 2 waves per SIMD the compiler
+// keeps the ~80 loads of an iteration in flight; at its default target of 4
+// it waits for them in small groups (0.74 against 0.81 ms per launch at
+// 256^3).
 template <int VIS, int MODE, int BC>
-__global__ __launch_bounds__(256) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
                                                  unsigned long long *flag, unsigned long long step_key)
 {
     constexpr int R = VisReach<VIS>::R;
