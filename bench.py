@@ -7,7 +7,9 @@ rho = 1, T = 1, Mt = 0.3, Re = 200, Pr = 0.72, Sutherland mu(T) with
 S/T_ref = 110.4/273.15; WENO5 + Steger-Warming + C8 central viscous terms;
 SSP-RK3 with the CFL (0.5) time step recomputed every step.
 
-A step = one RK3 step (3 RHS + 3 stage updates + the CFL reduction).
+A step = one RK3 step (3 RHS + 3 stage updates + the CFL reduction), run as
+ns3d_advance_cfl runs them: dt = cfl * min stays on the device, so the timed
+loop has no host round trip per step.
 Metric: cell-updates/s = N^3 / (seconds per step), for the mixed-precision
 path (`value`) and the FP64 path (`fp64`).  Inputs are resident in HBM
 before the timed region.  Per-kernel durations come from HIP events recorded
@@ -89,8 +91,7 @@ def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
     p, g = make(ns, n, Ma, Re, recon, prec)
     s = ns.Solver(p, g)
     s.upload_primitive(*st)
-    for _ in range(warmup):
-        s.rk3_step(s.timestep())
+    s.advance_cfl(warmup)
     s.sync()
     H.hipDeviceSynchronize()
     ev0, ev1 = C.c_void_p(), C.c_void_p()
@@ -99,8 +100,8 @@ def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
     s.profile(True)
     t0 = time.perf_counter()
     H.hipEventRecord(ev0, s.stream())
-    for _ in range(steps):
-        s.rk3_step(s.timestep())
+    # each step: the CFL reduction, dt = cfl * min on the device, 3 stages
+    s.advance_cfl(steps)
     H.hipEventRecord(ev1, s.stream())
     H.hipEventSynchronize(ev1)
     s.sync()

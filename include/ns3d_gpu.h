@@ -173,8 +173,18 @@ int ns3d_compute_rhs(ns3d_ctx *ctx);
  * finds rho, p or T <= 0 or NaN; ns3d_nonphysical_cell gives the cell. */
 int ns3d_rk3_step(ns3d_ctx *ctx, double dt);
 int ns3d_nonphysical_cell(const ns3d_ctx *ctx, int *i, int *j, int *k);
-/* Advance nsteps RK3 steps with fixed dt (no host round trip per step). */
+/* Advance nsteps RK3 steps with fixed dt (no host round trip per step).
+ * With NS3D_GRAPHS=1 in the environment at ns3d_create, the steps after the
+ * first replay a captured HIP graph of one step. */
 int ns3d_advance(ns3d_ctx *ctx, int nsteps, double dt);
+/* The time loop of main/maincycle (maincycle.cpp:18-107) without output
+ * stops or dt clipping: nsteps times dt = compute_timestep (timestep.cpp:
+ * 5-33, single rank), runge_kutta_3, t += dt, with dt kept on the device (no
+ * host round trip per step; graph replay as ns3d_advance).  *time: t in, t
+ * after the steps out (the same sums as the host loop).  The same dt and
+ * states as ns3d_compute_timestep_local + ns3d_rk3_step per step, bit for
+ * bit; NS3D_ERR_NONPHYSICAL as ns3d_rk3_step, reported after the last step. */
+int ns3d_advance_cfl(ns3d_ctx *ctx, int nsteps, double *time);
 /* compute_timestep (timestep.cpp:5-33) without the MPI_Allreduce: returns
  * cfl * min over interior cells; the caller reduces across ranks. */
 int ns3d_compute_timestep_local(ns3d_ctx *ctx, double *dt);

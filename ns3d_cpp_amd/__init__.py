@@ -233,6 +233,13 @@ class Solver:
     def advance(self, nsteps, dt):
         self._check(self._L.ns3d_advance(self.ctx, int(nsteps), float(dt)), "advance")
 
+    def advance_cfl(self, nsteps, time=0.0):
+        """nsteps RK3 steps, each with the CFL time step of its initial state
+        computed on the device; returns the time after them."""
+        t = C.c_double(time)
+        self._check(self._L.ns3d_advance_cfl(self.ctx, int(nsteps), C.byref(t)), "advance_cfl")
+        return t.value
+
     def nonphysical_cell(self):
         i, j, k = C.c_int(), C.c_int(), C.c_int()
         self._L.ns3d_nonphysical_cell(self.ctx, C.byref(i), C.byref(j), C.byref(k))

@@ -91,6 +91,7 @@ SIGNATURES = {
     "ns3d_rk3_step": (C.c_int, [C.c_void_p, C.c_double]),
     "ns3d_nonphysical_cell": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 3),
     "ns3d_advance": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "ns3d_advance_cfl": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "ns3d_compute_timestep_local": (C.c_int, [C.c_void_p, DP]),
     "ns3d_total_energy": (C.c_int, [C.c_void_p, DP]),
     "ns3d_sync": (C.c_int, [C.c_void_p]),

@@ -21,6 +21,19 @@
 
 using namespace ns3d;
 
+// Launch configuration a captured step depends on: kernel arguments and
+// grid shapes.  A graph is replayed only while this is unchanged.
+struct StepSig {
+    DevParams dp;
+    Fields f;
+    MFields m;
+    FaceArrays faces;
+    int mixed, char_recon, seg_len[3], seg_z, gradflux_minb, sweep_minw, m_gf_block[2], m_vd_block[2];
+    unsigned long long *d_flag;
+    double dt;
+    int dev_dt;
+};
+
 struct ns3d_ctx {
     ns3d_params P{};
     ns3d_grid G{};
@@ -32,6 +45,12 @@ struct ns3d_ctx {
     FaceArrays faces{};          // allocated on first ns3d_du_inviscous
     int device = 0;
     hipStream_t stream = nullptr;
+    // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
+    // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
+    // faster than eager launches (the kernels outlast the launch cost)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    StepSig graph_sig[2];
+    bool use_graphs = false;
     double *slab = nullptr;      // one allocation for all cell arrays
     double *face_slab = nullptr;
     unsigned long long *d_flag = nullptr;   // [0] non-physical key, [1] CFL bits
@@ -235,7 +254,7 @@ int inviscid(ns3d_ctx *c)
 }
 
 // one RHS evaluation of the mixed-precision path
-int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
+int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
 {
     static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
     if (c->P.char_recon) {
@@ -255,36 +274,129 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt)
         for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
     }
     TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1]));
-    const unsigned long long key = (unsigned long long)c->steps_done << 40;
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
-          launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, key, c->m_vd_block[0],
+          launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, dev_dt, c->m_vd_block[0],
                         c->m_vd_block[1]));
     return NS3D_OK;
 }
 
-// one RHS evaluation finished by k_vdiv in the given mode
-int rhs_pass(ns3d_ctx *c, int mode, double dt)
+// one RHS evaluation finished by k_vdiv in the given mode; with dev_dt the
+// stage kernels take dt from the device CFL reduction
+int rhs_pass(ns3d_ctx *c, int mode, double dt, int dev_dt = 0)
 {
-    if (c->mixed) return rhs_pass_mixed(c, mode, dt);
+    if (c->mixed) return rhs_pass_mixed(c, mode, dt, dev_dt);
     int st = inviscid(c);
     if (st) return st;
     TIMED(c, "gradflux", launch_gradflux(c->stream, c->dp, c->f, c->seg_z, c->gradflux_minb));
-    const unsigned long long key = (unsigned long long)c->steps_done << 40;
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
-          launch_vdiv_z(c->stream, c->dp, c->f, mode, dt, c->seg_z, c->d_flag, key));
+          launch_vdiv_z(c->stream, c->dp, c->f, mode, dt, c->seg_z, c->d_flag, dev_dt));
     return NS3D_OK;
 }
 
-// one SSP-RK3 step, range_kutta.cpp:102-147 (U0 recorded by stage 1)
-int rk3_enqueue(ns3d_ctx *c, double dt)
+// the end of a step: the device step counter (and simulated time) and its
+// host mirror
+int step_end(ns3d_ctx *c, int dev_dt)
 {
+    HIPCALL(c, launch_step_end(c->stream, c->d_flag, c->dp.cfl, dev_dt));
+    c->steps_done++;
+    return NS3D_OK;
+}
+
+// CFL reduction into d_flag[1] (compute_timestep, timestep.cpp:8-36)
+int cfl_enqueue(ns3d_ctx *c)
+{
+    double init = 1e9;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, sizeof(bits));
+    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 1, bits, 1));
+    if (c->mixed) TIMED(c, "cfl", launch_m_cfl(c->stream, c->dp, c->m, c->d_flag + 1));
+    else TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
+    return NS3D_OK;
+}
+
+// one SSP-RK3 step, range_kutta.cpp:102-147 (U0 recorded by stage 1); with
+// dev_dt the step's dt is cfl * the CFL reduction of its initial state,
+// taken on the device
+int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0)
+{
+    if (dev_dt) {
+        int st = cfl_enqueue(c);
+        if (st) return st;
+    }
     for (int stage = 1; stage <= 3; ++stage) {
-        int st = rhs_pass(c, stage, dt);
+        int st = rhs_pass(c, stage, dt, dev_dt);
         if (st) return st;
         st = fill_ghosts(c);
         if (st) return st;
     }
+    return step_end(c, dev_dt);
+}
+
+void step_sig(const ns3d_ctx *c, double dt, int dev_dt, StepSig &g)
+{
+    std::memset(&g, 0, sizeof(g));   // padding too: compared bytewise
+    g.dp = c->dp;
+    g.f = c->f;
+    g.m = c->m;
+    g.faces = c->faces;
+    g.mixed = c->mixed;
+    g.char_recon = c->P.char_recon;
+    for (int d = 0; d < 3; ++d) g.seg_len[d] = c->seg_len[d];
+    g.seg_z = c->seg_z;
+    g.gradflux_minb = c->gradflux_minb;
+    g.sweep_minw = c->sweep_minw;
+    for (int d = 0; d < 2; ++d) { g.m_gf_block[d] = c->m_gf_block[d]; g.m_vd_block[d] = c->m_vd_block[d]; }
+    g.d_flag = c->d_flag;
+    g.dt = dev_dt ? 0.0 : dt;
+    g.dev_dt = dev_dt;
+}
+
+// One RK3 step from a captured HIP graph: fixed-dt and device-dt steps each
+// keep one graph, captured on first use and again whenever the launch
+// configuration or the fixed dt changes.  The step holds no memset and no
+// allocation (the first step of a call runs eagerly and makes them).
+int step_graph(ns3d_ctx *c, double dt, int dev_dt)
+{
+    const int k = dev_dt ? 1 : 0;
+    StepSig want;
+    step_sig(c, dt, dev_dt, want);
+    if (!c->graph[k] || std::memcmp(&want, &c->graph_sig[k], sizeof(StepSig)) != 0) {
+        if (c->graph[k]) {
+            HIPCALL(c, hipGraphExecDestroy(c->graph[k]));
+            c->graph[k] = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const long long steps = c->steps_done;
+        const int st = rk3_enqueue(c, dt, dev_dt);
+        c->steps_done = steps;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (st) {
+            if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        const hipError_t ei = hipGraphInstantiate(&c->graph[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->graph[k] = nullptr;
+            return hip_fail(c, ei, "hipGraphInstantiate");
+        }
+        std::memcpy(&c->graph_sig[k], &want, sizeof(StepSig));
+    }
+    HIPCALL(c, hipGraphLaunch(c->graph[k], c->stream));
     c->steps_done++;
+    return NS3D_OK;
+}
+
+// nsteps RK3 steps: the first eagerly, the rest replayed from the step graph
+// (eagerly throughout while per-kernel profiling is on or graphs are off)
+int advance_steps(ns3d_ctx *c, int nsteps, double dt, int dev_dt)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        const int st = (s == 0 || c->prof || !c->use_graphs) ? rk3_enqueue(c, dt, dev_dt) : step_graph(c, dt, dev_dt);
+        if (st) return st;
+    }
     return NS3D_OK;
 }
 
@@ -573,6 +685,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.mdcd_disp = p.mdcd_disp; d.mdcd_diss = p.mdcd_diss; d.ud7l_alpha = p.ud7l_alpha;
     d.dx = g.dx; d.dy = g.dy; d.dz = g.dz;
     d.idx = 1.0 / g.dx; d.idy = 1.0 / g.dy; d.idz = 1.0 / g.dz;   // range_kutta.cpp:14-16
+    d.cfl = p.cfl;
     d.fvs = p.fvs_type; d.recon = p.recon; d.vis = p.vis_scheme; d.mu_law = p.mu_law;
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ng = g.ng;
     d.sx = (int)sx; d.sy = (int)sy; d.sz = (int)sz;
@@ -649,7 +762,9 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         int v = std::atoi(env);
         if (v > 0) for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = v;
     }
-    e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 4);
+    if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
+    e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 2);
+    if (e == hipSuccess) e = launch_fill_u64(c->stream, c->d_flag + 2, 0ull, 2);   // step counter, time
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { int r = hip_fail(c, e, "init"); ns3d_destroy(c); return r; }
     *out = c;
@@ -676,6 +791,8 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->mon_part) (void)hipFree(c->mon_part);
     for (auto &q : c->pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return NS3D_OK;
@@ -918,7 +1035,7 @@ int ns3d_compute_vis_flux(ns3d_ctx *c)
     // rhs = 0, then the viscous divergence of the stored fluxes
     for (int m = 0; m < 5; ++m)   // one fill per array: each holds < 2^31 cells
         HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[m], 0ull, (int)c->ncell));
-    HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0ull));
+    HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0));
     return stream_sync(c);
 }
 
@@ -1032,11 +1149,25 @@ int ns3d_advance(ns3d_ctx *c, int nsteps, double dt)
 {
     if (!c || nsteps < 0) return NS3D_ERR_INVALID_ARG;
     c->fw_valid = false;
-    for (int s = 0; s < nsteps; ++s) {
-        int st = rk3_enqueue(c, dt);
-        if (st) return st;
-    }
+    int st = advance_steps(c, nsteps, dt, 0);
+    if (st) return st;
     return check_flag(c);
+}
+
+int ns3d_advance_cfl(ns3d_ctx *c, int nsteps, double *time)
+{
+    if (!c || nsteps < 0 || !time) return NS3D_ERR_INVALID_ARG;
+    c->fw_valid = false;
+    unsigned long long bits;
+    std::memcpy(&bits, time, sizeof(bits));
+    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 3, bits, 1));
+    int st = advance_steps(c, nsteps, 0.0, 1);
+    if (st) return st;
+    HIPCALL(c, hipMemcpyAsync(&bits, c->d_flag + 3, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    st = check_flag(c);   // synchronises the stream
+    if (st && st != NS3D_ERR_NONPHYSICAL) return st;
+    std::memcpy(time, &bits, sizeof(bits));
+    return st;
 }
 
 int ns3d_nonphysical_cell(const ns3d_ctx *c, int *i, int *j, int *k)
@@ -1059,15 +1190,11 @@ int ns3d_nonphysical_cell(const ns3d_ctx *c, int *i, int *j, int *k)
 int ns3d_compute_timestep_local(ns3d_ctx *c, double *dt)
 {
     if (!c || !dt) return NS3D_ERR_INVALID_ARG;
-    double init = 1e9;
-    unsigned long long bits;
-    std::memcpy(&bits, &init, sizeof(bits));
-    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 1, bits, 1));
-    if (c->mixed) TIMED(c, "cfl", launch_m_cfl(c->stream, c->dp, c->m, c->d_flag + 1));
-    else TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
+    int st = cfl_enqueue(c);
+    if (st) return st;
     unsigned long long h = 0;
     HIPCALL(c, hipMemcpyAsync(&h, c->d_flag + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    int st = stream_sync(c);
+    st = stream_sync(c);
     if (st) return st;
     double m;
     std::memcpy(&m, &h, sizeof(m));
@@ -1252,7 +1379,8 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
                 if (!st) st = fill_ghosts(c);
                 if (st) return st;
             }
-            c->steps_done++;
+            st = step_end(c, 0);
+            if (st) return st;
         } else {
             st = rk3_enqueue(c, dt);
             if (st) return st;

@@ -20,7 +20,25 @@ struct DevParams {
     int bc[6];               // ns3d_bc_type of xmin, xmax, ymin, ymax, zmin, zmax
     double rho_ref, p_ref;   // reference state of the mixed-precision shadow
     int adj_lines;           // 1: adjoint sweeps may hold whole lines in LDS
+    double cfl;              // CFL number of the device time step
 };
+
+// Slots of a context's device flag words: [0] key of the first non-physical
+// cell, [1] the CFL minimum (bits of a positive double), [2] the RK3 step
+// counter, [3] the simulated time of the device-time-step loop.
+
+// dt = cfl * min over cells (timestep.cpp:32), from the CFL reduction
+__device__ __forceinline__ double cfl_dt(const DevParams &P, const unsigned long long *flag)
+{
+    return P.cfl * __longlong_as_double((long long)flag[1]);
+}
+
+// key of a non-physical cell: the step counter in the high bits, so the
+// atomicMin keeps the earliest step's lowest cell index
+__device__ __forceinline__ unsigned long long nonphysical_key(const unsigned long long *flag, int id)
+{
+    return (flag[2] << 40) | (unsigned long long)id;
+}
 
 enum { FVS_SW = 0, FVS_LF = 1 };
 enum { R_WENO5 = 0, R_LINEAR = 2, R_MDCD = 3, R_MDCD_HYBRID = 4, R_UD7 = 5, R_UD7L = 6 };

@@ -65,6 +65,7 @@ struct Prims64 {
 };
 
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+hipError_t launch_step_end(hipStream_t s, unsigned long long *flag, double cfl, int dev_dt);
 hipError_t launch_prim_setup(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_c2p_all(hipStream_t s, const DevParams &P, const Fields &F, const double *const *src);
 hipError_t launch_p2c_all(hipStream_t s, const DevParams &P, const Fields &F, double *const *dst);
@@ -74,7 +75,7 @@ bool xrow_supported(const DevParams &P);
 hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb);
 hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key);
+                         unsigned long long *flag, int dev_dt);
 hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &out,
                         int dir, int char_recon);
 hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &ff);
@@ -97,7 +98,7 @@ bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key, int bx = 64, int by = 4);
+                         unsigned long long *flag, int dev_dt, int bx = 64, int by = 4);
 hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double *partial);
 
 // tangent-linear / adjoint RHS (ns3d_adjoint.hip)

@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void k_vflux(DevParams P, Fields F)
 // stage 1; stage 3 flags non-physical cells (range_kutta.cpp:150-161).
 template <int VIS, int MODE, int BC>
 __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt, int seg_len,
-                                                unsigned long long *flag, unsigned long long step_key)
+                                                unsigned long long *flag, int dev_dt)
 {
     constexpr int R = VisReach<VIS>::R;
     const int ng = P.ng;
@@ -606,6 +606,7 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
     if (i >= ng + P.nx || j >= ng + P.ny) return;
     const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
+    if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
     int oxc[2 * R + 1], oyc[2 * R + 1];
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
             F.W[5][id] = Tn;
             if constexpr (MODE == 3) {
                 if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                    atomicMin(flag, step_key | (unsigned long long)id);
+                    atomicMin(flag, nonphysical_key(flag, id));
             }
         }
     }
@@ -831,6 +832,25 @@ static inline dim3 cell_grid(const DevParams &P)
     return dim3((P.nx + 63) / 64, (P.ny + 3) / 4, P.nz);
 }
 static const dim3 kCellBlock(64, 4, 1);
+
+// end of an RK3 step on the device: advance the step counter of the
+// non-physical key and, for the device time step, the simulated time
+// t += cfl * min (maincycle.cpp's t += dt, in the same order)
+__global__ void k_step_end(unsigned long long *flag, double cfl, int dev_dt)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    flag[2] = flag[2] + 1ull;
+    if (dev_dt) {
+        const double t = __longlong_as_double((long long)flag[3]) + cfl * __longlong_as_double((long long)flag[1]);
+        flag[3] = (unsigned long long)__double_as_longlong(t);
+    }
+}
+
+hipError_t launch_step_end(hipStream_t s, unsigned long long *flag, double cfl, int dev_dt)
+{
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, flag, cfl, dev_dt);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n)
 {
@@ -977,25 +997,25 @@ hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, i
 
 template <int VIS, int BC>
 static void vdivz_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                       unsigned long long *flag, unsigned long long key)
+                       unsigned long long *flag, int dev_dt)
 {
     dim3 g((P.nx + 63) / 64, (P.ny + 3) / 4, (P.nz + seg - 1) / seg);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_vdiv_z<VIS, 0, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    case 1: hipLaunchKernelGGL((k_vdiv_z<VIS, 1, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    case 2: hipLaunchKernelGGL((k_vdiv_z<VIS, 2, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
-    default: hipLaunchKernelGGL((k_vdiv_z<VIS, 3, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, key); break;
+    case 0: hipLaunchKernelGGL((k_vdiv_z<VIS, 0, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((k_vdiv_z<VIS, 1, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((k_vdiv_z<VIS, 2, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((k_vdiv_z<VIS, 3, BC>), g, kCellBlock, 0, s, P, F, dt, seg, flag, dev_dt); break;
     }
 }
 
 template <int BC>
 static hipError_t vdivz_vis(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                            unsigned long long *flag, unsigned long long key)
+                            unsigned long long *flag, int dev_dt)
 {
     switch (P.vis) {
-    case VIS_C8: vdivz_mode<VIS_C8, BC>(s, P, F, mode, dt, seg, flag, key); break;
-    case VIS_C6: vdivz_mode<VIS_C6, BC>(s, P, F, mode, dt, seg, flag, key); break;
-    case VIS_C4: vdivz_mode<VIS_C4, BC>(s, P, F, mode, dt, seg, flag, key); break;
+    case VIS_C8: vdivz_mode<VIS_C8, BC>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+    case VIS_C6: vdivz_mode<VIS_C6, BC>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+    case VIS_C4: vdivz_mode<VIS_C4, BC>(s, P, F, mode, dt, seg, flag, dev_dt); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1003,10 +1023,10 @@ static hipError_t vdivz_vis(hipStream_t s, const DevParams &P, const Fields &F, 
 
 // the closure-aware variant only where an axis is not periodic
 hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key)
+                         unsigned long long *flag, int dev_dt)
 {
-    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, key);
-    return vdivz_vis<1>(s, P, F, mode, dt, seg, flag, key);
+    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, dev_dt);
+    return vdivz_vis<1>(s, P, F, mode, dt, seg, flag, dev_dt);
 }
 
 using FaceFn = void (*)(DevParams, Fields, FaceArrays);

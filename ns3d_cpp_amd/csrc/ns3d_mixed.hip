@@ -722,7 +722,7 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
 // 256^3).
 template <int VIS, int MODE, int BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
-                                                 unsigned long long *flag, unsigned long long step_key)
+                                                 unsigned long long *flag, int dev_dt)
 {
     constexpr int R = VisReach<VIS>::R;
     const int ng = P.ng;
@@ -733,6 +733,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (i >= ng + P.nx || j >= ng + P.ny) return;
     const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
+    if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
     int oxc[2 * R + 1], oyc[2 * R + 1];
@@ -819,7 +820,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             M.S[5][id] = (float)(Tn - 1.0);
             if constexpr (MODE == 3) {
                 if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                    atomicMin(flag, step_key | (unsigned long long)id);
+                    atomicMin(flag, nonphysical_key(flag, id));
             }
         }
     }
@@ -1149,38 +1150,38 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
 
 template <int VIS, int BC>
 static void m_vdiv_mode(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                        unsigned long long *flag, unsigned long long key, int bx, int by)
+                        unsigned long long *flag, int dev_dt, int bx, int by)
 {
     dim3 g((P.nx + bx - 1) / bx, (P.ny + by - 1) / by, (P.nz + seg - 1) / seg);
     dim3 b(bx, by, 1);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    case 2: hipLaunchKernelGGL((km_vdiv_z<VIS, 2, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
-    default: hipLaunchKernelGGL((km_vdiv_z<VIS, 3, BC>), g, b, 0, s, P, M, dt, seg, flag, key); break;
+    case 0: hipLaunchKernelGGL((km_vdiv_z<VIS, 0, BC>), g, b, 0, s, P, M, dt, seg, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_z<VIS, 1, BC>), g, b, 0, s, P, M, dt, seg, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_z<VIS, 2, BC>), g, b, 0, s, P, M, dt, seg, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((km_vdiv_z<VIS, 3, BC>), g, b, 0, s, P, M, dt, seg, flag, dev_dt); break;
     }
 }
 
 template <int BC>
 static hipError_t m_vdiv_vis(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                             unsigned long long *flag, unsigned long long key, int bx, int by)
+                             unsigned long long *flag, int dev_dt, int bx, int by)
 {
     switch (P.vis) {
-    case VIS_C8: m_vdiv_mode<VIS_C8, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
-    case VIS_C6: m_vdiv_mode<VIS_C6, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
-    case VIS_C4: m_vdiv_mode<VIS_C4, BC>(s, P, M, mode, dt, seg, flag, key, bx, by); break;
+    case VIS_C8: m_vdiv_mode<VIS_C8, BC>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by); break;
+    case VIS_C6: m_vdiv_mode<VIS_C6, BC>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by); break;
+    case VIS_C4: m_vdiv_mode<VIS_C4, BC>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, unsigned long long key, int bx, int by)
+                         unsigned long long *flag, int dev_dt, int bx, int by)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
     if (P.periodic[0] && P.periodic[1] && P.periodic[2])
-        return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, key, bx, by);
-    return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, key, bx, by);
+        return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);
+    return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);
 }
 
 }  // namespace ns3d
