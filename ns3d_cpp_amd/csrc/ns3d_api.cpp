@@ -362,6 +362,9 @@ int step_graph(ns3d_ctx *c, double dt, int dev_dt)
     step_sig(c, dt, dev_dt, want);
     if (!c->graph[k] || std::memcmp(&want, &c->graph_sig[k], sizeof(StepSig)) != 0) {
         if (c->graph[k]) {
+            // the old graph's last launch may still be running
+            const int ss = stream_sync(c);
+            if (ss) return ss;
             HIPCALL(c, hipGraphExecDestroy(c->graph[k]));
             c->graph[k] = nullptr;
         }
