@@ -462,22 +462,34 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
 }
 
 // y / z sweeps: one thread marches a column segment keeping, per component
-// and side, the S FP32 differences of the stencil and the FP64 split value of
-// the last cell (to difference the next one); accumulates into A.
-// WENO5 fits 4 waves per SIMD without spilling; the 8-point linear schemes
-// spill there and run at 3.  The sweeps are VALU-bound and need the fourth
-// wave: unrolling the march to rotate the window by renaming (no moves, 9 %
-// fewer instructions) needs 165 VGPRs and ran 25 % slower at 3 waves.
+// and side, the S - T0 FP32 differences of the stencil window and the FP64
+// split value of the last cell (to difference the next one); accumulates
+// into A.  The plus side's window is in registers; the minus side's is a
+// per-lane ring in LDS (window position t of face n in slot (t - T0 + n) mod
+// L), which rotates by moving its base rather than by register moves and
+// leaves room for 4 waves per SIMD (the 8-point schemes spilled at 4 with
+// both sides in registers and ran at 3: their sweeps are 32 % faster this
+// way, WENO5's 3.5 %).  Each lane touches only its own column of the ring:
+// no barrier.  Measured slower: both sides in the ring (12.8 KB per wave
+// caps the CU at 3 waves per SIMD), forcing 5 waves (spills), and unrolling
+// the march to rotate registers by renaming (165 VGPRs, 3 waves): the sweeps
+// are VALU-bound and need the fourth wave.  The code's shape (the push / get
+// helpers) is the one measured: the compiler's register use follows it
+// (127 VGPRs for WENO5; a rewrite without them took 172, or spilled when
+// held to 128).
 template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RECON == R_WENO5 ? 4 : 3, 8)))
-void km_sweep(DevParams P, MFields M, int seg_len)
+__global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_len)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
-    constexpr int T0 = S == 2 ? 0 : 1;       // lowest window slot that is read
+    constexpr int T0 = S == 2 ? 0 : 1;       // lowest window position that is read
+    constexpr int L = S - T0;
+    constexpr int R0 = 5;                     // components 5..9 (minus side) are in the ring
+    __shared__ float ring[L][5][64];
     int bs, bt, bb;
     xcd_block3(bs, bt, bb);   // grid (nseg, lane tiles, b), as the launch
-    const int a = bt * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int a = bt * blockDim.x + lane;
     if (a >= P.nx) return;
     const int b = bb;
     const int nd = DIR == 1 ? P.ny : P.nz;
@@ -490,7 +502,14 @@ void km_sweep(DevParams P, MFields M, int seg_len)
     const float inv_d = (float)(DIR == 1 ? P.idy : P.idz);
     float e[S];
     diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);
-    float dw[10][S];          // window slot t <-> d(f - MID + t)
+    float wp[5][S];                            // plus side: wp[m][t], positions T0..S-1
+    int n0 = L - 1;                            // face index mod L; face -1 here
+    const auto slot = [&n0](int t) { const int q = t - T0 + n0; return q >= L ? q - L : q; };
+    // window value of component m (0..9) at position t
+    const auto get = [&](int m, int t) -> float {
+        if (m >= R0) return ring[slot(t)][m - R0][lane];
+        return wp[m][t];
+    };
     double prev[10];
     {
         double rho, u, v, w, p, Fp[5], Fm[5];
@@ -499,17 +518,39 @@ void km_sweep(DevParams P, MFields M, int seg_len)
 #pragma unroll
         for (int m = 0; m < 5; ++m) { prev[m] = Fp[m]; prev[5 + m] = Fm[m]; }
     }
+    // the newest position's values: the plus side after a shift, the minus
+    // side into the slot of position S - 1
+    const auto push = [&](const double *Fp, const double *Fm) {
+#pragma unroll
+        for (int t = T0; t < S - 1; ++t)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) wp[m][t] = wp[m][t + 1];
+        const int q = slot(S - 1);
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
+            const float d = (float)(Fv - prev[m]);
+            if (m >= R0) ring[q][m - R0][lane] = d;
+            else wp[m][S - 1] = d;
+            prev[m] = Fv;
+        }
+    };
+    // positions T0 + 1 .. S - 1 as of face -1
 #pragma unroll
     for (int t = T0 + 1; t < S; ++t) {
         double rho, u, v, w, p, Fp[5], Fm[5];
         shadow_prims(P, M, base + (c0 - 2 - MID + t) * st, rho, u, v, w, p);
         split_fast<FVS, DIR>(P, rho, u, v, w, p, Fp, Fm);
+        {
+            const int q = t - T0 - 1;   // slot(t) at face -1
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            dw[m][t] = (float)(Fp[m] - prev[m]);
-            dw[5 + m][t] = (float)(Fm[m] - prev[5 + m]);
-            prev[m] = Fp[m];
-            prev[5 + m] = Fm[m];
+            for (int m = 0; m < 5; ++m) wp[m][t] = (float)(Fp[m] - prev[m]);
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+                const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
+                if (m >= R0) ring[q][m - R0][lane] = (float)(Fv - prev[m]);
+                prev[m] = Fv;
+            }
         }
     }
     float nx_[5];
@@ -520,10 +561,7 @@ void km_sweep(DevParams P, MFields M, int seg_len)
     }
     float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int f = c0 - 1; f < c1; ++f) {
-#pragma unroll
-        for (int t = T0; t < S - 1; ++t)
-#pragma unroll
-            for (int m = 0; m < 10; ++m) dw[m][t] = dw[m][t + 1];
+        n0 = n0 + 1 == L ? 0 : n0 + 1;
         {
             float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
             if (f + 1 < c1) {
@@ -534,13 +572,7 @@ void km_sweep(DevParams P, MFields M, int seg_len)
             double Fp[5], Fm[5];
             split_fast<FVS, DIR>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
                                  P.p_ref + (double)cur[4], Fp, Fm);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                dw[m][S - 1] = (float)(Fp[m] - prev[m]);
-                dw[5 + m][S - 1] = (float)(Fm[m] - prev[5 + m]);
-                prev[m] = Fp[m];
-                prev[5 + m] = Fm[m];
-            }
+            push(Fp, Fm);
         }
         float rel[5];
 #pragma unroll
@@ -548,8 +580,8 @@ void km_sweep(DevParams P, MFields M, int seg_len)
             float D[S], Dm[S];
 #pragma unroll
             for (int t = 1; t < S; ++t) {
-                D[t] = dw[m][t];
-                Dm[t] = -dw[5 + m][S - t];
+                D[t] = get(m, t);
+                Dm[t] = -get(5 + m, S - t);
             }
             rel[m] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
         }
@@ -557,7 +589,7 @@ void km_sweep(DevParams P, MFields M, int seg_len)
             const int idc = base + f * st;
 #pragma unroll
             for (int m = 0; m < 5; ++m)
-                M.A[m][idc] += (dw[m][MID] + dw[5 + m][MID + 1] + (rel[m] - relprev[m])) * inv_d;
+                M.A[m][idc] += (get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m])) * inv_d;
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
