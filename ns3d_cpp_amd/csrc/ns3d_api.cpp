@@ -314,7 +314,8 @@ int cfl_enqueue(ns3d_ctx *c)
     return NS3D_OK;
 }
 
-// one SSP-RK3 step, range_kutta.cpp:102-147 (U0 recorded by stage 1); with
+// one SSP-RK3 step, range_kutta.cpp:102-147 (FP64: U0 recorded by stage 1;
+// mixed: the stages' FP32 increments over the step's initial state); with
 // dev_dt the step's dt is cfl * the CFL reduction of its initial state,
 // taken on the device
 int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0)

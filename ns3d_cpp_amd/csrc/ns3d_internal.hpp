@@ -10,7 +10,8 @@ namespace ns3d {
 // Device arrays of one context (all sx*sy*sz doubles, idx3 layout).
 struct Fields {
     double *W[6];    // rho, u, v, w, p, T
-    double *U0[5];   // conserved state at the start of the step
+    double *U0[5];   // conserved state at the start of the step (the mixed
+                     // mode keeps its FP32 stage increments MFields::dU here)
     double *A[5];    // inviscid accumulator, then RHS
     double *G[12];   // du_dx du_dy du_dz dv_* dw_* dT_*
     double *VF[12];  // Fv(momx,momy,momz,E) Hv(...) Gv(...)
