@@ -751,6 +751,62 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
     }
 }
 
+// the end of a cell's divergence pass: the RHS into `out` (MODE 0), or the
+// RK3 stage update + C2P into U / dU / the shadow (MODE 1..3)
+template <int MODE>
+__device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M, int id, const double r[5],
+                                            double dt, unsigned long long *flag)
+{
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) M.out[m][id] = r[m];
+    } else {
+        // SSP-RK3 (range_kutta.cpp:102-147) in increment form on the
+        // step's initial state U:
+        //   U1 = U + d1,  d1 = dt L(U)
+        //   U2 = 3/4 U + 1/4 (U1 + dt L(U1)) = U + d2,  d2 = (d1 + dt L(U1)) / 4
+        //   U3 = 1/3 U + 2/3 (U2 + dt L(U2)) = U + 2/3 (d2 + dt L(U2))
+        // U stays in FP64 and is read each stage but written only by the
+        // third; the increments are held in FP32 (their rounding is ~6e-8
+        // of one step's change).  The stage state is formed from the
+        // rounded increment, so the shadow matches what the next stage
+        // sees, and the third stage's increment is kept for the residual
+        // monitor (exactly U3 - U up to U3's own rounding).
+        const double gamma = P.gamma;
+        double Un[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            float d;
+            if constexpr (MODE == 1) d = (float)(dt * r[m]);
+            else if constexpr (MODE == 2) d = (float)(0.25 * ((double)M.dU[m][id] + dt * r[m]));
+            else d = (float)((2.0 / 3.0) * ((double)M.dU[m][id] + dt * r[m]));
+            M.dU[m][id] = d;
+            Un[m] = M.U[m][id] + (double)d;
+        }
+        double rn = Un[0];
+        if (rn <= 0.0) rn = 1e-12;
+        const double un = Un[1] / rn, vn = Un[2] / rn, wn = Un[3] / rn;
+        const double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+        const double pn = (gamma - 1.0) * rn * (Un[4] / rn - kinetic);
+        const double Tn = pn / (rn * P.Rgas);
+        if constexpr (MODE == 3) {
+            M.U[0][id] = rn;
+#pragma unroll
+            for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
+        }
+        M.S[0][id] = (float)(rn - P.rho_ref);
+        M.S[1][id] = (float)un;
+        M.S[2][id] = (float)vn;
+        M.S[3][id] = (float)wn;
+        M.S[4][id] = (float)(pn - P.p_ref);
+        M.S[5][id] = (float)(Tn - 1.0);
+        if constexpr (MODE == 3) {
+            if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
+                atomicMin(flag, nonphysical_key(flag, id));
+        }
+    }
+}
+
 // divergence of the FP32 viscous fluxes (periodic wrap on periodic axes,
 // the closures near the ends of the others), the RHS in FP64, and the stage
 // update + C2P in FP64 (MODE 1..3) or the RHS into `rhs_out` (MODE 0).
@@ -813,54 +869,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                              diff32_bc<VIS>(&win[m][R], idz, dzl, dzh);
             r[m + 1] = (double)vd - (double)M.A[m + 1][id];
         }
-        if constexpr (MODE == 0) {
-#pragma unroll
-            for (int m = 0; m < 5; ++m) M.out[m][id] = r[m];
-        } else {
-            // SSP-RK3 (range_kutta.cpp:102-147) in increment form on the
-            // step's initial state U:
-            //   U1 = U + d1,  d1 = dt L(U)
-            //   U2 = 3/4 U + 1/4 (U1 + dt L(U1)) = U + d2,  d2 = (d1 + dt L(U1)) / 4
-            //   U3 = 1/3 U + 2/3 (U2 + dt L(U2)) = U + 2/3 (d2 + dt L(U2))
-            // U stays in FP64 and is read each stage but written only by the
-            // third; the increments are held in FP32 (their rounding is ~6e-8
-            // of one step's change).  The stage state is formed from the
-            // rounded increment, so the shadow matches what the next stage
-            // sees, and the third stage's increment is kept for the residual
-            // monitor (exactly U3 - U up to U3's own rounding).
-            const double gamma = P.gamma;
-            double Un[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                float d;
-                if constexpr (MODE == 1) d = (float)(dt * r[m]);
-                else if constexpr (MODE == 2) d = (float)(0.25 * ((double)M.dU[m][id] + dt * r[m]));
-                else d = (float)((2.0 / 3.0) * ((double)M.dU[m][id] + dt * r[m]));
-                M.dU[m][id] = d;
-                Un[m] = M.U[m][id] + (double)d;
-            }
-            double rn = Un[0];
-            if (rn <= 0.0) rn = 1e-12;
-            const double un = Un[1] / rn, vn = Un[2] / rn, wn = Un[3] / rn;
-            const double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
-            const double pn = (gamma - 1.0) * rn * (Un[4] / rn - kinetic);
-            const double Tn = pn / (rn * P.Rgas);
-            if constexpr (MODE == 3) {
-                M.U[0][id] = rn;
-#pragma unroll
-                for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
-            }
-            M.S[0][id] = (float)(rn - P.rho_ref);
-            M.S[1][id] = (float)un;
-            M.S[2][id] = (float)vn;
-            M.S[3][id] = (float)wn;
-            M.S[4][id] = (float)(pn - P.p_ref);
-            M.S[5][id] = (float)(Tn - 1.0);
-            if constexpr (MODE == 3) {
-                if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                    atomicMin(flag, nonphysical_key(flag, id));
-            }
-        }
+        vdiv_finish<MODE>(P, M, id, r, dt, flag);
     }
 }
 
