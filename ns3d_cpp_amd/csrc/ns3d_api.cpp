@@ -690,6 +690,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.dx = g.dx; d.dy = g.dy; d.dz = g.dz;
     d.idx = 1.0 / g.dx; d.idy = 1.0 / g.dy; d.idz = 1.0 / g.dz;   // range_kutta.cpp:14-16
     d.cfl = p.cfl;
+    d.check_mask = 1 << 3;
     d.fvs = p.fvs_type; d.recon = p.recon; d.vis = p.vis_scheme; d.mu_law = p.mu_law;
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ng = g.ng;
     d.sx = (int)sx; d.sy = (int)sy; d.sz = (int)sz;

@@ -21,6 +21,7 @@ struct DevParams {
     double rho_ref, p_ref;   // reference state of the mixed-precision shadow
     int adj_lines;           // 1: adjoint sweeps may hold whole lines in LDS
     double cfl;              // CFL number of the device time step
+    int check_mask;          // bit s: stage s scans for non-physical cells (the third only)
 };
 
 // Slots of a context's device flag words: [0] key of the first non-physical
