@@ -1,0 +1,39 @@
+"""bench.py's contract parts that need no GPU: more than one GPU is refused
+with a message, and the per-kernel byte model adds up to what its comments
+state (the mixed stage kernel averages its three stages' 140 / 160 / 200 B)."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_byte_model():
+    b = load_bench()
+    fp, mx = b.ALG_BYTES["fp64"], b.ALG_BYTES["mixed"]
+    # mixed stage kernel: reads 9 fluxes (36), A (20), U (40), dU (20 in stages 2, 3);
+    # writes dU (20), the shadow (24), U (40 in stage 3)
+    stages = [36 + 20 + 40 + 20 + 24, 36 + 20 + 40 + 20 + 20 + 24, 36 + 20 + 40 + 20 + 20 + 24 + 40]
+    assert stages == [140, 160, 200]
+    assert abs(mx["vdiv_stage"] - sum(stages) / 3) < 1e-12
+    # one RHS + stage per RK3 stage, the CFL once per step
+    step = lambda a: 3 * sum(v for k, v in a.items() if k != "cfl") + a["cfl"]
+    assert step(fp) == 3 * (80 + 120 + 120 + 104 + 240) + 40
+    assert abs(step(mx) - (3 * (40 + 60 + 60 + 52) + 500 + 20)) < 1e-9
+    assert b.STEP_MODEL_BYTES == {"fp64": 2448, "mixed": 1564}
