@@ -257,6 +257,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--k0", type=float, default=4.0, help="spectrum peak wavenumber of the field")
+    ap.add_argument("--mt", type=float, default=0.3, help="turbulent Mach number of the field")
+    ap.add_argument("--re", type=float, default=200.0, help="Reynolds number")
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
@@ -270,7 +273,7 @@ def main():
     import ns3d_cpp_amd as ns
     from ns3d_cpp_amd import fields
 
-    n, Re, Mt, k0 = args.n, 200.0, 0.3, 4.0
+    n, Re, Mt, k0 = args.n, args.re, args.mt, args.k0
     H = hip()
     if args.adj_only:
         adj = adjoint_cost(ns, H, args.adj_n, args.adj_steps, args.adj_ckpt, args.seed)
@@ -312,10 +315,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"decaying isotropic turbulence {n}^3, k0=4, Mt=0.3, Re=200, "
+        "config": {"workload": f"decaying isotropic turbulence {n}^3, k0={k0:g}, Mt={Mt:g}, Re={Re:g}, "
                                f"{args.recon} + Steger-Warming + C8th, Sutherland 110.4/273.15, "
                                f"RK3 with CFL 0.5 dt every step, {head} path",
-                   "n": n, "recon": args.recon, "fvs": "StagerWarming", "vis": "C8th", "precision": head},
+                   "n": n, "k0": k0, "Mt": Mt, "Re": Re, "recon": args.recon, "fvs": "StagerWarming",
+                   "vis": "C8th", "precision": head},
         "roofline": h["roofline"],
         "step_model": h["step_model"],
         "kernel_bytes_per_cell_update": h["kernel_bytes_per_cell_update"],
