@@ -9,6 +9,13 @@
  * The reference itself cannot be built in this image (each hot-path file
  * includes <fftw3-mpi.h>, which is absent), so the restatement is pinned by
  * closed forms, invariants and convergence tests under tests/.
+ *
+ * Parity unpinned: the choice of tangential basis in the characteristic
+ * path (orc_char_matrices, InviscousFlux_new.cpp:568-578).  The checks pin
+ * L R = I, the eigen-decomposition of the flux Jacobian and the linear
+ * schemes' equality with component-wise faces, none of which depends on
+ * that choice; with WENO5 the choice changes the result, and it is
+ * restated as written, unchecked.
  */
 #ifndef NS3D_ORACLE_H
 #define NS3D_ORACLE_H
