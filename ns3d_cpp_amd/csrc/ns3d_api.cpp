@@ -51,9 +51,10 @@ struct ns3d_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     StepSig graph_sig[2];
     bool use_graphs = false;
+    bool cfl_fuse = true;   // mixed third stages reduce the next CFL minimum (NS3D_CFL_FUSE=0: off)
     double *slab = nullptr;      // one allocation for all cell arrays
     double *face_slab = nullptr;
-    unsigned long long *d_flag = nullptr;   // [0] non-physical key, [1] CFL bits
+    unsigned long long *d_flag = nullptr;   // FLAG_WORDS slots: non-physical key, CFL bits, ...
     double *d_partial = nullptr;            // per-block reduction partials
     double **d_ptrs = nullptr;              // 10 device-side pointer slots
     int nblocks = 0;
@@ -302,26 +303,37 @@ int step_end(ns3d_ctx *c, int dev_dt)
     return NS3D_OK;
 }
 
-// CFL reduction into d_flag[1] (compute_timestep, timestep.cpp:8-36)
-int cfl_enqueue(ns3d_ctx *c)
+// CFL reduction into the CFL slot (compute_timestep, timestep.cpp:8-36); with
+// next, the slot the steps' third stages reduce into starts over as well
+int cfl_enqueue(ns3d_ctx *c, bool next = false)
 {
     double init = 1e9;
     unsigned long long bits;
     std::memcpy(&bits, &init, sizeof(bits));
-    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 1, bits, 1));
-    if (c->mixed) TIMED(c, "cfl", launch_m_cfl(c->stream, c->dp, c->m, c->d_flag + 1));
-    else TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + 1));
+    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + FLAG_CFL, bits, 1));
+    if (next) HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + FLAG_CFL_NEXT, bits, 1));
+    if (c->mixed) TIMED(c, "cfl", launch_m_cfl(c->stream, c->dp, c->m, c->d_flag + FLAG_CFL));
+    else TIMED(c, "cfl", launch_cfl(c->stream, c->dp, c->f, c->d_flag + FLAG_CFL));
     return NS3D_OK;
+}
+
+// the mixed third stage can reduce the CFL minimum of the state it writes
+bool cfl_fused(const ns3d_ctx *c)
+{
+    return c->cfl_fuse && c->mixed && m_vdiv_cfl_next(c->dp, c->m_vd_block[0], c->m_vd_block[1]);
 }
 
 // one SSP-RK3 step, range_kutta.cpp:102-147 (FP64: U0 recorded by stage 1;
 // mixed: the stages' FP32 increments over the step's initial state); with
 // dev_dt the step's dt is cfl * the CFL reduction of its initial state,
-// taken on the device
-int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0)
+// taken on the device: by a CFL pass at the step's start, or, where the
+// third stage reduces it (cfl_fused), by the previous step of the same call
+// (first: the first step of a call, which runs the pass)
+int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0, bool first = true)
 {
-    if (dev_dt) {
-        int st = cfl_enqueue(c);
+    if (dev_dt && cfl_fused(c)) dev_dt = 2;
+    if (dev_dt && (first || dev_dt == 1)) {
+        int st = cfl_enqueue(c, dev_dt == 2);
         if (st) return st;
     }
     for (int stage = 1; stage <= 3; ++stage) {
@@ -372,7 +384,7 @@ int step_graph(ns3d_ctx *c, double dt, int dev_dt)
         hipGraph_t g = nullptr;
         HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const long long steps = c->steps_done;
-        const int st = rk3_enqueue(c, dt, dev_dt);
+        const int st = rk3_enqueue(c, dt, dev_dt, false);
         c->steps_done = steps;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st) {
@@ -398,7 +410,8 @@ int step_graph(ns3d_ctx *c, double dt, int dev_dt)
 int advance_steps(ns3d_ctx *c, int nsteps, double dt, int dev_dt)
 {
     for (int s = 0; s < nsteps; ++s) {
-        const int st = (s == 0 || c->prof || !c->use_graphs) ? rk3_enqueue(c, dt, dev_dt) : step_graph(c, dt, dev_dt);
+        const int st = (s == 0 || c->prof || !c->use_graphs) ? rk3_enqueue(c, dt, dev_dt, s == 0)
+                                                             : step_graph(c, dt, dev_dt);
         if (st) return st;
     }
     return NS3D_OK;
@@ -722,7 +735,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         if (st2) { ns3d_destroy(c); return st2; }
     }
     c->nblocks = ke_blocks(d);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->d_flag, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_flag, FLAG_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_partial, (size_t)c->nblocks * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_ptrs, 10 * sizeof(double *));
     if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc scratch"); ns3d_destroy(c); return r; }
@@ -768,8 +781,15 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         if (v > 0) for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = v;
     }
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
     e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 2);
-    if (e == hipSuccess) e = launch_fill_u64(c->stream, c->d_flag + 2, 0ull, 2);   // step counter, time
+    if (e == hipSuccess) e = launch_fill_u64(c->stream, c->d_flag + FLAG_STEP, 0ull, 2);   // step counter, time
+    if (e == hipSuccess) {
+        const double init = 1e9;
+        unsigned long long bits;
+        std::memcpy(&bits, &init, sizeof(bits));
+        e = launch_fill_u64(c->stream, c->d_flag + FLAG_CFL_NEXT, bits, 1);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { int r = hip_fail(c, e, "init"); ns3d_destroy(c); return r; }
     *out = c;
@@ -1165,10 +1185,10 @@ int ns3d_advance_cfl(ns3d_ctx *c, int nsteps, double *time)
     c->fw_valid = false;
     unsigned long long bits;
     std::memcpy(&bits, time, sizeof(bits));
-    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + 3, bits, 1));
+    HIPCALL(c, launch_fill_u64(c->stream, c->d_flag + FLAG_TIME, bits, 1));
     int st = advance_steps(c, nsteps, 0.0, 1);
     if (st) return st;
-    HIPCALL(c, hipMemcpyAsync(&bits, c->d_flag + 3, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipMemcpyAsync(&bits, c->d_flag + FLAG_TIME, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
     st = check_flag(c);   // synchronises the stream
     if (st && st != NS3D_ERR_NONPHYSICAL) return st;
     std::memcpy(time, &bits, sizeof(bits));
@@ -1198,7 +1218,7 @@ int ns3d_compute_timestep_local(ns3d_ctx *c, double *dt)
     int st = cfl_enqueue(c);
     if (st) return st;
     unsigned long long h = 0;
-    HIPCALL(c, hipMemcpyAsync(&h, c->d_flag + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipMemcpyAsync(&h, c->d_flag + FLAG_CFL, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     st = stream_sync(c);
     if (st) return st;
     double m;

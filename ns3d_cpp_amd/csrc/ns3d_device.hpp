@@ -26,19 +26,58 @@ struct DevParams {
 
 // Slots of a context's device flag words: [0] key of the first non-physical
 // cell, [1] the CFL minimum (bits of a positive double), [2] the RK3 step
-// counter, [3] the simulated time of the device-time-step loop.
+// counter, [3] the simulated time of the device-time-step loop, [4] the CFL
+// minimum of the state the running step produces, where its third stage
+// reduces it (the step's end moves it to [1] for the next step).
+enum { FLAG_BAD = 0, FLAG_CFL = 1, FLAG_STEP = 2, FLAG_TIME = 3, FLAG_CFL_NEXT = 4, FLAG_WORDS = 5 };
 
 // dt = cfl * min over cells (timestep.cpp:32), from the CFL reduction
 __device__ __forceinline__ double cfl_dt(const DevParams &P, const unsigned long long *flag)
 {
-    return P.cfl * __longlong_as_double((long long)flag[1]);
+    return P.cfl * __longlong_as_double((long long)flag[FLAG_CFL]);
+}
+
+// one cell's bound of compute_timestep (timestep.cpp:14-27): the smallest
+// d / (|velocity| + a) over the three axes
+__device__ __forceinline__ double cfl_cell(const DevParams &P, double rho, double u, double v, double w, double p)
+{
+    const double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
+    return fmin(P.dx / (fabs(u) + a), fmin(P.dy / (fabs(v) + a), P.dz / (fabs(w) + a)));
+}
+
+// The same minimum over a set of cells, taken per axis as d / max(|vel| + a):
+// a correctly rounded division is monotone in its divisor, so this equals
+// the minimum of the per-cell bounds bit for bit, with one division and one
+// square root per cell instead of four and one.  cfl_accum folds a cell in
+// (maxima start at 0; NaN bounds drop out as fmin drops them), cfl_bound
+// gives the set's minimum (1e9 for an empty set, compute_timestep's start).
+__device__ __forceinline__ void cfl_accum(const DevParams &P, double rho, double u, double v, double w, double p,
+                                          double &cx, double &cy, double &cz)
+{
+    const double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
+    cx = fmax(cx, fabs(u) + a);
+    cy = fmax(cy, fabs(v) + a);
+    cz = fmax(cz, fabs(w) + a);
+}
+
+__device__ __forceinline__ double cfl_bound(const DevParams &P, double cx, double cy, double cz)
+{
+    return fmin(1e9, fmin(P.dx / cx, fmin(P.dy / cy, P.dz / cz)));
+}
+
+// minimum over the 64 lanes of a wave (every lane active)
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 // key of a non-physical cell: the step counter in the high bits, so the
 // atomicMin keeps the earliest step's lowest cell index
 __device__ __forceinline__ unsigned long long nonphysical_key(const unsigned long long *flag, int id)
 {
-    return (flag[2] << 40) | (unsigned long long)id;
+    return (flag[FLAG_STEP] << 40) | (unsigned long long)id;
 }
 
 enum { FVS_SW = 0, FVS_LF = 1 };

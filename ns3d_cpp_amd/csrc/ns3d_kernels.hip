@@ -747,12 +747,6 @@ __global__ __launch_bounds__(256) void k_recon_fallback(long long n, int width, 
 
 // ----------------------------------------------------------- reductions
 
-__device__ __forceinline__ double wave_min(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -775,12 +769,7 @@ __global__ __launch_bounds__(256) void k_cfl(DevParams P, Fields F, unsigned lon
         const int j = r / P.nx;
         const int i = r - j * P.nx;
         const int id = ((P.ng + k) * P.sy + (P.ng + j)) * P.sx + (P.ng + i);
-        double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
-        double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
-        double dt_x = P.dx / (fabs(u) + a);
-        double dt_y = P.dy / (fabs(v) + a);
-        double dt_z = P.dz / (fabs(w) + a);
-        m = fmin(m, fmin(dt_x, fmin(dt_y, dt_z)));
+        m = fmin(m, cfl_cell(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id]));
     }
     m = wave_min(m);
     const int lane = threadIdx.x & 63;
@@ -839,14 +828,21 @@ static const dim3 kCellBlock(64, 4, 1);
 
 // end of an RK3 step on the device: advance the step counter of the
 // non-physical key and, for the device time step, the simulated time
-// t += cfl * min (maincycle.cpp's t += dt, in the same order)
+// t += cfl * min (maincycle.cpp's t += dt, in the same order); dev_dt 2:
+// the third stage reduced the next step's CFL minimum, which becomes the
+// current one, and its slot starts over
 __global__ void k_step_end(unsigned long long *flag, double cfl, int dev_dt)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    flag[2] = flag[2] + 1ull;
+    flag[FLAG_STEP] = flag[FLAG_STEP] + 1ull;
     if (dev_dt) {
-        const double t = __longlong_as_double((long long)flag[3]) + cfl * __longlong_as_double((long long)flag[1]);
-        flag[3] = (unsigned long long)__double_as_longlong(t);
+        const double t = __longlong_as_double((long long)flag[FLAG_TIME]) +
+                         cfl * __longlong_as_double((long long)flag[FLAG_CFL]);
+        flag[FLAG_TIME] = (unsigned long long)__double_as_longlong(t);
+    }
+    if (dev_dt == 2) {
+        flag[FLAG_CFL] = flag[FLAG_CFL_NEXT];
+        flag[FLAG_CFL_NEXT] = (unsigned long long)__double_as_longlong(1e9);
     }
 }
 

@@ -752,10 +752,12 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
 }
 
 // the end of a cell's divergence pass: the RHS into `out` (MODE 0), or the
-// RK3 stage update + C2P into U / dU / the shadow (MODE 1..3)
+// RK3 stage update + C2P into U / dU / the shadow (MODE 1..3); with cfl_next
+// the third stage also folds the new state's cell into the CFL maxima, from
+// the shadow values it stores (the values the CFL pass would read)
 template <int MODE>
 __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M, int id, const double r[5],
-                                            double dt, unsigned long long *flag)
+                                            double dt, unsigned long long *flag, bool cfl_next, double &cx, double &cy, double &cz)
 {
     if constexpr (MODE == 0) {
 #pragma unroll
@@ -794,15 +796,20 @@ __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M
 #pragma unroll
             for (int m = 1; m < 5; ++m) M.U[m][id] = Un[m];
         }
-        M.S[0][id] = (float)(rn - P.rho_ref);
-        M.S[1][id] = (float)un;
-        M.S[2][id] = (float)vn;
-        M.S[3][id] = (float)wn;
-        M.S[4][id] = (float)(pn - P.p_ref);
+        const float s0 = (float)(rn - P.rho_ref), s1 = (float)un, s2 = (float)vn, s3 = (float)wn;
+        const float s4 = (float)(pn - P.p_ref);
+        M.S[0][id] = s0;
+        M.S[1][id] = s1;
+        M.S[2][id] = s2;
+        M.S[3][id] = s3;
+        M.S[4][id] = s4;
         M.S[5][id] = (float)(Tn - 1.0);
         if constexpr (MODE == 3) {
             if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                atomicMin(flag, nonphysical_key(flag, id));
+                atomicMin(flag + FLAG_BAD, nonphysical_key(flag, id));
+            if (cfl_next)
+                cfl_accum(P, P.rho_ref + (double)s0, (double)s1, (double)s2, (double)s3, P.p_ref + (double)s4,
+                          cx, cy, cz);
         }
     }
 }
@@ -814,6 +821,10 @@ __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M
 // keeps the ~80 loads of an iteration in flight; at its default target of 4
 // it waits for them in small groups (0.74 against 0.81 ms per launch at
 // 256^3).
+// dt: the argument (dev_dt 0) or cfl * the CFL minimum in the flag words;
+// dev_dt 2 (the grid tiles the interior exactly, whole waves): the third
+// stage also reduces the new state's CFL minimum into the next step's slot,
+// which replaces the CFL pass at the start of the next step.
 template <int VIS, int MODE, int BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_z(DevParams P, MFields M, double dt, int seg_len,
                                                  unsigned long long *flag, int dev_dt)
@@ -848,6 +859,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         axis_dist(P.periodic[0], i, ng, P.nx, R, dxl, dxh);
         axis_dist(P.periodic[1], j, ng, P.ny, R, dyl, dyh);
     }
+    const bool cfl_next = MODE == 3 && dev_dt == 2;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         const int kn = wrapi_m(k + R, ng, P.nz) * sxy + col;
@@ -869,7 +882,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                              diff32_bc<VIS>(&win[m][R], idz, dzl, dzh);
             r[m + 1] = (double)vd - (double)M.A[m + 1][id];
         }
-        vdiv_finish<MODE>(P, M, id, r, dt, flag);
+        vdiv_finish<MODE>(P, M, id, r, dt, flag, cfl_next, cx, cy, cz);
+    }
+    if (cfl_next) {
+        const double cm = wave_min(cfl_bound(P, cx, cy, cz));
+        if (__lane_id() == 0) atomicMin(flag + FLAG_CFL_NEXT, (unsigned long long)__double_as_longlong(cm));
     }
 }
 
@@ -1019,7 +1036,7 @@ __global__ __launch_bounds__(256) void km_cfl(DevParams P, MFields M, unsigned l
     __shared__ double red[4];
     const int nxy = P.nx * P.ny;
     const int total = nxy * P.nz;
-    double m = 1e9;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const int k = t / nxy;
         const int r = t - k * nxy;
@@ -1028,11 +1045,9 @@ __global__ __launch_bounds__(256) void km_cfl(DevParams P, MFields M, unsigned l
         const int id = ((P.ng + k) * P.sy + (P.ng + j)) * P.sx + (P.ng + i);
         double rho, u, v, w, p;
         shadow_prims(P, M, id, rho, u, v, w, p);
-        const double a = sqrt(P.gamma * p / fmax(rho, 1e-14));
-        m = fmin(m, fmin(P.dx / (fabs(u) + a), fmin(P.dy / (fabs(v) + a), P.dz / (fabs(w) + a))));
+        cfl_accum(P, rho, u, v, w, p, cx, cy, cz);
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+    const double m = wave_min(cfl_bound(P, cx, cy, cz));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1222,10 +1237,16 @@ static hipError_t m_vdiv_vis(hipStream_t s, const DevParams &P, const MFields &M
     return hipGetLastError();
 }
 
+bool m_vdiv_cfl_next(const DevParams &P, int bx, int by)
+{
+    return (bx * by) % 64 == 0 && P.nx % bx == 0 && P.ny % by == 0;
+}
+
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, int dev_dt, int bx, int by)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
+    if (dev_dt == 2 && !m_vdiv_cfl_next(P, bx, by)) return hipErrorInvalidValue;   // partial waves
     if (P.periodic[0] && P.periodic[1] && P.periodic[2])
         return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);
     return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);

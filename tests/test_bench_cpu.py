@@ -1,6 +1,7 @@
 """bench.py's contract parts that need no GPU: more than one GPU is refused
 with a message, and the per-kernel byte model adds up to what its comments
-state (the mixed stage kernel averages its three stages' 140 / 160 / 200 B)."""
+state (the mixed stage kernel averages its three stages' 140 / 160 / 200 B;
+the mixed step has no CFL pass of its own)."""
 import importlib.util
 import os
 import subprocess
@@ -32,8 +33,10 @@ def test_byte_model():
     stages = [36 + 20 + 40 + 20 + 24, 36 + 20 + 40 + 20 + 20 + 24, 36 + 20 + 40 + 20 + 20 + 24 + 40]
     assert stages == [140, 160, 200]
     assert abs(mx["vdiv_stage"] - sum(stages) / 3) < 1e-12
-    # one RHS + stage per RK3 stage, the CFL once per step
-    step = lambda a: 3 * sum(v for k, v in a.items() if k != "cfl") + a["cfl"]
-    assert step(fp) == 3 * (80 + 120 + 120 + 104 + 240) + 40
-    assert abs(step(mx) - (3 * (40 + 60 + 60 + 52) + 500 + 20)) < 1e-9
+    # one RHS + stage per RK3 stage; the CFL pass once per step in FP64, and
+    # not at all in mixed mode (its third stage reduces the next minimum)
+    step = lambda a, ncfl: 3 * sum(v for k, v in a.items() if k != "cfl") + a["cfl"] * ncfl
+    assert step(fp, 1) == 3 * (80 + 120 + 120 + 104 + 240) + 40
+    assert abs(step(mx, 0) - (3 * (40 + 60 + 60 + 52) + 500)) < 1e-9
+    # the metric's model keeps the CFL read in both (2408 + 40, 1544 + 20)
     assert b.STEP_MODEL_BYTES == {"fp64": 2448, "mixed": 1564}

@@ -2,8 +2,10 @@
 and ns3d_advance_cfl after their first step) against eager steps: the same
 states bit for bit, replay after replay, with eager work and host
 synchronisation between the replays, across a change of dt (a new capture)
-and for the device-side CFL time step.  The non-physical-cell report keeps
-the earliest step's first cell when later steps run from the graph."""
+and for the device-side CFL time step (with the mixed third stage reducing
+the next step's CFL minimum where its workgroups tile the grid).  The
+non-physical-cell report keeps the earliest step's first cell when later
+steps run from the graph."""
 import numpy as np
 import pytest
 
@@ -99,6 +101,44 @@ def test_advance_cfl_matches_host_time_loop(gpu, case):
         t_b += dt
     assert t_a == t_b
     assert same(a, b)
+
+
+@pytest.mark.parametrize("graphs", ["1", "0"])
+def test_advance_cfl_fused_reduction_matches_host_time_loop(gpu, monkeypatch, graphs):
+    """Mixed mode on a grid the stage kernel's 64 x 4 workgroups tile
+    exactly: the third stage reduces the next step's CFL minimum, and only
+    the first step of a call runs the CFL pass.  dt, time and state equal
+    the host loop's bit for bit."""
+    monkeypatch.setenv("NS3D_GRAPHS", graphs)
+    n = 64
+    st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=7)
+    p, g = ns.make_params(n, Re=200.0, Ma=Ma, **CASES["mixed-periodic"])
+    a, b = ns.Solver(p, g), ns.Solver(p, g)
+    a.upload_primitive(*st)
+    b.upload_primitive(*st)
+    a.profile(True)
+    t_a = a.advance_cfl(4, 0.0)
+    rep = a.profile_report()
+    a.profile(False)
+    assert rep["cfl"][1] == 1 and rep["vdiv_stage"][1] == 12
+    t_b = 0.0
+    for _ in range(4):
+        dt = b.timestep()
+        b.rk3_step(dt)
+        t_b += dt
+    assert t_a == t_b
+    assert same(a, b)
+    # a host step in between: the next call starts from a fresh CFL pass
+    a.rk3_step(1e-3)
+    b.rk3_step(1e-3)
+    t_a = a.advance_cfl(5, t_a)
+    for _ in range(5):
+        dt = b.timestep()
+        b.rk3_step(dt)
+        t_b += dt
+    assert t_a == t_b
+    assert same(a, b)
+    assert a.timestep() == b.timestep()
 
 
 def test_graphs_off_and_profiling_give_the_same_states(gpu, monkeypatch):
