@@ -104,7 +104,8 @@ def test_advance_cfl_matches_host_time_loop(gpu, case):
 
 
 @pytest.mark.parametrize("graphs", ["1", "0"])
-def test_advance_cfl_fused_reduction_matches_host_time_loop(gpu, monkeypatch, graphs):
+@pytest.mark.parametrize("case", ["mixed-periodic", "mixed-walls"])
+def test_advance_cfl_fused_reduction_matches_host_time_loop(gpu, monkeypatch, case, graphs):
     """Mixed mode on a grid the stage kernel's 64 x 4 workgroups tile
     exactly: the third stage reduces the next step's CFL minimum, and only
     the first step of a call runs the CFL pass.  dt, time and state equal
@@ -112,7 +113,7 @@ def test_advance_cfl_fused_reduction_matches_host_time_loop(gpu, monkeypatch, gr
     monkeypatch.setenv("NS3D_GRAPHS", graphs)
     n = 64
     st, Ma = fields.hit(n, k0=4, Mt=0.3, seed=7)
-    p, g = ns.make_params(n, Re=200.0, Ma=Ma, **CASES["mixed-periodic"])
+    p, g = ns.make_params(n, Re=200.0, Ma=Ma, **CASES[case])
     a, b = ns.Solver(p, g), ns.Solver(p, g)
     a.upload_primitive(*st)
     b.upload_primitive(*st)
