@@ -256,26 +256,29 @@ __global__ __launch_bounds__(64, 2) void km_sweep64(DevParams P, MFields M, int 
 // WENO5-JS face value minus f[2] from D[1..4]: one reciprocal for the
 // weights, after scaling the (eps + beta)^2 near their maximum (an exact
 // power of two from the exponent bits: the y / z sweeps ran 2.5 % faster
-// than with a second, quarter-rate, reciprocal).
+// than with a second, quarter-rate, reciprocal).  The weights see only
+// ratios of the T_k and of the alpha_k, so t_k = 4 (eps + beta_k) (the 1/4
+// of beta's second square folded away) and alpha_k scaled by 10 (linear
+// weights 1, 6, 3) leave them unchanged and save a multiply each.
 __device__ __forceinline__ float rel32_weno5(const float *D)
 {
     const float D1 = D[1], D2 = D[2], D3 = D[3], D4 = D[4];
-    const float c13 = 13.0f / 12.0f;
+    const float c13 = 13.0f / 3.0f;
     const float s0 = D2 - D1, r0 = 3.0f * D2 - D1;
     const float s1 = D3 - D2, r1 = D2 + D3;
     const float s2 = D4 - D3, r2 = D4 - 3.0f * D3;
-    const float t0 = 1e-6f + c13 * s0 * s0 + 0.25f * r0 * r0;
-    const float t1 = 1e-6f + c13 * s1 * s1 + 0.25f * r1 * r1;
-    const float t2 = 1e-6f + c13 * s2 * s2 + 0.25f * r2 * r2;
+    const float t0 = 4e-6f + c13 * s0 * s0 + r0 * r0;
+    const float t1 = 4e-6f + c13 * s1 * s1 + r1 * r1;
+    const float t2 = 4e-6f + c13 * s2 * s2 + r2 * r2;
     float T0 = t0 * t0, T1 = t1 * t1, T2 = t2 * t2;
     // scale by the power of two 2^(127 - e), e the biased exponent of the
-    // largest T (T >= 1e-12 > 0, so e is in 87..254): the largest scaled T is
-    // in [1, 2), exactly, with integer ops instead of a reciprocal
+    // largest T (T >= 1.6e-11 > 0, so e is in 91..254): the largest scaled T
+    // is in [1, 2), exactly, with integer ops instead of a reciprocal
     const unsigned e = (__float_as_uint(fmaxf(T0, fmaxf(T1, T2))) >> 23) & 0xffu;
     const float im = __uint_as_float((254u - e) << 23);
     T0 *= im; T1 *= im; T2 *= im;
-    // alpha_k = d_k / T_k, scaled by T0 T1 T2 (the largest T is 1 now)
-    const float a0 = 0.1f * T1 * T2, a1 = 0.6f * T0 * T2, a2 = 0.3f * T0 * T1;
+    // 10 alpha_k = 10 d_k / T_k, scaled by T0 T1 T2 (the largest T is < 2)
+    const float a0 = T1 * T2, a1 = 6.0f * T0 * T2, a2 = 3.0f * T0 * T1;
     const float q0 = 5.0f * D2 - 2.0f * D1;
     const float q1 = D2 + 2.0f * D3;
     const float q2 = 4.0f * D3 - D4;
