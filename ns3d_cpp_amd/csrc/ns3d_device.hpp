@@ -80,7 +80,17 @@ __device__ __forceinline__ unsigned long long nonphysical_key(const unsigned lon
     return (flag[FLAG_STEP] << 40) | (unsigned long long)id;
 }
 
-enum { FVS_SW = 0, FVS_LF = 1 };
+// FVS_SWE: Steger-Warming with the eps-smoothed eigenvalue split (sw_eps !=
+// 0).  The split kernels take the form as a template value: chosen at run
+// time, the compiler evaluated both forms of every eigenvalue (three FP64
+// square roots per split) and selected one.  P.fvs itself stays SW or LF.
+enum { FVS_SW = 0, FVS_LF = 1, FVS_SWE = 2 };
+
+// the split kernels' template value for the parameters
+__host__ __device__ inline int fvs_kind(const DevParams &P)
+{
+    return P.fvs == FVS_LF ? FVS_LF : (P.sw_eps != 0.0 ? FVS_SWE : FVS_SW);
+}
 enum { R_WENO5 = 0, R_LINEAR = 2, R_MDCD = 3, R_MDCD_HYBRID = 4, R_UD7 = 5, R_UD7L = 6 };
 enum { VIS_C8 = 0, VIS_C6 = 1, VIS_C4 = 2 };
 
@@ -182,6 +192,21 @@ __device__ __forceinline__ void lam_split(double lam, double eps, double &lp, do
     }
 }
 
+// lam_split with its form fixed by the kernel's FVS value (the same
+// expressions)
+template <int FVS>
+__device__ __forceinline__ void lam_split_fvs(double lam, double eps, double &lp, double &lm)
+{
+    if constexpr (FVS == FVS_SWE) {
+        double r = sqrt(lam * lam + eps * eps);
+        lp = 0.5 * (lam + r);
+        lm = 0.5 * (lam - r);
+    } else {
+        lp = 0.5 * (lam + fabs(lam));
+        lm = 0.5 * (lam - fabs(lam));
+    }
+}
+
 // split_StagerWarming / split_LaxFriedrichs (InviscousFlux_new.cpp:439-530)
 // for the axis (N1,N2,N3); E and c come from p2c_energy_sound.
 template <int FVS, int DIR>
@@ -194,15 +219,15 @@ __device__ __forceinline__ void split_state(const DevParams &P, double rho, doub
     const double gamma = P.gamma;
     double E, c;
     p2c_energy_sound(gamma, rho, u, v, w, p, E, c);
-    if constexpr (FVS == FVS_SW) {
+    if constexpr (FVS == FVS_SW || FVS == FVS_SWE) {
         double un = u * n1 + v * n2 + w * n3;
         double lambda1 = un - c;
         double lambda2 = un;
         double lambda3 = un + c;
         double l1p, l1m, l2p, l2m, l3p, l3m;
-        lam_split(lambda1, P.sw_eps, l1p, l1m);
-        lam_split(lambda2, P.sw_eps, l2p, l2m);
-        lam_split(lambda3, P.sw_eps, l3p, l3m);
+        lam_split_fvs<FVS>(lambda1, P.sw_eps, l1p, l1m);
+        lam_split_fvs<FVS>(lambda2, P.sw_eps, l2p, l2m);
+        lam_split_fvs<FVS>(lambda3, P.sw_eps, l3p, l3m);
         double S1_p = l1p + l3p;
         double S1_m = l1m + l3m;
         double S2_p = l3p - l1p;

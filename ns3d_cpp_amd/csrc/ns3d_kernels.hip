@@ -924,8 +924,13 @@ static SweepFn pick_sweep_r(int recon, int dir)
 hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int seg_len, int minw)
 {
     SweepFn fn;
-    if (minw == 3) fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW, 3>(P.recon, dir) : pick_sweep_r<FVS_LF, 3>(P.recon, dir);
-    else fn = P.fvs == FVS_SW ? pick_sweep_r<FVS_SW, 2>(P.recon, dir) : pick_sweep_r<FVS_LF, 2>(P.recon, dir);
+    const int fk = fvs_kind(P);
+    if (minw == 3)
+        fn = fk == FVS_SW ? pick_sweep_r<FVS_SW, 3>(P.recon, dir)
+             : fk == FVS_SWE ? pick_sweep_r<FVS_SWE, 3>(P.recon, dir) : pick_sweep_r<FVS_LF, 3>(P.recon, dir);
+    else
+        fn = fk == FVS_SW ? pick_sweep_r<FVS_SW, 2>(P.recon, dir)
+             : fk == FVS_SWE ? pick_sweep_r<FVS_SWE, 2>(P.recon, dir) : pick_sweep_r<FVS_LF, 2>(P.recon, dir);
     if (!fn) return hipErrorInvalidValue;
     const int na = dir == 0 ? P.ny : P.nx;
     const int nb = dir == 2 ? P.ny : P.nz;
@@ -966,7 +971,9 @@ bool xrow_supported(const DevParams &P)
 
 hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F)
 {
-    XrowFn fn = P.fvs == FVS_SW ? pick_xrow<FVS_SW>(P.recon) : pick_xrow<FVS_LF>(P.recon);
+    const int fk = fvs_kind(P);
+    XrowFn fn = fk == FVS_SW ? pick_xrow<FVS_SW>(P.recon)
+                : fk == FVS_SWE ? pick_xrow<FVS_SWE>(P.recon) : pick_xrow<FVS_LF>(P.recon);
     if (!fn || !xrow_supported(P)) return hipErrorInvalidValue;
     const size_t lds = xrow_lds_bytes(P);
     const int threads = ((P.nx + 63) / 64) * 64;
@@ -1057,7 +1064,10 @@ hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, cons
                         int dir, int char_recon)
 {
     FaceFn fn;
-    if (P.fvs == FVS_SW) fn = char_recon ? pick_faces_r<FVS_SW, 1>(P.recon, dir) : pick_faces_r<FVS_SW, 0>(P.recon, dir);
+    const int fk = fvs_kind(P);
+    if (fk == FVS_SW) fn = char_recon ? pick_faces_r<FVS_SW, 1>(P.recon, dir) : pick_faces_r<FVS_SW, 0>(P.recon, dir);
+    else if (fk == FVS_SWE)
+        fn = char_recon ? pick_faces_r<FVS_SWE, 1>(P.recon, dir) : pick_faces_r<FVS_SWE, 0>(P.recon, dir);
     else fn = char_recon ? pick_faces_r<FVS_LF, 1>(P.recon, dir) : pick_faces_r<FVS_LF, 0>(P.recon, dir);
     if (!fn) return hipErrorInvalidValue;
     const int n0 = dir == 0 ? P.nx + 1 : P.nx;
@@ -1101,7 +1111,9 @@ static void split_batch_dir(hipStream_t s, const DevParams &P, long long n, cons
 
 hipError_t launch_split_batch(hipStream_t s, const DevParams &P, long long n, const SplitBatch &b, int dir)
 {
-    if (P.fvs == FVS_SW) split_batch_dir<FVS_SW>(s, P, n, b, dir);
+    const int fk = fvs_kind(P);
+    if (fk == FVS_SW) split_batch_dir<FVS_SW>(s, P, n, b, dir);
+    else if (fk == FVS_SWE) split_batch_dir<FVS_SWE>(s, P, n, b, dir);
     else split_batch_dir<FVS_LF>(s, P, n, b, dir);
     return hipGetLastError();
 }

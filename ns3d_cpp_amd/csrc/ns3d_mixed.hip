@@ -364,11 +364,11 @@ __device__ __forceinline__ void split_fast(const DevParams &P, double rho, doubl
     const double c2 = gamma * p * irho;
     const double c = sqrt(c2);
     const double un = DIR == 0 ? u : (DIR == 1 ? v : w);
-    if constexpr (FVS == FVS_SW) {
+    if constexpr (FVS == FVS_SW || FVS == FVS_SWE) {
         double l1p, l1m, l2p, l2m, l3p, l3m;
-        lam_split(un - c, P.sw_eps, l1p, l1m);
-        lam_split(un, P.sw_eps, l2p, l2m);
-        lam_split(un + c, P.sw_eps, l3p, l3m);
+        lam_split_fvs<FVS>(un - c, P.sw_eps, l1p, l1m);
+        lam_split_fvs<FVS>(un, P.sw_eps, l2p, l2m);
+        lam_split_fvs<FVS>(un + c, P.sw_eps, l3p, l3m);
         const double H = (E + p) * irho;
         const double ga1 = rho * (0.5 / gamma), ga2 = 2.0 * (gamma - 1.0);
         const double S1p = l1p + l3p, S1m = l1m + l3m;
@@ -1173,8 +1173,13 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
 {
     if (dir == 0) {
         MXrowFn fn;
-        if (P.periodic[0]) fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW, 1>(P.recon) : m_pick_xrow<FVS_LF, 1>(P.recon);
-        else fn = P.fvs == FVS_SW ? m_pick_xrow<FVS_SW, 0>(P.recon) : m_pick_xrow<FVS_LF, 0>(P.recon);
+        const int fk = fvs_kind(P);
+        if (P.periodic[0])
+            fn = fk == FVS_SW ? m_pick_xrow<FVS_SW, 1>(P.recon)
+                 : fk == FVS_SWE ? m_pick_xrow<FVS_SWE, 1>(P.recon) : m_pick_xrow<FVS_LF, 1>(P.recon);
+        else
+            fn = fk == FVS_SW ? m_pick_xrow<FVS_SW, 0>(P.recon)
+                 : fk == FVS_SWE ? m_pick_xrow<FVS_SWE, 0>(P.recon) : m_pick_xrow<FVS_LF, 0>(P.recon);
         int threads;
         size_t lds;
         m_xrow_shape(P, threads, lds);
@@ -1183,8 +1188,13 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
         return hipGetLastError();
     }
     MSweepFn fn;
-    if (dir == 1) fn = P.fvs == FVS_SW ? m_pick_sweep<1, FVS_SW>(P.recon) : m_pick_sweep<1, FVS_LF>(P.recon);
-    else fn = P.fvs == FVS_SW ? m_pick_sweep<2, FVS_SW>(P.recon) : m_pick_sweep<2, FVS_LF>(P.recon);
+    const int fk = fvs_kind(P);
+    if (dir == 1)
+        fn = fk == FVS_SW ? m_pick_sweep<1, FVS_SW>(P.recon)
+             : fk == FVS_SWE ? m_pick_sweep<1, FVS_SWE>(P.recon) : m_pick_sweep<1, FVS_LF>(P.recon);
+    else
+        fn = fk == FVS_SW ? m_pick_sweep<2, FVS_SW>(P.recon)
+             : fk == FVS_SWE ? m_pick_sweep<2, FVS_SWE>(P.recon) : m_pick_sweep<2, FVS_LF>(P.recon);
     if (!fn) return hipErrorInvalidValue;
     const int nb = dir == 2 ? P.ny : P.nz;
     const int nd = dir == 1 ? P.ny : P.nz;
