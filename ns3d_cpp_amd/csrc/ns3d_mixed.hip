@@ -279,7 +279,7 @@ __device__ __forceinline__ float rel32_weno5(const float *D)
     T0 *= im; T1 *= im; T2 *= im;
     // 10 alpha_k = 10 d_k / T_k, scaled by T0 T1 T2 (the largest T is < 2)
     const float a0 = T1 * T2, a1 = 6.0f * T0 * T2, a2 = 3.0f * T0 * T1;
-    const float q0 = 5.0f * D2 - 2.0f * D1;
+    const float q0 = 2.0f * r0 - D2;          // 5 D2 - 2 D1
     const float q1 = D2 + 2.0f * D3;
     const float q2 = 4.0f * D3 - D4;
     return (a0 * q0 + a1 * q1 + a2 * q2) * (rcpf(a0 + a1 + a2) * (1.0f / 6.0f));
@@ -346,11 +346,13 @@ __device__ __forceinline__ float rel32(const float *e, const float *D)
     }
 }
 
-// split_state with reciprocals instead of divisions (this path does not
-// reproduce the reference's rounding).  Contraction into FMAs is off here:
-// the sweeps call it from several places and difference the results of
-// neighbouring cells, so a uniform state must give bitwise equal values
-// whichever call site evaluated them (free-stream preservation).
+// split_state with reciprocals instead of divisions and explicit FMAs (this
+// path does not reproduce the reference's rounding).  Contraction is off
+// here and every FMA is written out: the sweeps call it from several places
+// and difference the results of neighbouring cells, so a uniform state must
+// give bitwise equal values whichever call site evaluated them (free-stream
+// preservation).  Without eps the eigenvalue split is max(l, 0), min(l, 0),
+// which 0.5 (l +- |l|) equals up to the sign of a zero.
 template <int FVS, int DIR>
 __device__ __forceinline__ void split_fast(const DevParams &P, double rho, double u, double v, double w, double p,
                                            double Fp[5], double Fm[5])
@@ -358,32 +360,39 @@ __device__ __forceinline__ void split_fast(const DevParams &P, double rho, doubl
 #pragma clang fp contract(off)
     const double gamma = P.gamma;
     const double irho = rcp64(rho);
-    const double V = 0.5 * (u * u + v * v + w * w);
+    const double V = 0.5 * fma(w, w, fma(v, v, u * u));
     const double ig1 = 1.0 / (gamma - 1.0);
-    const double E = p * ig1 + rho * V;
+    const double E = fma(rho, V, p * ig1);
     const double c2 = gamma * p * irho;
     const double c = sqrt(c2);
     const double un = DIR == 0 ? u : (DIR == 1 ? v : w);
     if constexpr (FVS == FVS_SW || FVS == FVS_SWE) {
         double l1p, l1m, l2p, l2m, l3p, l3m;
-        lam_split_fvs<FVS>(un - c, P.sw_eps, l1p, l1m);
-        lam_split_fvs<FVS>(un, P.sw_eps, l2p, l2m);
-        lam_split_fvs<FVS>(un + c, P.sw_eps, l3p, l3m);
+        if constexpr (FVS == FVS_SWE) {
+            lam_split_fvs<FVS>(un - c, P.sw_eps, l1p, l1m);
+            lam_split_fvs<FVS>(un, P.sw_eps, l2p, l2m);
+            lam_split_fvs<FVS>(un + c, P.sw_eps, l3p, l3m);
+        } else {
+            l1p = fmax(un - c, 0.0); l1m = fmin(un - c, 0.0);
+            l2p = fmax(un, 0.0); l2m = fmin(un, 0.0);
+            l3p = fmax(un + c, 0.0); l3m = fmin(un + c, 0.0);
+        }
         const double H = (E + p) * irho;
         const double ga1 = rho * (0.5 / gamma), ga2 = 2.0 * (gamma - 1.0);
         const double S1p = l1p + l3p, S1m = l1m + l3m;
         const double S2p = (l3p - l1p) * c, S2m = (l3m - l1m) * c;
+        const double Gp = fma(ga2, l2p, S1p), Gm = fma(ga2, l2m, S1m);   // S1 + ga2 l2
         const double g2p = ga2 * l2p, g2m = ga2 * l2m;
-        Fp[0] = ga1 * (S1p + g2p);
-        Fp[1] = ga1 * ((S1p + g2p) * u + (DIR == 0 ? S2p : 0.0));
-        Fp[2] = ga1 * ((S1p + g2p) * v + (DIR == 1 ? S2p : 0.0));
-        Fp[3] = ga1 * ((S1p + g2p) * w + (DIR == 2 ? S2p : 0.0));
-        Fp[4] = ga1 * (S1p * H + g2p * V + S2p * un);
-        Fm[0] = ga1 * (S1m + g2m);
-        Fm[1] = ga1 * ((S1m + g2m) * u + (DIR == 0 ? S2m : 0.0));
-        Fm[2] = ga1 * ((S1m + g2m) * v + (DIR == 1 ? S2m : 0.0));
-        Fm[3] = ga1 * ((S1m + g2m) * w + (DIR == 2 ? S2m : 0.0));
-        Fm[4] = ga1 * (S1m * H + g2m * V + S2m * un);
+        Fp[0] = ga1 * Gp;
+        Fp[1] = ga1 * (DIR == 0 ? fma(Gp, u, S2p) : Gp * u);
+        Fp[2] = ga1 * (DIR == 1 ? fma(Gp, v, S2p) : Gp * v);
+        Fp[3] = ga1 * (DIR == 2 ? fma(Gp, w, S2p) : Gp * w);
+        Fp[4] = ga1 * fma(S1p, H, fma(g2p, V, S2p * un));
+        Fm[0] = ga1 * Gm;
+        Fm[1] = ga1 * (DIR == 0 ? fma(Gm, u, S2m) : Gm * u);
+        Fm[2] = ga1 * (DIR == 1 ? fma(Gm, v, S2m) : Gm * v);
+        Fm[3] = ga1 * (DIR == 2 ? fma(Gm, w, S2m) : Gm * w);
+        Fm[4] = ga1 * fma(S1m, H, fma(g2m, V, S2m * un));
     } else {
         const double a = fabs(un) + c;
         const double F1 = rho * un;
