@@ -62,7 +62,7 @@ struct ns3d_ctx {
     int seg_len[3] = {64, 64, 64};
     int seg_z = 32;                          // z segments of the z-marching kernels
     int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
-    int sweep_minw = 2;                      // waves-per-EU floor of k_sweep
+    int sweep_minw = 0;                      // waves-per-EU floor of k_sweep (0: per scheme)
     int m_gf_block[2] = {64, 4};             // workgroup shape of km_gradflux
     int m_vd_block[2] = {64, 4};             // workgroup shape of km_vdiv_z
     unsigned long long bad_key = ~0ull;

@@ -898,7 +898,8 @@ hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F
 using SweepFn = void (*)(DevParams, Fields, int);
 
 // x stores (ACC = 0), y and z accumulate (ACC = 1); MINW is the waves-per-SIMD
-// floor the register allocation must allow (2: no spills, 3: a few spills).
+// floor the register allocation must allow (2: no scheme spills; 3: WENO5
+// without eps fits, the others spill).
 template <int RECON, int FVS, int MINW>
 static SweepFn pick_sweep(int dir)
 {
@@ -925,6 +926,10 @@ hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int 
 {
     SweepFn fn;
     const int fk = fvs_kind(P);
+    // 0: per scheme.  WENO5 without eps fits the 168 VGPRs of 3 waves per
+    // SIMD with no spill (y / z sweeps 1.22 against 1.24 ms at 256^3); the
+    // other schemes spill there and run 1.6-2.7x slower, so they keep 2
+    if (minw == 0) minw = P.recon == R_WENO5 && fk != FVS_SWE ? 3 : 2;
     if (minw == 3)
         fn = fk == FVS_SW ? pick_sweep_r<FVS_SW, 3>(P.recon, dir)
              : fk == FVS_SWE ? pick_sweep_r<FVS_SWE, 3>(P.recon, dir) : pick_sweep_r<FVS_LF, 3>(P.recon, dir);
