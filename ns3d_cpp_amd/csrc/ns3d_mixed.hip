@@ -578,14 +578,24 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
         for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
     }
     float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // the accumulator of the next cell written, loaded an iteration ahead
+    // (a load at the point of the update left its latency exposed)
+    float an[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) an[m] = M.A[m][base + c0 * st];
     for (int f = c0 - 1; f < c1; ++f) {
         n0 = n0 + 1 == L ? 0 : n0 + 1;
+        float acur[5] = {an[0], an[1], an[2], an[3], an[4]};
         {
             float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
             if (f + 1 < c1) {
                 const int id = base + (f + 1 - MID + S - 1) * st;
 #pragma unroll
                 for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+                if (f >= c0) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) an[m] = M.A[m][base + (f + 1) * st];
+                }
             }
             double Fp[5], Fm[5];
             split_fast<FVS, DIR>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
@@ -607,7 +617,7 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
             const int idc = base + f * st;
 #pragma unroll
             for (int m = 0; m < 5; ++m)
-                M.A[m][idc] += (get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m])) * inv_d;
+                M.A[m][idc] = acur[m] + (get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m])) * inv_d;
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
