@@ -182,16 +182,34 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
         int id = base + (c0 - 1 - MID + S - 1) * st;
         nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
     }
+    // y / z: the accumulator of the next cell written, loaded an iteration
+    // ahead (a load at the point of the update left its latency exposed)
+    double an[5];
+    if constexpr (ACC == 1) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) an[m] = F.A[m][base + c0 * st];
+    }
     for (int f = c0 - 1; f < c1; ++f) {
 #pragma unroll
         for (int s = 0; s < S - 1; ++s)
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m]; }
+        double acur[5];
+        if constexpr (ACC == 1) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) acur[m] = an[m];
+        }
         {
             double cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
             if (f + 1 < c1) {
                 int id = base + (f + 1 - MID + S - 1) * st;
                 nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
+                if constexpr (ACC == 1) {
+                    if (f >= c0) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) an[m] = F.A[m][base + (f + 1) * st];
+                    }
+                }
             }
             split_state<FVS, DIR>(P, cur[0], cur[1], cur[2], cur[3], cur[4], fp[S - 1], fm[S - 1]);
         }
@@ -210,8 +228,8 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
                 double ad = (ff[m] - fprev[m]) * inv_d;
-                if (ACC == 0) F.A[m][idc] = ad;
-                else F.A[m][idc] = F.A[m][idc] + ad;
+                if constexpr (ACC == 0) F.A[m][idc] = ad;
+                else F.A[m][idc] = acur[m] + ad;
             }
         }
 #pragma unroll
@@ -898,8 +916,8 @@ hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F
 using SweepFn = void (*)(DevParams, Fields, int);
 
 // x stores (ACC = 0), y and z accumulate (ACC = 1); MINW is the waves-per-SIMD
-// floor the register allocation must allow (2: no scheme spills; 3: WENO5
-// without eps fits, the others spill).
+// floor the register allocation must allow (2: no scheme spills; 3: they
+// spill).
 template <int RECON, int FVS, int MINW>
 static SweepFn pick_sweep(int dir)
 {
@@ -926,10 +944,10 @@ hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int 
 {
     SweepFn fn;
     const int fk = fvs_kind(P);
-    // 0: per scheme.  WENO5 without eps fits the 168 VGPRs of 3 waves per
-    // SIMD with no spill (y / z sweeps 1.22 against 1.24 ms at 256^3); the
-    // other schemes spill there and run 1.6-2.7x slower, so they keep 2
-    if (minw == 0) minw = P.recon == R_WENO5 && fk != FVS_SWE ? 3 : 2;
+    // 0: the default, 2.  At 3 every scheme spills since the y / z sweeps
+    // hold the next cell's accumulator (WENO5 at 3 without it: 1.22 ms per
+    // y / z sweep at 256^3; at 2 with it: 1.10 ms)
+    if (minw == 0) minw = 2;
     if (minw == 3)
         fn = fk == FVS_SW ? pick_sweep_r<FVS_SW, 3>(P.recon, dir)
              : fk == FVS_SWE ? pick_sweep_r<FVS_SWE, 3>(P.recon, dir) : pick_sweep_r<FVS_LF, 3>(P.recon, dir);
