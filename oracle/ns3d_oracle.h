@@ -10,12 +10,13 @@
  * includes <fftw3-mpi.h>, which is absent), so the restatement is pinned by
  * closed forms, invariants and convergence tests under tests/.
  *
- * Parity unpinned: the choice of tangential basis in the characteristic
- * path (orc_char_matrices, InviscousFlux_new.cpp:568-578).  The checks pin
- * L R = I, the eigen-decomposition of the flux Jacobian and the linear
- * schemes' equality with component-wise faces, none of which depends on
- * that choice; with WENO5 the choice changes the result, and it is
- * restated as written, unchecked.
+ * WENO5's constants (eps, 13/12, 1/4, the linear weights and the sub-stencil
+ * faces) are pinned by an exact rational derivation from the scheme's
+ * definition, and the characteristic path's tangential basis (the
+ * coordinate axes other than the normal, InviscousFlux_new.cpp:568-578) by
+ * its structure, by the exact zero face flux of a tangential velocity that
+ * is zero everywhere, and by a bitwise recomposition of the face from the
+ * matrices (tests/test_oracle_pins_exact.py).
  */
 #ifndef NS3D_ORACLE_H
 #define NS3D_ORACLE_H
