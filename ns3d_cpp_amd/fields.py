@@ -41,23 +41,44 @@ def hit_velocity(n, k0=4.0, u0=1.0, seed=1234):
     """
     rng = np.random.default_rng(seed)
     kf = np.fft.fftfreq(n, d=1.0 / n)
-    KZ, KY, KX = np.meshgrid(kf, kf, kf, indexing="ij")
+    # broadcast (z, y, x) wavenumber axes: the same elementwise arithmetic as
+    # full meshgrids, without their memory
+    KZ, KY, KX = kf[:, None, None], kf[None, :, None], kf[None, None, :]
     kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
     amp = np.zeros_like(kk)
     keep = (kk > 0) & (kk < n / 2)
     amp[keep] = np.sqrt(hit_spectrum(kk[keep], k0, u0) / (4.0 * np.pi * kk[keep] ** 2))
-    a = [rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape) for _ in range(3)]
+    del keep
+    a = []
+    for _ in range(3):
+        c = rng.standard_normal(kk.shape).astype(np.complex128)
+        c += 1j * rng.standard_normal(kk.shape)
+        a.append(c)
     k2 = np.where(kk > 0, kk ** 2, 1.0)
-    kdota = (KX * a[0] + KY * a[1] + KZ * a[2]) / k2
-    a[0] = a[0] - KX * kdota
-    a[1] = a[1] - KY * kdota
-    a[2] = a[2] - KZ * kdota
-    norm = np.sqrt(np.abs(a[0]) ** 2 + np.abs(a[1]) ** 2 + np.abs(a[2]) ** 2)
-    norm = np.where(norm > 0, norm, 1.0)
+    del kk
+    kdota = KX * a[0]
+    kdota += KY * a[1]
+    kdota += KZ * a[2]
+    kdota /= k2
+    del k2
+    a[0] -= KX * kdota
+    a[1] -= KY * kdota
+    a[2] -= KZ * kdota
+    del kdota
+    norm = np.abs(a[0]) ** 2
+    norm += np.abs(a[1]) ** 2
+    norm += np.abs(a[2]) ** 2
+    np.sqrt(norm, out=norm)
+    norm[norm == 0] = 1.0
     vel = []
     for c in range(3):
-        uh = amp * a[c] / norm
-        vel.append(np.real(np.fft.ifftn(uh)) * n ** 3)
+        uh = amp * a[c]
+        uh /= norm
+        a[c] = None
+        v = np.real(np.fft.ifftn(uh))
+        del uh
+        v *= n ** 3
+        vel.append(v)
     u, v, w = vel
     scale = u0 / np.sqrt(np.mean(u * u + v * v + w * w) / 3.0)
     return u * scale, v * scale, w * scale

@@ -1,6 +1,8 @@
-"""Flow monitors of the reference's post-processing, computed on the host from
-fields downloaded off the device (validation and history output; nothing here
-is on the time-stepping path).
+"""Checker of the device monitors: the reference's post-processing restated
+in numpy on fields downloaded off the device.  Test infrastructure, like the
+C oracle beside it: imported only by tests/ as the checker of the device
+spectrum, statistics and dissipation; the product computes its monitors on
+the device (ns3d_monitors.hip).
 
 * ``energy_spectrum`` restates ``compute_energy_spectrum_rank0``
   (EnergySpectrum.cpp:62-154) with numpy's FFT in place of FFTW: forward DFT

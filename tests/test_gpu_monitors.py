@@ -1,11 +1,17 @@
 """Device monitors against their host restatements: the energy spectrum and
-the turbulence statistics (numpy FFT, ns3d_cpp_amd.monitors, pinned by
+the turbulence statistics (numpy FFT, oracle/monitors.py, pinned by
 closed forms in test_monitors.py) and the residual diagnostics."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
-import ns3d_cpp_amd as ns
-from ns3d_cpp_amd import fields, monitors
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import monitors  # noqa: E402
+
+import ns3d_cpp_amd as ns  # noqa: E402
+from ns3d_cpp_amd import fields  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 

@@ -277,6 +277,41 @@ def test_nonphysical_detection(gpu):
     assert (i, j, k) == (ii, jj, kk)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("comp", [0, 4])
+def test_nan_is_reported_nonphysical(gpu, prec, comp):
+    """A NaN in one cell's density or energy: the step reports
+    NS3D_ERR_NONPHYSICAL at the first non-physical cell in (k, j, i) order
+    (NaN included).  The reference's scan (range_kutta.cpp:150-161) tests
+    rho <= 0 and p <= 0, which are false for NaN, so it would go on; the
+    oracle restates that and reports nothing."""
+    n = 12
+    st = rough_field(n, 9)
+    p, g = ns.make_params(n, Re=100.0, Ma=0.3, recon="WENO5", vis_scheme="C8th", precision=prec)
+    po, go = ns.make_params(n, Re=100.0, Ma=0.3, recon="WENO5", vis_scheme="C8th")
+    o = Oracle(po, go)
+    o.set_primitive(*st)
+    U = [o.interior(a).copy() for a in o.conserved()]
+    U[comp][6, 3, 9] = np.nan
+    o.set_conserved(*U)
+    status, _ = o.step(1e-4)
+    assert status == 0            # the reference's check misses NaN
+    s = ns.Solver(p, g)
+    s.upload_conserved(*U)
+    with pytest.raises(ns.NS3DError) as e:
+        s.rk3_step(1e-4)
+    assert e.value.status == ns._lib.NS3D_ERR_NONPHYSICAL
+    i, j, k = s.nonphysical_cell()
+    prim = s.primitive()
+    ng = s.ng
+    bad = ~(prim["rho"] > 0) | ~(prim["p"] > 0) | ~(prim["T"] > 0)
+    inter = np.zeros_like(bad)
+    inter[ng:-ng, ng:-ng, ng:-ng] = True
+    kk, jj, ii = np.argwhere(bad & inter)[0]
+    assert (i, j, k) == (ii, jj, kk)
+    s.close()
+
+
 def test_healthy_step_reports_ok(gpu):
     n = 12
     s, _ = pair(n, rough_field(n, 8), recon="WENO5", vis_scheme="C8th")

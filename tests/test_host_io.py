@@ -19,14 +19,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "ns3d_cpp_amd", "host")
 
 
-@pytest.fixture(scope="module")
-def tool(tmp_path_factory):
+# every test below runs twice: built plainly, and built under AddressSanitizer
+# and UndefinedBehaviorSanitizer (a report makes the harness exit non-zero)
+BUILDS = {"plain": [], "asan_ubsan": ["-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                                      "-fno-omit-frame-pointer", "-g"]}
+SAN_ENV = {"ASAN_OPTIONS": "detect_leaks=1", "UBSAN_OPTIONS": "print_stacktrace=1"}
+
+
+@pytest.fixture(scope="module", params=list(BUILDS))
+def tool(request, tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("io") / "host_io_check")
-    subprocess.run(["g++", "-O1", "-std=c++17", "-I", HOST, os.path.join(ROOT, "tests", "host_io_check.cpp"),
-                    os.path.join(HOST, "ns3d_io.cpp"), "-o", exe], check=True)
+    subprocess.run(["g++", "-O1", "-std=c++17", *BUILDS[request.param], "-I", HOST,
+                    os.path.join(ROOT, "tests", "host_io_check.cpp"), os.path.join(HOST, "ns3d_io.cpp"), "-o", exe],
+                   check=True)
+    env = {**os.environ, **SAN_ENV}
 
     def run(*args, cwd=None):
-        r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, cwd=cwd)
+        r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, cwd=cwd, env=env)
         assert r.returncode == 0, r.stdout + r.stderr
         return r.stdout
     return run

@@ -1,12 +1,15 @@
-"""Host-side monitors: the energy spectrum of EnergySpectrum.cpp:62-154 and
+"""The monitors' checker (oracle/monitors.py): the energy spectrum of EnergySpectrum.cpp:62-154 and
 the statistics of ana_isotropic.cpp:226-330, pinned by closed forms
 (single Fourier modes, Parseval), by a brute-force DFT on a tiny grid and by
 the shell rule (floor of |k|)."""
 import itertools
+import os
+import sys
 
 import numpy as np
 
-from ns3d_cpp_amd import monitors
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import monitors  # noqa: E402  (the monitors' checker, oracle/monitors.py)
 
 
 def grid(n):
