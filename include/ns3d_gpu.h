@@ -249,6 +249,22 @@ int ns3d_profile_entry(ns3d_ctx *ctx, int idx, char *name, int name_len,
  * Overwrites the stored gradient / viscous-flux scratch. */
 int ns3d_copy_probe(ns3d_ctx *ctx, int reps, double *gbps);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  compute_timestep (timestep.cpp:5-33) of caller-owned
+ * device arrays rho, u, v, w, p in the idx3 layout of (p, g) (sx*sy*sz
+ * doubles each; ghost values unused; p->gamma, p->cfl and the grid's
+ * extents are read, nothing needs finalizing), enqueued on `stream` (a
+ * hipStream_t; NULL for the null stream): *dt_dev (device memory) receives
+ * cfl * min over interior cells, bit for bit the value
+ * ns3d_compute_timestep_local gives for the same state.  Allocates nothing:
+ * its scratch is a static 16 KiB device array per device (2048 per-block
+ * minima), shared by every call on that device, so calls that overlap on
+ * one device race.  NS3D_ERR_INVALID_ARG for null pointers, non-positive
+ * extents or more than 2^31 - 1 cells per array. */
+int ns3d_cfl_dt_noalloc(const ns3d_params *p, const ns3d_grid *g, const double *rho, const double *u,
+                        const double *v, const double *w, const double *pr, double *dt_dev, void *stream);
+
 /* ---- Differentiation (no counterpart in the reference, which has no
  * derivative code).  FP64 contexts on a periodic box with a component-wise
  * WENO5 / linear / MDCD / UD7 / UD7L reconstruction and every grid extent

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ns3d_advance": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "ns3d_advance_cfl": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "ns3d_compute_timestep_local": (C.c_int, [C.c_void_p, DP]),
+    "ns3d_cfl_dt_noalloc": (C.c_int, [C.POINTER(Params), C.POINTER(Grid)] + [C.c_void_p] * 7),
     "ns3d_total_energy": (C.c_int, [C.c_void_p, DP]),
     "ns3d_sync": (C.c_int, [C.c_void_p]),
     "ns3d_stream": (C.c_void_p, [C.c_void_p]),

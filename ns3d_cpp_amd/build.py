@@ -25,8 +25,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_monitors.hip", "ns3d_api.cpp",
-           "ns3d_params.cpp"]
+SOURCES = ["ns3d_kernels.hip", "ns3d_mixed.hip", "ns3d_adjoint.hip", "ns3d_monitors.hip", "ns3d_noalloc.hip",
+           "ns3d_api.cpp", "ns3d_params.cpp"]
 # the mixed-precision path and the derivative kernels do not reproduce the
 # reference's rounding, so their translation units contract a*b+c into FMAs
 # within an expression (-ffp-contract=on).  Not "fast": that lets the backend
