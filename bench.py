@@ -78,12 +78,13 @@ def make(ns, n, Ma, Re, recon, precision):
                           vis_scheme="C8th", sutherland_s=110.4 / 273.15, precision=precision)
 
 
-def traffic_from_profiles(prec, kernel):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
+def traffic_from_profiles(prec, kernel, n, recon="WENO5"):
+    """HBM bytes per launch of `kernel` at n^3 measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration) and
-    committed under profiles/; None if not recorded."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{prec}_hit256.json")
-    if not os.path.exists(path):
+    committed under profiles/traffic_<prec>_hit<n>.json; None where no
+    profile of that size and scheme exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{prec}_hit{n}.json")
+    if recon != "WENO5" or not os.path.exists(path):
         return None
     try:
         return json.load(open(path)).get(kernel)
@@ -129,7 +130,7 @@ def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
     dom = max(rep, key=lambda k: rep[k][0])
     dom_avg = rep[dom][0] / rep[dom][1]
     achieved = alg.get(dom, 0) * n ** 3 / (dom_avg * 1e-3) / 1e9
-    traffic = traffic_from_profiles(prec, dom)
+    traffic = traffic_from_profiles(prec, dom, n, recon)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
     if prec == "fp64" and dom.startswith("sweep"):
@@ -229,25 +230,56 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
             "kernels_ms": {k: round(v[0], 2) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_oracle(ns, Oracle, n, state, nsteps, dt, **cfg):
+    p, g = ns.make_params(n, **cfg)
+    o = Oracle(p, g)
+    o.set_primitive(*state)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.step(dt)       # dt <= 0: the CFL dt, as the GPU run
+    t = time.perf_counter() - t0
+    o.close()
+    return n ** 3 * nsteps / t, t
+
+
 def cpu_baseline(n, nsteps, Re, Mt, k0, seed):
-    """The CPU FP64 oracle on the host's cores, on a bounded sample."""
+    """The CPU FP64 oracle on the host's cores, on bounded samples: the
+    benchmark's own workload at n^3 (the reported value), and the 64^3
+    Taylor-Green vortex and the 128^3 MDCD + Lax-Friedrichs turbulence
+    workloads beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ns3d_cpp_amd as ns
     from ns3d_cpp_amd import fields
     from oracle import Oracle, load
+    cores = int(load().orc_num_threads())
     st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=seed)
-    p, g = make(ns, n, Ma, Re, "WENO5", "fp64")
-    o = Oracle(p, g)
-    o.set_primitive(*st)
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        o.step(0.0)      # CFL dt, as the GPU run
-    t = time.perf_counter() - t0
-    o.close()
-    return {"value": n ** 3 * nsteps / t, "unit": "cell-updates/s", "cores": int(load().orc_num_threads()),
-            "kind": "port",
+    v, t = _time_oracle(ns, Oracle, n, st, nsteps, 0.0, Re=Re, Ma=Ma, Pr=0.72, cfl=0.5, recon="WENO5",
+                        fvs_type="StagerWarming", vis_scheme="C8th", sutherland_s=110.4 / 273.15)
+    del st
+    legs = []
+    v64, t64 = _time_oracle(ns, Oracle, 64, fields.tgv(64, Ma=0.1), 20, 1e-3, Re=1600.0, Ma=0.1, Pr=0.72,
+                            recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th")
+    legs.append({"workload": "Taylor-Green vortex 64^3, WENO5 + Steger-Warming + C8th, dt 1e-3",
+                 "value": v64, "steps": 20, "seconds": round(t64, 2)})
+    st2, Ma2 = fields.hit(128, k0=4, Mt=0.3, seed=seed)
+    v128, t128 = _time_oracle(ns, Oracle, 128, st2, 5, 0.0, Re=100.0, Ma=Ma2, Pr=0.72, cfl=0.5, recon="MDCD",
+                              fvs_type="LaxFriedrichs", vis_scheme="C8th", sutherland_s=110.4 / 273.15)
+    legs.append({"workload": "isotropic turbulence 128^3, MDCD + Lax-Friedrichs + C8th, CFL 0.5",
+                 "value": v128, "steps": 5, "seconds": round(t128, 2)})
+    return {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{nsteps} RK3 step(s) of the same isotropic-turbulence workload at {n}^3 "
-                      f"(WENO5+SW+C8, CFL dt) with the C oracle, OpenMP threads = cores; {t:.1f} s"}
+                      f"(WENO5+SW+C8, CFL dt) with the C oracle, OpenMP threads = cores; {t:.1f} s",
+            "legs": legs}
 
 
 def main():
@@ -270,6 +302,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--adj-only", action="store_true", help="only the adjoint leg (for profiling it)")
+    ap.add_argument("--no-other-scheme", dest="other_scheme", action="store_false",
+                    help="skip the timing of the workload's other scheme (UD7 beside WENO5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -286,14 +320,35 @@ def main():
         print(json.dumps({"adjoint": adj}), flush=True)
         return
     st, Ma = fields.hit(n, k0=k0, Mt=Mt, seed=args.seed)
-    # achievable streaming bandwidth on this device
+    # achievable streaming bandwidth on this device: a grid-stride 16-byte
+    # copy and a flat nontemporal one over 12 cell arrays
     p, g = make(ns, n, Ma, Re, args.recon, "fp64")
     probe = ns.Solver(p, g)
+    probe.profile(True)
     copy_gbps = probe.copy_probe(10)
+    prep = probe.profile_report()
+    cbytes = 2 * 8 * 12 * ns.lib().ns3d_cells(probe.ctx)
+    stream = {"best_GBps": round(copy_gbps, 1)}
+    for kname in ("copy", "copy_nt"):
+        if kname in prep:
+            stream[kname + "_GBps"] = round(cbytes / (prep[kname][0] / prep[kname][1] * 1e-3) / 1e9, 1)
+    probe.profile(False)
     probe.close()
 
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     res = {prec: time_path(ns, H, n, st, Ma, Re, args.recon, prec, args.steps, args.warmup) for prec in precs}
+    # the other scheme of the metric's workload: UD7 (or WENO5 when UD7 ran)
+    other = "UPWIND_7TH" if args.recon == "WENO5" else "WENO5"
+    legs = {}
+    if args.other_scheme:
+        leg = {}
+        for prec in precs:
+            r = time_path(ns, H, n, st, Ma, Re, other, prec, args.steps, args.warmup)
+            leg[prec] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "step_model": r["step_model"],
+                         "roofline": r["roofline"]}
+        if "mixed" in leg and "fp64" in leg:
+            leg["speedup_mixed_vs_fp64"] = round(leg["mixed"]["value"] / leg["fp64"]["value"], 3)
+        legs[other] = leg
     err = None
     if args.precision == "both" and args.err_steps > 0:
         err = mixed_error(ns, n, st, Ma, Re, args.recon, args.err_steps)
@@ -330,9 +385,13 @@ def main():
         "step_model": h["step_model"],
         "kernel_bytes_per_cell_update": h["kernel_bytes_per_cell_update"],
         "kernel_bytes_hbm_frac": h["kernel_bytes_hbm_frac"],
-        "copy_probe_GBps": round(copy_gbps, 1),
+        "stream_probe": stream,
         "kernels": h["kernels"],
     }
+    line["roofline"]["stream_GBps"] = stream["best_GBps"]
+    line["roofline"]["frac_of_stream"] = round(line["roofline"]["achieved"] / stream["best_GBps"], 4)
+    if legs:
+        line["other_scheme"] = legs
     if head == "mixed" and "fp64" in res:
         line["fp64"] = res["fp64"]
         line["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)

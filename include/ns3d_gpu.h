@@ -244,8 +244,9 @@ int ns3d_profile_count(ns3d_ctx *ctx);
 int ns3d_profile_entry(ns3d_ctx *ctx, int idx, char *name, int name_len,
                        double *total_ms, long long *launches);
 /* Streaming-copy probe over the context's scratch arrays (12 cell arrays,
- * read + write): achieved GB/s of `reps` copies.  No reference counterpart;
- * it measures the achievable HBM bandwidth of the device the context is on.
+ * read + write): the better achieved GB/s of `reps` copies by a grid-stride
+ * 16-byte copy and by a flat nontemporal one.  No reference counterpart; it
+ * measures the achievable HBM bandwidth of the device the context is on.
  * Overwrites the stored gradient / viscous-flux scratch. */
 int ns3d_copy_probe(ns3d_ctx *ctx, int reps, double *gbps);
 

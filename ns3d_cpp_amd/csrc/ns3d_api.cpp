@@ -1272,16 +1272,28 @@ int ns3d_copy_probe(ns3d_ctx *c, int reps, double *gbps)
     hipEvent_t a = nullptr, b = nullptr;
     HIPCALL(c, hipEventCreate(&a));
     HIPCALL(c, hipEventCreate(&b));
-    HIPCALL(c, launch_copy(c->stream, c->f.G[0], c->f.VF[0], n));
-    HIPCALL(c, hipEventRecord(a, c->stream));
-    for (int r = 0; r < reps; ++r) TIMED(c, "copy", launch_copy(c->stream, c->f.G[0], c->f.VF[0], n));
-    HIPCALL(c, hipEventRecord(b, c->stream));
-    HIPCALL(c, hipEventSynchronize(b));
-    float ms = 0.0f;
-    HIPCALL(c, hipEventElapsedTime(&ms, a, b));
+    // two kernels: the grid-stride copy (k_copy, which also calibrates the
+    // traffic counters) and the flat nontemporal copy (k_copy_nt); the
+    // better rate is returned
+    double best = 0.0;
+    for (int kind = 0; kind < 2; ++kind) {
+        const auto copy = [&]() {
+            return kind == 0 ? launch_copy(c->stream, c->f.G[0], c->f.VF[0], n)
+                             : launch_copy_nt(c->stream, c->f.G[0], c->f.VF[0], n);
+        };
+        HIPCALL(c, copy());
+        HIPCALL(c, hipEventRecord(a, c->stream));
+        for (int r = 0; r < reps; ++r) TIMED(c, kind == 0 ? "copy" : "copy_nt", copy());
+        HIPCALL(c, hipEventRecord(b, c->stream));
+        HIPCALL(c, hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCALL(c, hipEventElapsedTime(&ms, a, b));
+        const double g = 2.0 * 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+        if (g > best) best = g;
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    *gbps = 2.0 * 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    *gbps = best;
     return NS3D_OK;
 }
 

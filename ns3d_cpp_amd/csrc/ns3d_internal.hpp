@@ -87,6 +87,7 @@ hipError_t launch_split_batch(hipStream_t s, const DevParams &P, long long n, co
 hipError_t launch_recon_batch(hipStream_t s, const DevParams &P, long long n, int width, const double *st,
                               double flag, double *out);
 hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n);
+hipError_t launch_copy_nt(hipStream_t s, const double *src, double *dst, long n);
 int ke_blocks(const DevParams &P);
 hipError_t launch_m_from_prims(hipStream_t s, const DevParams &P, const MFields &M, const double *const *src,
                                int write_u);

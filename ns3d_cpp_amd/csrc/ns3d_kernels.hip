@@ -1166,6 +1166,31 @@ hipError_t launch_cfl(hipStream_t s, const DevParams &P, const Fields &F, unsign
     return hipGetLastError();
 }
 
+// The same copy with one 16-byte element per thread (a flat grid) and
+// nontemporal loads and stores: the best streaming rate measured on the
+// device, reported beside the 8 TB/s peak.
+__global__ __launch_bounds__(256) void k_copy_nt(const double2 *__restrict__ src, double2 *__restrict__ dst, long n2)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) {
+        double2 v;
+        v.x = __builtin_nontemporal_load(&src[t].x);
+        v.y = __builtin_nontemporal_load(&src[t].y);
+        __builtin_nontemporal_store(v.x, &dst[t].x);
+        __builtin_nontemporal_store(v.y, &dst[t].y);
+    }
+}
+
+hipError_t launch_copy_nt(hipStream_t s, const double *src, double *dst, long n)
+{
+    const long n2 = n / 2;
+    const long blocks = (n2 + 255) / 256;
+    if (blocks > 2147483647L) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_nt, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst), n2);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(hipStream_t s, const double *src, double *dst, long n)
 {
     const long n2 = n / 2;

@@ -40,3 +40,22 @@ def test_byte_model():
     assert abs(step(mx, 0) - (3 * (40 + 60 + 60 + 52) + 500)) < 1e-9
     # the metric's model keeps the CFL read in both (2408 + 40, 1544 + 20)
     assert b.STEP_MODEL_BYTES == {"fp64": 2448, "mixed": 1564}
+
+
+def test_traffic_record_is_keyed_by_size_and_scheme():
+    """The roofline's measured traffic comes from the profile of the same
+    size and scheme, and is null where none was recorded."""
+    b = load_bench()
+    assert isinstance(b.traffic_from_profiles("mixed", "vdiv_stage", 256), int)
+    assert b.traffic_from_profiles("mixed", "vdiv_stage", 256, "UPWIND_7TH") is None
+    assert b.traffic_from_profiles("mixed", "vdiv_stage", 96) is None
+
+
+def test_prof_kinds_and_stages():
+    sys.path.insert(0, ROOT)
+    from ns3d_cpp_amd import prof
+    assert prof.kind_of("void ns3d::km_vdiv_z<0, 2, 0>(ns3d::DevParams, ns3d::MFields, double)") == "vdiv_stage"
+    assert prof.stage_of("void ns3d::km_vdiv_z<0, 2, 0>(ns3d::DevParams)") == 2
+    assert prof.stage_of("void ns3d::km_vdiv_z<0, 0, 0>(ns3d::DevParams)") is None
+    assert prof.kind_of("ns3d::k_copy_nt(double2 const*, double2*, long)") == "copy_nt"
+    assert prof.kind_of("ns3d::k_copy(double2 const*, double2*, long)") == "copy"
