@@ -218,6 +218,23 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
     mem = s.adj_memory()
     base = ns.lib().ns3d_cells(s.ctx) * 40 * 8
     gnorm = float(np.sqrt(sum(np.sum(s.interior(a) ** 2) for a in lam0)))
+    # the same gradient with every stage state of the run kept in HBM (one
+    # checkpoint segment: no recomputation), where the device memory allows
+    s.upload_conserved(*U0)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    J1 = s.adj_forward(nsteps, dt, nsteps)
+    H.hipEventRecord(ev[1], s.stream())
+    lam1, _ = s.adj_backward()
+    H.hipEventRecord(ev[2], s.stream())
+    H.hipEventSynchronize(ev[2])
+    a_f, a_b = C.c_float(), C.c_float()
+    H.hipEventElapsedTime(C.byref(a_f), ev[0], ev[1])
+    H.hipEventElapsedTime(C.byref(a_b), ev[1], ev[2])
+    same = J1 == J and all(np.array_equal(s.interior(a), s.interior(b)) for a, b in zip(lam0, lam1))
+    store_all = {"ckpt_every": nsteps, "adj_forward_ms": round(a_f.value, 3), "adj_backward_ms": round(a_b.value, 3),
+                 "cost_vs_forward": round((a_f.value + a_b.value) / fwd.value, 3),
+                 "device_GB": round((s.adj_memory() + base) / 1e9, 3), "gradient_bitwise_equal": bool(same)}
     s.close()
     total = t_f.value + t_b.value
     return {"config": f"isotropic turbulence {n}^3 FP64, k0=4, Mt=0.3, Re=100, WENO5+SW+C8, {nsteps} RK3 steps, "
@@ -227,6 +244,7 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
             "adj_backward_ms": round(t_b.value, 3), "adjoint_total_ms": round(total, 3),
             "cost_vs_forward": round(total / fwd.value, 3),
             "device_bytes": int(mem + base), "device_GB": round((mem + base) / 1e9, 3),
+            "store_all_stages": store_all,
             "kernels_ms": {k: round(v[0], 2) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}}
 
 

@@ -371,8 +371,12 @@ __global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, Adj
     const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
     const int tx = threadIdx.x, q = threadIdx.y;
     const int L = n * TX;                // one LDS plane
-    double *spl = lds;                   // [n][TX] split values of one side
-    double *con = lds + L;               // [S][n][TX] reverse contributions
+    // two buffers of [n][TX] split values of one side + [S][n][TX] reverse
+    // contributions, alternating between the ten (component, side) passes:
+    // a pass writes the buffer the previous pass did not touch, so it needs
+    // no barrier after its gather (two per pass instead of three)
+    double *const buf0 = lds;
+    double *const buf1 = lds + (size_t)(1 + S) * L;
     int bx, by, bz;
     xcd_block3(bx, by, bz);
     int base, st, ncols;
@@ -405,6 +409,8 @@ __global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, Adj
         const double gF = (A.X[m][idn] - A.X[m][id]) * inv_d;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
+            double *const spl = side == 0 ? buf0 : buf1;
+            double *const con = spl + L;
             if (act) spl[me] = side == 0 ? fp[m] : fm[m];
             __syncthreads();
             if (act) {
@@ -432,7 +438,6 @@ __global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, Adj
                 if (side == 0) lp[m] = a;
                 else lm[m] = a;
             }
-            __syncthreads();
         }
     }
     if (!act) return;
@@ -744,15 +749,25 @@ hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, co
 
 // Lines per block of the whole-line adjoint sweep along `dir` (0: the sweep
 // falls back to the per-component marching kernel): as many neighbouring
-// x-columns as fit 1024 threads and 64 KiB of LDS for the widest stencil.
+// x-columns as fit 1024 threads and half the device's LDS (two workgroups
+// per CU) for the scheme's stencil (two buffers of 1 + S planes per line).
+// A whole line must fit the device's LDS.  At 128^3 on the MI355X (160 KiB)
+// that is 4 lines per block; 8 lines at one workgroup per CU ran the y sweep
+// in 0.24 ms against 0.21 ms.
+static long long adj_line_bytes(const DevParams &P, int n)
+{
+    const int S = P.recon == R_UD7 || P.recon == R_UD7L ? 8 : P.recon == R_LINEAR ? 2 : 6;
+    return 2LL * (1 + S) * n * (long long)sizeof(double);
+}
+
 int adj_line_tx(const DevParams &P, int dir)
 {
     const int n = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
-    const long long per_line = (long long)(1 + Stencil<R_UD7>::S) * n * sizeof(double);
-    if (!P.adj_lines || n > 1024 || per_line > 65536) return 0;
+    const long long per_line = adj_line_bytes(P, n);
+    if (!P.adj_lines || n > 1024 || per_line > P.lds_max) return 0;
     if (dir == 0) return 1;
     int tx = 16;
-    while (tx > 1 && ((long long)tx * n > 1024 || tx * per_line > 65536)) tx >>= 1;
+    while (tx > 1 && ((long long)tx * n > 1024 || tx * per_line > P.lds_max / 2)) tx >>= 1;
     return tx;
 }
 
@@ -764,11 +779,19 @@ static void adj_sweep_r(hipStream_t s, const DevParams &P, const Fields &F, cons
 {
     const int tx = adj_line_tx(P, dir);
     if (tx > 0) {
-        constexpr int S = Stencil<RECON>::S;
         const int n = dir == 0 ? P.nx : dir == 1 ? P.ny : P.nz;
-        const size_t lds = (size_t)(1 + S) * n * tx * sizeof(double);
+        const size_t lds = (size_t)adj_line_bytes(P, n) * tx;
         const dim3 blk(tx, n);
         const int store = dir == 0 ? 1 : 0;
+        if (lds > 65536) {
+            // beyond the default dynamic LDS limit of a launch
+            (void)hipFuncSetAttribute((const void *)ka_sweep_line<0, RECON, FVS>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            (void)hipFuncSetAttribute((const void *)ka_sweep_line<1, RECON, FVS>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            (void)hipFuncSetAttribute((const void *)ka_sweep_line<2, RECON, FVS>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        }
         if (dir == 0)
             hipLaunchKernelGGL((ka_sweep_line<0, RECON, FVS>), dim3(P.ny, P.nz), blk, lds, s, P, F, A, store);
         else if (dir == 1)

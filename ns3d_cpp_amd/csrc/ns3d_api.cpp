@@ -710,6 +710,18 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     for (int a = 0; a < 3; ++a)
         d.periodic[a] = p.bc[2 * a] == NS3D_BC_PERIODIC && p.bc[2 * a + 1] == NS3D_BC_PERIODIC;
     for (int f = 0; f < 6; ++f) d.bc[f] = p.bc[f];
+    // LDS per workgroup of this device (the whole-row and whole-line kernels
+    // size their tiles from it)
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess || lds <= 0)
+            lds = 65536;
+        d.lds_max = lds;
+        if (const char *env = std::getenv("NS3D_LDS_MAX")) {
+            const int v = std::atoi(env);
+            if (v > 0 && v < d.lds_max) d.lds_max = v;
+        }
+    }
 
     e = hipMalloc((void **)&c->slab, (size_t)ncell * kCellArrays * sizeof(double));
     if (e != hipSuccess) { int r = hip_fail(c, e, "hipMalloc fields"); ns3d_destroy(c); return r; }

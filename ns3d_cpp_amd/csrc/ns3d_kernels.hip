@@ -986,10 +986,11 @@ static size_t xrow_lds_bytes(const DevParams &P)
     return (size_t)(10 * (P.nx + S - 1) + 5 * P.nx) * sizeof(double);
 }
 
-// full-row blocks need a periodic x axis and a row that fits in 64 KiB of LDS
+// full-row blocks need a periodic x axis and a row that fits in the LDS of
+// one workgroup
 bool xrow_supported(const DevParams &P)
 {
-    return P.periodic[0] && P.nx <= 1024 && xrow_lds_bytes(P) <= 65536;
+    return P.periodic[0] && P.nx <= 1024 && xrow_lds_bytes(P) <= (size_t)P.lds_max;
 }
 
 hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F)
@@ -1000,6 +1001,7 @@ hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F)
     if (!fn || !xrow_supported(P)) return hipErrorInvalidValue;
     const size_t lds = xrow_lds_bytes(P);
     const int threads = ((P.nx + 63) / 64) * 64;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, F);
     return hipGetLastError();
 }

@@ -1179,13 +1179,14 @@ static void m_xrow_shape(const DevParams &P, int &threads, size_t &lds)
     }
 }
 
-// the mixed x sweep holds whole rows (1024 threads, 64 KiB of LDS at most)
+// the mixed x sweep holds whole rows (1024 threads, the LDS of one
+// workgroup at most)
 bool m_xrow_supported(const DevParams &P)
 {
     int threads;
     size_t lds;
     m_xrow_shape(P, threads, lds);
-    return threads <= 1024 && lds <= 65536;
+    return threads <= 1024 && lds <= (size_t)P.lds_max;
 }
 
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len)
@@ -1202,7 +1203,8 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
         int threads;
         size_t lds;
         m_xrow_shape(P, threads, lds);
-        if (!fn || threads > 1024 || lds > 65536) return hipErrorInvalidValue;
+        if (!fn || threads > 1024 || lds > (size_t)P.lds_max) return hipErrorInvalidValue;
+        if (lds > 65536) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(fn, dim3(P.ny, P.nz), dim3(threads), lds, s, P, M);
         return hipGetLastError();
     }
