@@ -221,7 +221,9 @@ int ns3d_split_batch(const ns3d_params *p, long long n, const double *const W[5]
 /* reconstruct_select (reconstruction.cpp:13-85, ns3d_func.h:96) of n
  * stencils of `width` values each (row-major), reversed for flag < 0.
  * NS3D_ERR_CONFIG when width is not the scheme's stencil (the reference
- * throws std::runtime_error); other schemes take its averaging fallback. */
+ * throws std::runtime_error); other schemes take its averaging fallback,
+ * except with width 2 and flag < 0, where the reference reads past the
+ * stencil: NS3D_ERR_CONFIG. */
 int ns3d_reconstruct_batch(const ns3d_params *p, long long n, int width, const double *stencils, double flag,
                            double *out);
 

@@ -1595,6 +1595,9 @@ int ns3d_reconstruct_batch(const ns3d_params *p, long long n, int width, const d
     default: break;
     }
     if (need && width != need) return NS3D_ERR_CONFIG;
+    // the fallback (reconstruction.cpp:77-84) on the minus side reads one
+    // value past a two-value stencil in the reference (undefined): refused
+    if (!need && flag < 0.0 && width == 2) return NS3D_ERR_CONFIG;
     if (n == 0) return NS3D_OK;
     DevBuf buf;
     const size_t sb = (size_t)n * width * sizeof(double), ob = (size_t)n * sizeof(double);

@@ -235,7 +235,7 @@ bool write_restart_file(Field3D &F, const GridDesc &G, const SolverParams &P, co
     }
     const double *a[5] = {F.rho.data(), kind ? F.rhou.data() : F.u.data(), kind ? F.rhov.data() : F.v.data(),
                           kind ? F.rhow.data() : F.w.data(), kind ? F.E.data() : F.p.data()};
-    return write_restart(path, RestartInfo{F.nx, F.ny, F.nz, kind, F.step, F.time}, zone_of(F, G), a);
+    return write_restart(path, RestartInfo{F.nx, F.ny, F.nz, kind, F.step, F.time, F.next_output}, zone_of(F, G), a);
 }
 
 bool read_restart_file(Field3D &F, const GridDesc &G, const std::string &path)
@@ -251,6 +251,7 @@ bool read_restart_file(Field3D &F, const GridDesc &G, const std::string &path)
         check(ns3d_upload_primitive(F.ctx, a[0], a[1], a[2], a[3], a[4]), "upload_primitive", &F);
     F.step = info.step;
     F.time = info.time;
+    F.next_output = info.next_output;
     return true;
 }
 
@@ -262,10 +263,17 @@ void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
     double current_time = F.time;
     double output_Timefreq = P.output_timefreq;
     const double TotalTime = P.total_time;
-    // a restarted run resumes the output schedule by the same repeated
-    // additions an unbroken run makes
-    while (F.time > 0.0 && output_Timefreq <= current_time + 1e-12 && P.output_timefreq > 0.0)
-        output_Timefreq += P.output_timefreq;
+    if (std::isfinite(F.next_output)) {
+        // a restarted run takes the schedule where the run that wrote the
+        // file left it (an output due at the restart time is still owed)
+        output_Timefreq = F.next_output;
+    } else {
+        // a field without the schedule: resume it by the same repeated
+        // additions an unbroken run makes
+        while (F.time > 0.0 && output_Timefreq <= current_time + 1e-12 && P.output_timefreq > 0.0)
+            output_Timefreq += P.output_timefreq;
+    }
+    F.next_output = output_Timefreq;
     int step = (int)F.step + 1;
     for (; step <= P.max_steps; ++step) {
         double dt = P.dt_fixed;
@@ -313,6 +321,7 @@ void time_advance(Field3D &F, GridDesc &G, SolverParams &P)
         }
         F.step = step;
         F.time = current_time;
+        F.next_output = output_Timefreq;
         if (current_time >= TotalTime) break;
     }
     const double total = std::chrono::duration<double>(clk::now() - t_start).count();

@@ -8,6 +8,7 @@
 // its host arrays (maincycle.cpp:62-102).
 #pragma once
 
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -40,6 +41,9 @@ struct Field3D {
     // them and leaves its last step and time here (0, 0 for a fresh field)
     long long step = 0;
     double time = 0.0;
+    // the output time the schedule waits for after that step (maincycle.cpp:
+    // 33-45); NaN when unknown (a fresh field, or a restart file without it)
+    double next_output = std::nan("");
 
     Field3D() = default;
     Field3D(const Field3D &) = delete;
@@ -95,7 +99,8 @@ bool initialize_from_tecplot_downsample(Field3D &F, const GridDesc &G, const Sol
                                         const std::string &filename, int src_nx, int src_ny, int src_nz);
 // Binary restart of the context's canonical state (primitives in FP64 mode,
 // conserved variables in mixed mode) with F.step and F.time; reading uploads
-// the state (boundary fill included) and sets F.step and F.time.
+// the state (boundary fill included) and sets F.step, F.time and
+// F.next_output.
 bool write_restart_file(Field3D &F, const GridDesc &G, const SolverParams &P, const std::string &path);
 bool read_restart_file(Field3D &F, const GridDesc &G, const std::string &path);
 // time_advance (maincycle.cpp:7-113): dt fixed or CFL, dt clipping to the

@@ -294,7 +294,8 @@ bool write_residual_line(const std::string &path, int step, const double res[5],
 }
 
 namespace {
-const char kRestartMagic[8] = {'N', 'S', '3', 'D', 'R', 'S', 'T', '1'};
+const char kRestartMagic[8] = {'N', 'S', '3', 'D', 'R', 'S', 'T', '2'};
+const char kRestartMagicV1[8] = {'N', 'S', '3', 'D', 'R', 'S', 'T', '1'};
 }
 
 bool write_restart(const std::string &path, const RestartInfo &info, const ZoneDesc &Z, const double *const a[5])
@@ -309,7 +310,8 @@ bool write_restart(const std::string &path, const RestartInfo &info, const ZoneD
     const int64_t step = info.step;
     bool ok = std::fwrite(kRestartMagic, 1, 8, f.get()) == 8 && std::fwrite(dims, sizeof(int32_t), 4, f.get()) == 4 &&
               std::fwrite(&step, sizeof(step), 1, f.get()) == 1 &&
-              std::fwrite(&info.time, sizeof(double), 1, f.get()) == 1;
+              std::fwrite(&info.time, sizeof(double), 1, f.get()) == 1 &&
+              std::fwrite(&info.next_output, sizeof(double), 1, f.get()) == 1;
     for (int q = 0; q < 5 && ok; ++q)
         for (int k = 0; k < Z.nz && ok; ++k)
             for (int j = 0; j < Z.ny && ok; ++j)
@@ -328,10 +330,13 @@ bool read_restart(const std::string &path, RestartInfo &info, const ZoneDesc &Z,
     char magic[8];
     int32_t dims[4];
     int64_t step;
-    double time;
-    if (std::fread(magic, 1, 8, f.get()) != 8 || std::memcmp(magic, kRestartMagic, 8) != 0 ||
-        std::fread(dims, sizeof(int32_t), 4, f.get()) != 4 || std::fread(&step, sizeof(step), 1, f.get()) != 1 ||
-        std::fread(&time, sizeof(double), 1, f.get()) != 1) {
+    double time, next_output = std::nan("");
+    const bool head = std::fread(magic, 1, 8, f.get()) == 8;
+    const bool v2 = head && std::memcmp(magic, kRestartMagic, 8) == 0;
+    const bool v1 = head && std::memcmp(magic, kRestartMagicV1, 8) == 0;
+    if (!(v1 || v2) || std::fread(dims, sizeof(int32_t), 4, f.get()) != 4 ||
+        std::fread(&step, sizeof(step), 1, f.get()) != 1 || std::fread(&time, sizeof(double), 1, f.get()) != 1 ||
+        (v2 && std::fread(&next_output, sizeof(double), 1, f.get()) != 1)) {
         std::cerr << "Error: " << path << " is not a restart file\n";
         return false;
     }
@@ -347,7 +352,7 @@ bool read_restart(const std::string &path, RestartInfo &info, const ZoneDesc &Z,
                     std::cerr << "Error: restart file " << path << " is truncated\n";
                     return false;
                 }
-    info = RestartInfo{dims[0], dims[1], dims[2], dims[3], (long long)step, time};
+    info = RestartInfo{dims[0], dims[1], dims[2], dims[3], (long long)step, time, next_output};
     return true;
 }
 

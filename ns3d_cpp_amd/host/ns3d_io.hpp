@@ -69,14 +69,18 @@ bool write_residual_line(const std::string &path, int step, const double res[5],
 
 // Binary restart file (no counterpart in the reference, whose only restart
 // path is the 8-digit Tecplot reader).  Little-endian: the 8-byte magic
-// "NS3DRST1", int32 nx, ny, nz, kind, int64 step, double time, then five
-// interior arrays (x fastest) of doubles: kind 0 = rho, u, v, w, p (the
-// canonical state of the FP64 mode), kind 1 = rho, rho u, rho v, rho w, E
-// (that of the mixed mode), so a run continues bit for bit.
+// "NS3DRST2", int32 nx, ny, nz, kind, int64 step, double time, double
+// next_output (the output time the schedule of maincycle.cpp:33-45 waits
+// for), then five interior arrays (x fastest) of doubles: kind 0 = rho, u,
+// v, w, p (the canonical state of the FP64 mode), kind 1 = rho, rho u,
+// rho v, rho w, E (that of the mixed mode), so a run continues bit for bit,
+// output schedule included.  Files of the earlier "NS3DRST1" layout (no
+// next_output) are read with next_output = NaN (unknown).
 struct RestartInfo {
     int nx, ny, nz, kind;
     long long step;
     double time;
+    double next_output;
 };
 bool write_restart(const std::string &path, const RestartInfo &info, const ZoneDesc &Z, const double *const a[5]);
 // reads into the interior of padded arrays; false (with a message) on a

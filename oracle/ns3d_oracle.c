@@ -544,10 +544,17 @@ double orc_reconstruct(const double *st, int n, double flag, const ns3d_params *
     case NS3D_RECON_UPWIND_7TH: need = 8; break;
     case NS3D_RECON_UD7L: need = 8; break;
     default:
-        /* fallback of reconstruction.cpp:77-84 (reached for WCNS) */
+        /* fallback of reconstruction.cpp:77-84 (reached for WCNS).  With two
+         * values and flag < 0 the reference reads st[2], past the stencil
+         * (undefined); that case is reported as an error, as the GPU entry
+         * point refuses it. */
         if (n >= 2) {
             int mid = n / 2;
             if (flag >= 0.0) return 0.5 * (st[mid] + st[mid - 1]);
+            if (mid + 1 >= n) {
+                if (err) *err = 1;
+                return NAN;
+            }
             return 0.5 * (st[mid] + st[mid + 1]);
         }
         return n == 0 ? 0.0 : st[0];

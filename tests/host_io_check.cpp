@@ -9,7 +9,7 @@
 //   stats <spec> <out> time ke tke u_rms taylor_Li Re_lambda Mach_t mu rho
 //   resid <out> step r0 r1 r2 r3 r4 Etot
 //   paths time                       tecplot_field_path and time_prefix
-//   rwrite <out> nx ny nz ng kind step time   pattern field -> write_restart
+//   rwrite <out> nx ny nz ng kind step time next_output   pattern field -> write_restart
 //   rread <in> nx ny nz ng <raw>     read_restart -> header on stdout, padded arrays raw
 #include <cmath>
 #include <cstdio>
@@ -100,7 +100,7 @@ int main(int argc, char **argv)
         std::fclose(f);
         return 0;
     }
-    if ((cmd == "rwrite" && argc == 10) || (cmd == "rread" && argc == 8)) {
+    if ((cmd == "rwrite" && argc == 11) || (cmd == "rread" && argc == 8)) {
         ZoneDesc Z{};
         Z.nx = std::atoi(argv[3]); Z.ny = std::atoi(argv[4]); Z.nz = std::atoi(argv[5]); Z.ng = std::atoi(argv[6]);
         Z.sx = Z.nx + 2 * Z.ng; Z.sy = Z.ny + 2 * Z.ng;
@@ -115,13 +115,15 @@ int main(int argc, char **argv)
                         const size_t id = ((size_t)(k + Z.ng) * Z.sy + (j + Z.ng)) * Z.sx + (i + Z.ng);
                         for (int q = 0; q < 5; ++q) a[q][id] = pattern(q, i, j, k) + 1e-13 * q;
                     }
-            const RestartInfo info{Z.nx, Z.ny, Z.nz, std::atoi(argv[7]), std::atoll(argv[8]), arg(argv, 9)};
+            const RestartInfo info{Z.nx, Z.ny, Z.nz, std::atoi(argv[7]), std::atoll(argv[8]), arg(argv, 9),
+                                   arg(argv, 10)};
             const double *c[5] = {p[0], p[1], p[2], p[3], p[4]};
             return write_restart(argv[2], info, Z, c) ? 0 : 1;
         }
         RestartInfo info{};
         if (!read_restart(argv[2], info, Z, p)) return 1;
-        std::printf("%d %d %d %d %lld %.17g\n", info.nx, info.ny, info.nz, info.kind, info.step, info.time);
+        std::printf("%d %d %d %d %lld %.17g %.17g\n", info.nx, info.ny, info.nz, info.kind, info.step, info.time,
+                    info.next_output);
         FILE *f = std::fopen(argv[7], "wb");
         if (!f) return 1;
         for (int q = 0; q < 5; ++q) std::fwrite(a[q].data(), sizeof(double), tot, f);

@@ -117,3 +117,23 @@ def test_create_argument_errors():
     assert lib.ns3d_create(C.byref(q), C.byref(h), C.byref(ctx)) == _lib.NS3D_ERR_GRID_TOO_SMALL
     for s in range(0, 9):
         assert lib.ns3d_status_string(s)
+
+
+def test_recon_fallback_past_the_stencil_is_refused():
+    """reconstruct_select's averaging fallback (reconstruction.cpp:77-84, the
+    WCNS value) reads vstencil[mid + 1] on the minus side; with two values
+    that is past the stencil.  The entry point refuses it before any device
+    work, and the oracle reports it; width 3 and the plus side are served."""
+    lib = ns.lib()
+    p, _ = ns.make_params(8)
+    p.recon = 1   # WCNS: no scheme of its own, the fallback
+    st = (C.c_double * 6)(1.0, 2.0, 3.0, 4.0, 5.0, 6.0)
+    out = (C.c_double * 3)()
+    assert lib.ns3d_reconstruct_batch(C.byref(p), 3, 2, st, -1.0, out) == _lib.NS3D_ERR_CONFIG
+    orc = load_oracle()
+    err = C.c_int(0)
+    v = orc.orc_reconstruct(st, 2, -1.0, C.byref(p), C.byref(err))
+    assert err.value == 1 and v != v
+    err.value = 0
+    assert orc.orc_reconstruct(st, 2, 1.0, C.byref(p), C.byref(err)) == 0.5 * (2.0 + 1.0) and err.value == 0
+    assert orc.orc_reconstruct(st, 3, -1.0, C.byref(p), C.byref(err)) == 0.5 * (2.0 + 3.0) and err.value == 0

@@ -255,12 +255,46 @@ def test_driver_restart_continues_bitwise(gpu, tmp_path, precision, split):
         runs[name] = run_driver(d, inp, *args)
     a = (tmp_path / "whole" / "r.bin").read_bytes()
     b = (tmp_path / "second" / "r.bin").read_bytes()
-    assert a[:8] == b"NS3DRST1" and len(a) == 40 + 5 * n ** 3 * 8
+    assert a[:8] == b"NS3DRST2" and len(a) == 48 + 5 * n ** 3 * 8
     assert len(a) == len(b)
-    diff = np.flatnonzero(np.frombuffer(a[40:], np.uint8) != np.frombuffer(b[40:], np.uint8))
-    assert a[:40] == b[:40] and diff.size == 0, diff[:10]
+    diff = np.flatnonzero(np.frombuffer(a[48:], np.uint8) != np.frombuffer(b[48:], np.uint8))
+    assert a[:48] == b[:48] and diff.size == 0, diff[:10]
     # the histories of the second run continue those of the whole run
     whole = (tmp_path / "whole" / "output" / "output_residuals.dat").read_text().splitlines()
     second = (tmp_path / "second" / "output" / "output_residuals.dat").read_text().splitlines()
     tail = [line for line in whole[2:] if int(line.split()[0]) > split]
     assert [line for line in second if line and line[0].isdigit()] == tail
+
+
+def test_driver_restart_keeps_an_owed_output(gpu, tmp_path):
+    """Fixed dt = 1e-3 and output_Timefreq = 0.003: the third step lands on
+    t = 0.003 exactly, without the clamp of maincycle.cpp:41-45, so the
+    unbroken run writes that output at the end of the fourth step
+    (maincycle.cpp:33-36).  A run restarted after step 3 resumes the schedule
+    from the restart file and writes the same field files after it."""
+    n = 12
+    x = np.arange(n) * (6.283185307179586 / n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    st = [1.0 + 0.1 * np.sin(X + 2 * Z), np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z),
+          0.2 * np.sin(Y + Z), 1.0 / (1.4 * 0.01) + 0.5 * np.cos(X + Y)]
+    (tmp_path / "ic.raw").write_bytes(b"".join(np.ascontiguousarray(a).tobytes() for a in st))
+    dirs = {}
+    for name, steps in (("whole", 7), ("first", 3), ("second", 7)):
+        d = tmp_path / name
+        d.mkdir()
+        inp = solver_in(d / "solver.in", n, max_steps=steps, dt=1e-3, out_freq=0.003, monitor=100)
+        args = ["--restart-out", str(d / "r.bin")]
+        if name == "second":
+            args += ["--restart-in", str(tmp_path / "first" / "r.bin")]
+        else:
+            args += ["--ic", f"raw:{tmp_path / 'ic.raw'}"]
+        run_driver(d, inp, *args)
+        dirs[name] = sorted(p.name for p in (d / "output").iterdir() if p.name.startswith("time_"))
+    # in-loop outputs at t = 0.004 (owed from 0.003) and 0.007; t = 0 and the
+    # end-of-run file besides
+    assert "time_400" in dirs["whole"] and "time_700" in dirs["whole"]
+    assert [x for x in dirs["whole"] if x != "time_0"] == [x for x in dirs["second"] if x != "time_300"]
+    for t in ("time_400", "time_700"):
+        a = (tmp_path / "whole" / "output" / t / "field_rank0.dat").read_bytes()
+        b = (tmp_path / "second" / "output" / t / "field_rank0.dat").read_bytes()
+        assert a == b, t

@@ -182,12 +182,12 @@ def test_residual_file(tool, tmp_path):
 def test_restart_round_trip(tool, tmp_path):
     nx, ny, nz, ng = 6, 5, 4, 2
     path = tmp_path / "r.bin"
-    tool("rwrite", path, nx, ny, nz, ng, 1, 123456789012, 0.1 + 0.2)
+    tool("rwrite", path, nx, ny, nz, ng, 1, 123456789012, 0.1 + 0.2, 0.375)
     raw = path.read_bytes()
-    assert raw[:8] == b"NS3DRST1" and len(raw) == 8 + 16 + 8 + 8 + 5 * nx * ny * nz * 8
+    assert raw[:8] == b"NS3DRST2" and len(raw) == 8 + 16 + 8 + 8 + 8 + 5 * nx * ny * nz * 8
     head = tool("rread", path, nx, ny, nz, ng, tmp_path / "out.raw").split()
     assert head[:4] == [str(nx), str(ny), str(nz), "1"] and int(head[4]) == 123456789012
-    assert float(head[5]) == 0.1 + 0.2
+    assert float(head[5]) == 0.1 + 0.2 and float(head[6]) == 0.375
     a = load_raw(tmp_path / "out.raw", nx, ny, nz, ng)
     inner = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
@@ -198,9 +198,29 @@ def test_restart_round_trip(tool, tmp_path):
         assert np.all(np.isnan(ghost))
 
 
+def test_restart_reads_the_earlier_layout(tool, tmp_path):
+    """A file of the first layout (magic NS3DRST1, no next output time)
+    reads with the schedule unknown (NaN)."""
+    nx, ny, nz, ng = 3, 2, 2, 1
+    path = tmp_path / "r.bin"
+    tool("rwrite", path, nx, ny, nz, ng, 0, 7, 0.5, 0.75)
+    raw = path.read_bytes()
+    old = b"NS3DRST1" + raw[8:40] + raw[48:]
+    (tmp_path / "old.bin").write_bytes(old)
+    head = tool("rread", tmp_path / "old.bin", nx, ny, nz, ng, tmp_path / "o.raw").split()
+    assert int(head[4]) == 7 and float(head[5]) == 0.5 and head[6] == "nan"
+    a = load_raw(tmp_path / "o.raw", nx, ny, nz, ng)
+    b_path = tmp_path / "o2.raw"
+    tool("rread", path, nx, ny, nz, ng, b_path)
+    b = load_raw(b_path, nx, ny, nz, ng)
+    inner = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    for q in range(5):
+        assert np.array_equal(a[q][inner], b[q][inner])
+
+
 def test_restart_rejects_wrong_size_and_truncation(tool, tmp_path):
     path = tmp_path / "r.bin"
-    tool("rwrite", path, 6, 5, 4, 2, 0, 3, 1.5)
+    tool("rwrite", path, 6, 5, 4, 2, 0, 3, 1.5, 2.0)
     with pytest.raises(AssertionError, match="holds a 6x5x4 grid"):
         tool("rread", path, 6, 5, 5, 2, tmp_path / "o.raw")
     path.write_bytes(path.read_bytes()[:-8])
