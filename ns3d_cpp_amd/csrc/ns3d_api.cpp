@@ -710,6 +710,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     for (int a = 0; a < 3; ++a)
         d.periodic[a] = p.bc[2 * a] == NS3D_BC_PERIODIC && p.bc[2 * a + 1] == NS3D_BC_PERIODIC;
     for (int f = 0; f < 6; ++f) d.bc[f] = p.bc[f];
+    set_fp32_pairs(d);
     // LDS per workgroup of this device (the whole-row and whole-line kernels
     // size their tiles from it)
     {

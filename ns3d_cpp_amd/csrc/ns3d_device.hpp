@@ -9,6 +9,14 @@
 
 namespace ns3d {
 
+// A double constant as the float pair h + l (h the nearest float, l the
+// nearest float to the remainder), for FP32 arithmetic: s * h + s * l scales
+// by the constant with the rounding of one product, where a float rounded
+// once would scale every use in a run by the same wrong factor.
+struct F2 {
+    float h, l;
+};
+
 // Scalars every kernel needs, passed by value.
 struct DevParams {
     double gamma, Rgas, Cp, Pr, mu, sutherland_s, mu_power, sw_eps;
@@ -23,7 +31,42 @@ struct DevParams {
     int lds_max;             // LDS bytes one workgroup may use on this device
     double cfl;              // CFL number of the device time step
     int check_mask;          // bit s: stage s scans for non-physical cells (the third only)
+    // constants of the mixed-precision path (set_fp32_pairs): id2[a] =
+    // 1 / d_a; dsc[a][f] = 1 / (K_f d_a) for the integer-weight derivative sums,
+    // K = 840 (C8), 60 (C6), 12 (C4), 2 (second-order closures); mu0 the
+    // mu law's constant (mu, or mu (1 + S) for Sutherland), mk0 = mu0 Cp / Pr,
+    // suth = S
+    F2 id2[3];
+    F2 dsc[3][4];
+    F2 mu0, mk0, suth;
 };
+
+// float pair of x (host and device)
+__host__ __device__ inline F2 f2_of(double x)
+{
+    F2 r;
+    r.h = (float)x;
+    r.l = (float)(x - (double)r.h);
+    return r;
+}
+
+// s times the constant c (one FMA; contraction-independent)
+__device__ __forceinline__ float mul2(float s, F2 c) { return fmaf(s, c.h, s * c.l); }
+
+// the F2 members of d from its FP64 members
+inline void set_fp32_pairs(DevParams &d)
+{
+    const double id[3] = {d.idx, d.idy, d.idz};
+    const double K[4] = {840.0, 60.0, 12.0, 2.0};
+    for (int a = 0; a < 3; ++a) {
+        d.id2[a] = f2_of(id[a]);
+        for (int f = 0; f < 4; ++f) d.dsc[a][f] = f2_of(id[a] / K[f]);
+    }
+    const double mu0 = d.mu_law == 0 ? d.mu * (1.0 + d.sutherland_s) : d.mu;
+    d.mu0 = f2_of(mu0);
+    d.mk0 = f2_of(mu0 * (d.Cp / d.Pr));
+    d.suth = f2_of(d.sutherland_s);
+}
 
 // Slots of a context's device flag words: [0] key of the first non-physical
 // cell, [1] the CFL minimum (bits of a positive double), [2] the RK3 step

@@ -259,7 +259,10 @@ __global__ __launch_bounds__(64, 2) void km_sweep64(DevParams P, MFields M, int 
 // than with a second, quarter-rate, reciprocal).  The weights see only
 // ratios of the T_k and of the alpha_k, so t_k = 4 (eps + beta_k) (the 1/4
 // of beta's second square folded away) and alpha_k scaled by 10 (linear
-// weights 1, 6, 3) leave them unchanged and save a multiply each.
+// weights 1, 6, 3) leave them unchanged and save a multiply each.  The 1/6
+// of the candidate polynomials goes into the reciprocal's argument: a float
+// 1/6 applied after it scaled every reconstructed difference of a run by
+// the same factor 1 + 3.0e-8, and that bias accumulated over the steps.
 __device__ __forceinline__ float rel32_weno5(const float *D)
 {
     const float D1 = D[1], D2 = D[2], D3 = D[3], D4 = D[4];
@@ -282,7 +285,7 @@ __device__ __forceinline__ float rel32_weno5(const float *D)
     const float q0 = 2.0f * r0 - D2;          // 5 D2 - 2 D1
     const float q1 = D2 + 2.0f * D3;
     const float q2 = 4.0f * D3 - D4;
-    return (a0 * q0 + a1 * q1 + a2 * q2) * (rcpf(a0 + a1 + a2) * (1.0f / 6.0f));
+    return (a0 * q0 + a1 * q1 + a2 * q2) * rcpf(6.0f * (a0 + a1 + a2));
 }
 
 // coefficients e[t] of a linear scheme in difference form:
@@ -305,8 +308,10 @@ __device__ __forceinline__ void diff_coeffs(const DevParams &P, float *e)
         c[4] = -1.5 * disp + 2.5 * diss - 1.0 / 12.0;
         c[5] = 0.5 * disp - 0.5 * diss;
     } else if constexpr (RECON == R_UD7) {
-        c[0] = -1.0 / 140.0; c[1] = 5.0 / 84.0; c[2] = -101.0 / 420.0; c[3] = 319.0 / 420.0;
-        c[4] = 107.0 / 210.0; c[5] = -19.0 / 210.0; c[6] = 1.0 / 105.0; c[7] = 0.0;
+        // 420 c: e comes out as integers, exact in FP32 (rel_scale applies
+        // the 1/420)
+        c[0] = -3.0; c[1] = 25.0; c[2] = -101.0; c[3] = 319.0;
+        c[4] = 214.0; c[5] = -38.0; c[6] = 4.0; c[7] = 0.0;
     } else {
         const double a = P.ud7l_alpha, b = 1.0 - a;
         c[0] = -3.0 / 420.0 * a - 3.0 / 840.0 * b;
@@ -329,6 +334,23 @@ __device__ __forceinline__ void diff_coeffs(const DevParams &P, float *e)
             for (int q = 0; q < S; ++q) acc -= q < t ? c[q] : 0.0;
         }
         e[t] = (float)acc;
+    }
+}
+
+// The reconstructed differences of both sides of a face, scaled: UD7's
+// weights are applied as integers (diff_coeffs) and their 1/420 here as a
+// float pair.  Weights rounded to FP32 would change the scheme by the same
+// relative amount at every face of a run (its dissipation, in effect),
+// which accumulates over the steps instead of averaging out.  MDCD's and
+// UD7L's weights depend on run parameters and keep their FP32 rounding.
+template <int RECON>
+__device__ __forceinline__ float rel_scale(float r)
+{
+    if constexpr (RECON == R_UD7) {
+        constexpr F2 K = {(float)(1.0 / 420.0), (float)(1.0 / 420.0 - (double)(float)(1.0 / 420.0))};
+        return mul2(r, K);
+    } else {
+        return r;
     }
 }
 
@@ -461,7 +483,7 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
                 D[tt] = dl[m * NSLOT + c + 1 + tt];
                 Dm[tt] = -dl[(5 + m) * NSLOT + c + 1 + S - tt];
             }
-            rl[m * NF + t] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
+            rl[m * NF + t] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
         }
     }
     __syncthreads();
@@ -469,12 +491,14 @@ __global__ __launch_bounds__(1024) void km_sweep_xrow(DevParams P, MFields M)
         const int fr = t + SH;
         const int fl = PER ? (t == 0 ? TX - 1 : t - 1) : t;
         const int id = row + P.ng + t;
-        const float ix = (float)P.idx;
+        // 1/dx as a float pair: 1/dx rounded once to float would scale
+        // every inviscid term of a run by the same factor (1 - 4.0e-8 for
+        // dx = 2 pi / 2^k), an error that accumulates over the steps
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             const float dF = dl[m * NSLOT + t + MID + 1] + dl[(5 + m) * NSLOT + t + MID + 2] +
                              (rl[m * NF + fr] - rl[m * NF + fl]);
-            M.A[m][id] = dF * ix;
+            M.A[m][id] = mul2(dF, P.id2[0]);
         }
     }
 }
@@ -517,7 +541,6 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
     int base, st;
     if (DIR == 1) { base = ((P.ng + b) * P.sy) * P.sx + (P.ng + a); st = P.sx; }
     else { base = (P.ng + b) * P.sx + (P.ng + a); st = P.sx * P.sy; }
-    const float inv_d = (float)(DIR == 1 ? P.idy : P.idz);
     float e[S];
     diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);
     float wp[5][S];                            // plus side: wp[m][t], positions T0..S-1
@@ -611,13 +634,15 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
                 D[t] = get(m, t);
                 Dm[t] = -get(5 + m, S - t);
             }
-            rel[m] = rel32<RECON>(e, D) + rel32<RECON>(e, Dm);
+            rel[m] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
         }
         if (f >= c0) {
             const int idc = base + f * st;
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                M.A[m][idc] = acur[m] + (get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m])) * inv_d;
+            for (int m = 0; m < 5; ++m) {
+                const float dF = get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m]);
+                M.A[m][idc] = acur[m] + mul2(dF, P.id2[DIR]);   // 1/d as a pair, as in the x sweep
+            }
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
@@ -626,51 +651,63 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
 
 // --------------------------------------------------------------- viscous
 
-__device__ __forceinline__ float get_mu32(const DevParams &P, float T)
+// mu(T) / mu0 in FP32 (get_mu of the FP64 path with its constant factor
+// mu0 = mu, or mu (1 + S) for Sutherland, left out: the callers apply mu0
+// as a float pair)
+__device__ __forceinline__ float mu_shape32(const DevParams &P, float T)
 {
-    if (P.mu_law == 1) return (float)P.mu;
-    if (P.mu_law == 2) return (float)P.mu * powf(T, (float)P.mu_power);
-    const float S_ref = (float)P.sutherland_s;
-    return (float)P.mu * (T * sqrtf(T)) * (1.0f + S_ref) / (T + S_ref);
+    if (P.mu_law == 1) return 1.0f;
+    if (P.mu_law == 2) return powf(T, (float)P.mu_power);
+    return (T * sqrtf(T)) / ((T + P.suth.h) + P.suth.l);
 }
 
+// integer-weight central sums: the derivative is the sum times 1 / (K d),
+// K = 840 (C8), 60 (C6), 12 (C4), with the scale as a float pair (rounded
+// fractional weights such as 0.8f, or 1/d rounded once, would bias every
+// derivative of a run by the same factor)
 template <int VIS>
-__device__ __forceinline__ float diff32(const float *v, float h_inv)
+__device__ __forceinline__ float dsum32(const float *v)
 {
     if constexpr (VIS == VIS_C8) {
-        return (0.8f * (v[1] - v[-1]) - 0.2f * (v[2] - v[-2]) + (4.0f / 105.0f) * (v[3] - v[-3]) -
-                (1.0f / 280.0f) * (v[4] - v[-4])) * h_inv;
+        return 672.0f * (v[1] - v[-1]) - 168.0f * (v[2] - v[-2]) + 32.0f * (v[3] - v[-3]) - 3.0f * (v[4] - v[-4]);
     } else if constexpr (VIS == VIS_C6) {
-        return (v[3] - 9.0f * v[2] + 45.0f * v[1] - 45.0f * v[-1] + 9.0f * v[-2] - v[-3]) * (h_inv / 60.0f);
+        return 45.0f * (v[1] - v[-1]) - 9.0f * (v[2] - v[-2]) + (v[3] - v[-3]);
     } else {
-        return (-v[2] + 8.0f * v[1] - 8.0f * v[-1] + v[-2]) * (h_inv / 12.0f);
+        return 8.0f * (v[1] - v[-1]) - (v[2] - v[-2]);
     }
+}
+
+// sc: the axis' row of DevParams::dsc
+template <int VIS>
+__device__ __forceinline__ float diff32(const float *v, const F2 *sc)
+{
+    return mul2(dsum32<VIS>(v), sc[VIS == VIS_C8 ? 0 : (VIS == VIS_C6 ? 1 : 2)]);
 }
 
 // the closures of choose_scheme on non-periodic axes (as diff_closure) in
 // FP32; periodic axes pass dlo = dhi = R
-__device__ __forceinline__ float diff32_closure(const float *v, float h_inv, int dlo, int dhi)
+__device__ __forceinline__ float diff32_closure(const float *v, const F2 *sc, int dlo, int dhi)
 {
-    if (dlo == 0) return (-3.0f * v[0] + 4.0f * v[1] - v[2]) * (0.5f * h_inv);
-    if (dhi == 0) return (-3.0f * v[0] + 4.0f * v[-1] - v[-2]) * (0.5f * h_inv);
+    if (dlo == 0) return mul2(-3.0f * v[0] + 4.0f * v[1] - v[2], sc[3]);
+    if (dhi == 0) return mul2(-3.0f * v[0] + 4.0f * v[-1] - v[-2], sc[3]);
     const int d = dlo < dhi ? dlo : dhi;
-    if (d == 1) return (v[1] - v[-1]) * (0.5f * h_inv);
-    if (d == 2) return diff32<VIS_C4>(v, h_inv);
-    return diff32<VIS_C6>(v, h_inv);
+    if (d == 1) return mul2(v[1] - v[-1], sc[3]);
+    if (d == 2) return diff32<VIS_C4>(v, sc);
+    return diff32<VIS_C6>(v, sc);
 }
 
 template <int VIS>
-__device__ __forceinline__ float diff32_bc(const float *v, float h_inv, int dlo, int dhi)
+__device__ __forceinline__ float diff32_bc(const float *v, const F2 *sc, int dlo, int dhi)
 {
     constexpr int R = VisReach<VIS>::R;
-    if (dlo >= R && dhi >= R) return diff32<VIS>(v, h_inv);
-    return diff32_closure(v, h_inv, dlo, dhi);
+    if (dlo >= R && dhi >= R) return diff32<VIS>(v, sc);
+    return diff32_closure(v, sc, dlo, dhi);
 }
 
 // x and y derivatives of the four shadow quantities at cell id
 template <int VIS>
-__device__ __forceinline__ void gradxy32(const float *const src[4], int id, int sx, float idx, float idy, int dxl,
-                                         int dxh, int dyl, int dyh, float g[8])
+__device__ __forceinline__ void gradxy32(const float *const src[4], int id, int sx, const F2 *scx, const F2 *scy,
+                                         int dxl, int dxh, int dyl, int dyh, float g[8])
 {
     constexpr int R = VisReach<VIS>::R;
 #pragma unroll
@@ -681,8 +718,8 @@ __device__ __forceinline__ void gradxy32(const float *const src[4], int id, int 
             wx[o + R] = src[q][id + o];
             wy[o + R] = src[q][id + o * sx];
         }
-        g[2 * q + 0] = diff32_bc<VIS>(wx + R, idx, dxl, dxh);
-        g[2 * q + 1] = diff32_bc<VIS>(wy + R, idy, dyl, dyh);
+        g[2 * q + 0] = diff32_bc<VIS>(wx + R, scx, dxl, dxh);
+        g[2 * q + 1] = diff32_bc<VIS>(wy + R, scy, dyl, dyh);
     }
 }
 
@@ -709,7 +746,6 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
     const int sxy = P.sx * P.sy;
     const int col = j * P.sx + i;
     const float *const src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
-    const float idx = (float)P.idx, idy = (float)P.idy, idz = (float)P.idz;
     int dxl = R, dxh = R, dyl = R, dyh = R, dzl = R, dzh = R;
     if constexpr (BC) {
         axis_dist(P.periodic[0], i, P.ng, P.nx, R, dxl, dxh);
@@ -723,8 +759,7 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
         for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
 #pragma unroll
     for (int s = 1; s <= R; ++s)
-        if (k0 + s - 1 < k1) gradxy32<VIS>(src, (k0 + s - 1) * sxy + col, P.sx, idx, idy, dxl, dxh, dyl, dyh, gxy[s]);
-    const float cq = (float)(P.Cp / P.Pr);
+        if (k0 + s - 1 < k1) gradxy32<VIS>(src, (k0 + s - 1) * sxy + col, P.sx, P.dsc[0], P.dsc[1], dxl, dxh, dyl, dyh, gxy[s]);
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         if constexpr (BC) axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
@@ -742,34 +777,35 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
         // loads, but keeps the kernel at 126 VGPRs (4 waves per SIMD); the
         // unconditional form keeps them in flight at 160 (3 waves) and ran
         // 10 % slower
-        if (k + R < k1) gradxy32<VIS>(src, id + R * sxy, P.sx, idx, idy, dxl, dxh, dyl, dyh, gxy[R]);
+        if (k + R < k1) gradxy32<VIS>(src, id + R * sxy, P.sx, P.dsc[0], P.dsc[1], dxl, dxh, dyl, dyh, gxy[R]);
         float g[12];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             g[3 * q + 0] = gxy[0][2 * q];
             g[3 * q + 1] = gxy[0][2 * q + 1];
-            g[3 * q + 2] = diff32_bc<VIS>(&win[q][R], idz, dzl, dzh);
+            g[3 * q + 2] = diff32_bc<VIS>(&win[q][R], P.dsc[2], dzl, dzh);
         }
-        const float mu = get_mu32(P, 1.0f + win[3][R]);
-        const float divU = g[0] + g[4] + g[8];
-        const float two3 = 2.0f / 3.0f;
-        const float txx = 2.0f * mu * g[0] - two3 * mu * divU;
-        const float tyy = 2.0f * mu * g[4] - two3 * mu * divU;
-        const float tzz = 2.0f * mu * g[8] - two3 * mu * divU;
-        const float txy = mu * (g[1] + g[3]);
-        const float txz = mu * (g[2] + g[6]);
-        const float tyz = mu * (g[5] + g[7]);
-        const float mk = mu * cq;
+        // tau = mu0 * f(T) * (2 S - 2/3 div u I), q = mk0 f(T) grad T:
+        // the constants mu0, mk0 and 2/3 as float pairs
+        constexpr F2 TWO3 = {(float)(2.0 / 3.0), (float)(2.0 / 3.0 - (double)(float)(2.0 / 3.0))};
+        const float f = mu_shape32(P, 1.0f + win[3][R]);
+        const float d3 = mul2(g[0] + g[4] + g[8], TWO3);
+        const float txx = mul2(f * (2.0f * g[0] - d3), P.mu0);
+        const float tyy = mul2(f * (2.0f * g[4] - d3), P.mu0);
+        const float tzz = mul2(f * (2.0f * g[8] - d3), P.mu0);
+        const float txy = mul2(f * (g[1] + g[3]), P.mu0);
+        const float txz = mul2(f * (g[2] + g[6]), P.mu0);
+        const float tyz = mul2(f * (g[5] + g[7]), P.mu0);
         const float u = win[0][R], v = win[1][R], w = win[2][R];
         M.VF[0][id] = txx;
         M.VF[1][id] = txy;
         M.VF[2][id] = txz;
-        M.VF[3][id] = u * txx + v * txy + w * txz + mk * g[9];
+        M.VF[3][id] = u * txx + v * txy + w * txz + mul2(f * g[9], P.mk0);
         M.VF[5][id] = tyy;
         M.VF[6][id] = tyz;
-        M.VF[7][id] = u * txy + v * tyy + w * tyz + mk * g[10];
+        M.VF[7][id] = u * txy + v * tyy + w * tyz + mul2(f * g[10], P.mk0);
         M.VF[10][id] = tzz;
-        M.VF[11][id] = u * txz + v * tyz + w * tzz + mk * g[11];
+        M.VF[11][id] = u * txz + v * tyz + w * tzz + mul2(f * g[11], P.mk0);
     }
 }
 
@@ -779,7 +815,8 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
 // the shadow values it stores (the values the CFL pass would read)
 template <int MODE>
 __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M, int id, const double r[5],
-                                            double dt, unsigned long long *flag, bool cfl_next, double &cx, double &cy, double &cz)
+                                            const double u0[5], const float du0[5], double dt, unsigned long long *flag,
+                                            bool cfl_next, double &cx, double &cy, double &cz)
 {
     if constexpr (MODE == 0) {
 #pragma unroll
@@ -803,9 +840,9 @@ __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M
             float d;
             if constexpr (MODE == 1) d = (float)(dt * r[m]);
             else if constexpr (MODE == 2) d = (float)(0.25 * ((double)M.dU[m][id] + dt * r[m]));
-            else d = (float)((2.0 / 3.0) * ((double)M.dU[m][id] + dt * r[m]));
+            else d = (float)((2.0 / 3.0) * ((double)du0[m] + dt * r[m]));
             M.dU[m][id] = d;
-            Un[m] = M.U[m][id] + (double)d;
+            Un[m] = (MODE == 3 ? u0[m] : M.U[m][id]) + (double)d;
         }
         double rn = Un[0];
         if (rn <= 0.0) rn = 1e-12;
@@ -869,7 +906,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         oxc[o + R] = j * P.sx + wrapi_m(i + o, ng, P.nx);
         oyc[o + R] = wrapi_m(j + o, ng, P.ny) * P.sx + i;
     }
-    const float idx = (float)P.idx, idy = (float)P.idy, idz = (float)P.idz;
     float win[4][2 * R + 1];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -887,24 +923,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int id = k * sxy + col;
         const int kn = wrapi_m(k + R, ng, P.nz) * sxy + col;
         if constexpr (BC) axis_dist(P.periodic[2], k, ng, P.nz, R, dzl, dzh);
-        double r[5];
-        r[0] = -(double)M.A[0][id];
+        // every load of the cell's divergence first, then the arithmetic:
+        // the scheduling barrier keeps the loads one burst
+        float wx[4][2 * R + 1], wy[4][2 * R + 1], an[5];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
 #pragma unroll
             for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
             win[m][2 * R] = M.VF[8 + m][kn];
-            float wx[2 * R + 1], wy[2 * R + 1];
 #pragma unroll
             for (int o = 0; o < 2 * R + 1; ++o) {
-                wx[o] = M.VF[m][k * sxy + oxc[o]];
-                wy[o] = M.VF[4 + m][k * sxy + oyc[o]];
+                wx[m][o] = M.VF[m][k * sxy + oxc[o]];
+                wy[m][o] = M.VF[4 + m][k * sxy + oyc[o]];
             }
-            const float vd = diff32_bc<VIS>(wx + R, idx, dxl, dxh) + diff32_bc<VIS>(wy + R, idy, dyl, dyh) +
-                             diff32_bc<VIS>(&win[m][R], idz, dzl, dzh);
-            r[m + 1] = (double)vd - (double)M.A[m + 1][id];
         }
-        vdiv_finish<MODE>(P, M, id, r, dt, flag, cfl_next, cx, cy, cz);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) an[m] = M.A[m][id];
+        // the third stage takes U and dU into the burst as well (u0, du0);
+        // the first two read them in vdiv_finish (measured both ways: 0.90 against 1.10 ms
+        // for the third stage at 256^3, 0.60 / 0.66 against 0.70 / 0.73 ms
+        // for the first two)
+        double u0[5];
+        float du0[5];
+        if constexpr (MODE == 3) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                u0[m] = M.U[m][id];
+                du0[m] = M.dU[m][id];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        double r[5];
+        r[0] = -(double)an[0];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const float vd = diff32_bc<VIS>(wx[m] + R, P.dsc[0], dxl, dxh) + diff32_bc<VIS>(wy[m] + R, P.dsc[1], dyl, dyh) +
+                             diff32_bc<VIS>(&win[m][R], P.dsc[2], dzl, dzh);
+            r[m + 1] = (double)vd - (double)an[m + 1];
+        }
+        vdiv_finish<MODE>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
     }
     if (cfl_next) {
         const double cm = wave_min(cfl_bound(P, cx, cy, cz));
