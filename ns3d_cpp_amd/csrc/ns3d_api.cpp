@@ -45,6 +45,7 @@ struct ns3d_ctx {
     FaceArrays faces{};          // allocated on first ns3d_du_inviscous
     int device = 0;
     hipStream_t stream = nullptr;
+    int gf_rows = 2;   // rows per thread of km_gradflux on periodic boxes (NS3D_GF_ROWS)
     // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
     // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
     // faster than eager launches (the kernels outlast the launch cost)
@@ -274,7 +275,8 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
     } else {
         for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
     }
-    TIMED(c, "gradflux", launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1]));
+    TIMED(c, "gradflux",
+          launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1], c->gf_rows));
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
           launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, dev_dt, c->m_vd_block[0],
                         c->m_vd_block[1]));
@@ -795,6 +797,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     }
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_GF_ROWS")) c->gf_rows = std::atoi(env) == 1 ? 1 : 2;
     e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 2);
     if (e == hipSuccess) e = launch_fill_u64(c->stream, c->d_flag + FLAG_STEP, 0ull, 2);   // step counter, time
     if (e == hipSuccess) {

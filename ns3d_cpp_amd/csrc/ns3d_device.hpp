@@ -158,6 +158,40 @@ __device__ __forceinline__ int xcd_logical(int wg, int nwg)
     return (wg & 7) * (nwg >> 3) + (wg >> 3);
 }
 
+// Thread t of the periodic ghost fill of one axis: the ghost cell dst of
+// the 2 ng layers on the axis' two faces (face cells only: the stencils are
+// axis-aligned and read no edge or corner ghosts) and its periodic image
+// src in the interior.  On the x faces the ghost index runs fastest (the
+// ghost cells of a row are adjacent); on the y and z faces x runs fastest,
+// so a wave writes whole row segments.  False past the last ghost cell.
+__device__ __forceinline__ bool periodic_ghost_pair(const DevParams &P, int axis, int t, int &dst, int &src)
+{
+    const int ng = P.ng;
+    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const int na = axis == 0 ? P.ny : P.nx;   // fastest cross axis
+    const int nb = axis == 2 ? P.ny : P.nz;
+    if (t >= 2 * ng * na * nb) return false;
+    int g, a, b;
+    if (axis == 0) {
+        g = t % (2 * ng);
+        const int rest = t / (2 * ng);
+        a = ng + rest % na;
+        b = ng + rest / na;
+    } else {
+        a = ng + t % na;
+        const int rest = t / na;
+        g = rest % (2 * ng);
+        b = ng + rest / (2 * ng);
+    }
+    int q, qs;   // ghost layer and its image along the axis
+    if (g < ng) { q = g; qs = g + nd; }
+    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
+    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
+    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
+    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+    return true;
+}
+
 // logical (x, y, z) block coordinates of a 3D grid after the XCD remap
 __device__ __forceinline__ void xcd_block3(int &bx, int &by, int &bz)
 {

@@ -98,7 +98,8 @@ hipError_t launch_m_shadow_to_prims64(hipStream_t s, const DevParams &P, const M
 hipError_t launch_m_face_diff(hipStream_t s, const DevParams &P, const MFields &M, const FaceArrays &ff);
 bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);
-hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by);
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by,
+                             int rows);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, int dev_dt, int bx = 64, int by = 4);
 // the workgroup shape tiles the interior with whole waves: km_vdiv_z can

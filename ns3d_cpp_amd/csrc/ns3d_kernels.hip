@@ -78,24 +78,8 @@ __global__ void k_fill_periodic(DevParams P, Fields F)
 {
     const int axis = blockIdx.y;
     if (!P.periodic[axis]) return;
-    const int ng = P.ng;
-    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
-    const int na = axis == 0 ? P.ny : P.nx;   // fastest cross axis
-    const int nb = axis == 2 ? P.ny : P.nz;
-    const int total = 2 * ng * na * nb;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    int g = t % (2 * ng);            // 0..ng-1 low side, ng..2ng-1 high side
-    int rest = t / (2 * ng);
-    int a = ng + rest % na;
-    int b = ng + rest / na;
-    int q, qs;
-    if (g < ng) { q = g; qs = g + nd; }
-    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
     int dst, src;
-    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
-    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
-    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+    if (!periodic_ghost_pair(P, axis, blockIdx.x * blockDim.x + threadIdx.x, dst, src)) return;
 #pragma unroll
     for (int m = 0; m < 6; ++m) F.W[m][dst] = F.W[m][src];
 }

@@ -723,6 +723,33 @@ __device__ __forceinline__ void gradxy32(const float *const src[4], int id, int 
     }
 }
 
+// the viscous flux of one cell from its 12 gradients g (u, v, w, T-1 along
+// x, y, z) and its shadow u, v, w, T-1, into the 9 distinct flux arrays:
+// tau = mu0 * f(T) * (2 S - 2/3 div u I), q = mk0 f(T) grad T, with the
+// constants mu0, mk0 and 2/3 as float pairs
+__device__ __forceinline__ void vflux32_store(const DevParams &P, const MFields &M, int id, const float g[12], float u,
+                                              float v, float w, float t1)
+{
+    constexpr F2 TWO3 = {(float)(2.0 / 3.0), (float)(2.0 / 3.0 - (double)(float)(2.0 / 3.0))};
+    const float f = mu_shape32(P, 1.0f + t1);
+    const float d3 = mul2(g[0] + g[4] + g[8], TWO3);
+    const float txx = mul2(f * (2.0f * g[0] - d3), P.mu0);
+    const float tyy = mul2(f * (2.0f * g[4] - d3), P.mu0);
+    const float tzz = mul2(f * (2.0f * g[8] - d3), P.mu0);
+    const float txy = mul2(f * (g[1] + g[3]), P.mu0);
+    const float txz = mul2(f * (g[2] + g[6]), P.mu0);
+    const float tyz = mul2(f * (g[5] + g[7]), P.mu0);
+    M.VF[0][id] = txx;
+    M.VF[1][id] = txy;
+    M.VF[2][id] = txz;
+    M.VF[3][id] = u * txx + v * txy + w * txz + mul2(f * g[9], P.mk0);
+    M.VF[5][id] = tyy;
+    M.VF[6][id] = tyz;
+    M.VF[7][id] = u * txy + v * tyy + w * tyz + mul2(f * g[10], P.mk0);
+    M.VF[10][id] = tzz;
+    M.VF[11][id] = u * txz + v * tyz + w * tzz + mul2(f * g[11], P.mk0);
+}
+
 // gradients + viscous flux in FP32 from the shadow (u, v, w, T-1),
 // z-marching register window as k_gradflux.  The stress tensor is symmetric:
 // VF[4], VF[8], VF[9] (txy, txz, tyz of the y and z fluxes) alias VF[1],
@@ -733,11 +760,9 @@ __device__ __forceinline__ void gradxy32(const float *const src[4], int id, int 
 // are still in L2 (taking them R planes later re-fetched the plane from HBM:
 // the XCD streams about 1 MB per plane step through its 4 MB L2).
 template <int VIS, int BC>
-__global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len)
+__device__ __forceinline__ void gradflux_tile(const DevParams &P, const MFields &M, int seg_len, int bx, int by, int bz)
 {
     constexpr int R = VisReach<VIS>::R;
-    int bx, by, bz;
-    xcd_block3(bx, by, bz);
     const int i = P.ng + bx * blockDim.x + threadIdx.x;
     const int j = P.ng + by * blockDim.y + threadIdx.y;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
@@ -785,28 +810,105 @@ __global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int s
             g[3 * q + 1] = gxy[0][2 * q + 1];
             g[3 * q + 2] = diff32_bc<VIS>(&win[q][R], P.dsc[2], dzl, dzh);
         }
-        // tau = mu0 * f(T) * (2 S - 2/3 div u I), q = mk0 f(T) grad T:
-        // the constants mu0, mk0 and 2/3 as float pairs
-        constexpr F2 TWO3 = {(float)(2.0 / 3.0), (float)(2.0 / 3.0 - (double)(float)(2.0 / 3.0))};
-        const float f = mu_shape32(P, 1.0f + win[3][R]);
-        const float d3 = mul2(g[0] + g[4] + g[8], TWO3);
-        const float txx = mul2(f * (2.0f * g[0] - d3), P.mu0);
-        const float tyy = mul2(f * (2.0f * g[4] - d3), P.mu0);
-        const float tzz = mul2(f * (2.0f * g[8] - d3), P.mu0);
-        const float txy = mul2(f * (g[1] + g[3]), P.mu0);
-        const float txz = mul2(f * (g[2] + g[6]), P.mu0);
-        const float tyz = mul2(f * (g[5] + g[7]), P.mu0);
-        const float u = win[0][R], v = win[1][R], w = win[2][R];
-        M.VF[0][id] = txx;
-        M.VF[1][id] = txy;
-        M.VF[2][id] = txz;
-        M.VF[3][id] = u * txx + v * txy + w * txz + mul2(f * g[9], P.mk0);
-        M.VF[5][id] = tyy;
-        M.VF[6][id] = tyz;
-        M.VF[7][id] = u * txy + v * tyy + w * tyz + mul2(f * g[10], P.mk0);
-        M.VF[10][id] = tzz;
-        M.VF[11][id] = u * txz + v * tyz + w * tzz + mul2(f * g[11], P.mk0);
+        vflux32_store(P, M, id, g, win[0][R], win[1][R], win[2][R], win[3][R]);
     }
+}
+
+// rows j and j + 1 of a periodic box per thread: each y-neighbour load
+// serves both rows (2 R + 2 loads per quantity instead of 2 (2 R + 1)),
+// and a 64 x 4 workgroup covers 8 rows, so its y halo is 2x its rows
+// instead of 3x.  Interior cells only (periodic axes: no closures).
+template <int VIS>
+__device__ __forceinline__ void gradxy32_rows2(const float *const src[4], int id, int sx, const F2 *scx,
+                                               const F2 *scy, float g0[8], float g1[8])
+{
+    constexpr int R = VisReach<VIS>::R;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float wx0[2 * R + 1], wx1[2 * R + 1], wy[2 * R + 2];
+#pragma unroll
+        for (int o = -R; o <= R; ++o) {
+            wx0[o + R] = src[q][id + o];
+            wx1[o + R] = src[q][id + sx + o];
+        }
+#pragma unroll
+        for (int o = -R; o <= R + 1; ++o) wy[o + R] = src[q][id + o * sx];
+        g0[2 * q + 0] = diff32<VIS>(wx0 + R, scx);
+        g1[2 * q + 0] = diff32<VIS>(wx1 + R, scx);
+        g0[2 * q + 1] = diff32<VIS>(wy + R, scy);
+        g1[2 * q + 1] = diff32<VIS>(wy + R + 1, scy);
+    }
+}
+
+template <int VIS>
+__device__ __forceinline__ void gradflux_tile_rows2(const DevParams &P, const MFields &M, int seg_len, int bx, int by,
+                                                    int bz)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int i = P.ng + bx * blockDim.x + threadIdx.x;
+    const int j = P.ng + 2 * (by * blockDim.y + threadIdx.y);   // ny is even: j + 1 is interior too
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int k0 = P.ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, P.ng + P.nz);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    const float *const src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
+    float win[2][4][2 * R + 1];
+    float gxy[R + 1][2][8];   // slot s: x / y derivatives of plane k + s, rows j and j + 1
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int o = 1; o < 2 * R + 1; ++o) win[r][q][o] = src[q][(k0 - R - 1 + o) * sxy + col + r * sx];
+#pragma unroll
+    for (int s = 1; s <= R; ++s)
+        if (k0 + s - 1 < k1) gradxy32_rows2<VIS>(src, (k0 + s - 1) * sxy + col, sx, P.dsc[0], P.dsc[1], gxy[s][0], gxy[s][1]);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) gxy[s][r][q] = gxy[s + 1][r][q];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int o = 0; o < 2 * R; ++o) win[r][q][o] = win[r][q][o + 1];
+                win[r][q][2 * R] = src[q][id + R * sxy + r * sx];
+            }
+        if (k + R < k1) gradxy32_rows2<VIS>(src, id + R * sxy, sx, P.dsc[0], P.dsc[1], gxy[R][0], gxy[R][1]);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            float g[12];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                g[3 * q + 0] = gxy[0][r][2 * q];
+                g[3 * q + 1] = gxy[0][r][2 * q + 1];
+                g[3 * q + 2] = diff32<VIS>(&win[r][q][R], P.dsc[2]);
+            }
+            vflux32_store(P, M, id + r * sx, g, win[r][0][R], win[r][1][R], win[r][2][R], win[r][3][R]);
+        }
+    }
+}
+
+// one workgroup per tile of blockDim.x x blockDim.y column segments (1D
+// grid, XCD remap as xcd_block3)
+template <int VIS, int BC>
+__global__ __launch_bounds__(256) void km_gradflux(DevParams P, MFields M, int seg_len, int gx, int gy)
+{
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    gradflux_tile<VIS, BC>(P, M, seg_len, t % gx, (t / gx) % gy, t / (gx * gy));
+}
+
+template <int VIS>
+__global__ __launch_bounds__(256) void km_gradflux_rows2(DevParams P, MFields M, int seg_len, int gx, int gy)
+{
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    gradflux_tile_rows2<VIS>(P, M, seg_len, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 // the end of a cell's divergence pass: the RHS into `out` (MODE 0), or the
@@ -1017,24 +1119,8 @@ __global__ void km_fill_periodic(DevParams P, MFields M)
 {
     const int axis = blockIdx.y;
     if (!P.periodic[axis]) return;
-    const int ng = P.ng;
-    const int nd = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
-    const int na = axis == 0 ? P.ny : P.nx;
-    const int nb = axis == 2 ? P.ny : P.nz;
-    const int total = 2 * ng * na * nb;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int g = t % (2 * ng);
-    const int rest = t / (2 * ng);
-    const int a = ng + rest % na;
-    const int b = ng + rest / na;
-    int q, qs;
-    if (g < ng) { q = g; qs = g + nd; }
-    else { q = ng + nd + (g - ng); qs = ng + (g - ng); }
     int dst, src;
-    if (axis == 0) { dst = (b * P.sy + a) * P.sx + q; src = (b * P.sy + a) * P.sx + qs; }
-    else if (axis == 1) { dst = (b * P.sy + q) * P.sx + a; src = (b * P.sy + qs) * P.sx + a; }
-    else { dst = (q * P.sy + b) * P.sx + a; src = (qs * P.sy + b) * P.sx + a; }
+    if (!periodic_ghost_pair(P, axis, blockIdx.x * blockDim.x + threadIdx.x, dst, src)) return;
 #pragma unroll
     for (int m = 0; m < 6; ++m) M.S[m][dst] = M.S[m][src];
 }
@@ -1281,23 +1367,38 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
 }
 
 template <int BC>
-static void m_gradflux_vis(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, dim3 g, dim3 b)
+static void m_gradflux_vis(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int nwg, dim3 b, int gx,
+                           int gy)
 {
     switch (P.vis) {
-    case VIS_C8: hipLaunchKernelGGL((km_gradflux<VIS_C8, BC>), g, b, 0, s, P, M, seg_len); break;
-    case VIS_C6: hipLaunchKernelGGL((km_gradflux<VIS_C6, BC>), g, b, 0, s, P, M, seg_len); break;
-    default: hipLaunchKernelGGL((km_gradflux<VIS_C4, BC>), g, b, 0, s, P, M, seg_len); break;
+    case VIS_C8: hipLaunchKernelGGL((km_gradflux<VIS_C8, BC>), dim3(nwg), b, 0, s, P, M, seg_len, gx, gy); break;
+    case VIS_C6: hipLaunchKernelGGL((km_gradflux<VIS_C6, BC>), dim3(nwg), b, 0, s, P, M, seg_len, gx, gy); break;
+    default: hipLaunchKernelGGL((km_gradflux<VIS_C4, BC>), dim3(nwg), b, 0, s, P, M, seg_len, gx, gy); break;
     }
 }
 
-hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by)
+// rows 2: two rows per thread where the box is periodic and ny is a
+// multiple of 2 by (else one)
+hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by, int rows)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
     if (P.vis != VIS_C8 && P.vis != VIS_C6 && P.vis != VIS_C4) return hipErrorInvalidValue;
-    dim3 g((P.nx + bx - 1) / bx, (P.ny + by - 1) / by, (P.nz + seg_len - 1) / seg_len);
+    const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
     dim3 b(bx, by, 1);
-    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) m_gradflux_vis<0>(s, P, M, seg_len, g, b);
-    else m_gradflux_vis<1>(s, P, M, seg_len, g, b);
+    const int gx = (P.nx + bx - 1) / bx, gz = (P.nz + seg_len - 1) / seg_len;
+    if (rows == 2 && per && P.ny % (2 * by) == 0) {
+        const int gy = P.ny / (2 * by);
+        const int n = gx * gy * gz;
+        switch (P.vis) {
+        case VIS_C8: hipLaunchKernelGGL((km_gradflux_rows2<VIS_C8>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        case VIS_C6: hipLaunchKernelGGL((km_gradflux_rows2<VIS_C6>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        default: hipLaunchKernelGGL((km_gradflux_rows2<VIS_C4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        }
+        return hipGetLastError();
+    }
+    const int gy = (P.ny + by - 1) / by;
+    if (per) m_gradflux_vis<0>(s, P, M, seg_len, gx * gy * gz, b, gx, gy);
+    else m_gradflux_vis<1>(s, P, M, seg_len, gx * gy * gz, b, gx, gy);
     return hipGetLastError();
 }
 
