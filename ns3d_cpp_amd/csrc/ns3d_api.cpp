@@ -45,7 +45,9 @@ struct ns3d_ctx {
     FaceArrays faces{};          // allocated on first ns3d_du_inviscous
     int device = 0;
     hipStream_t stream = nullptr;
-    int gf_rows = 2;   // rows per thread of km_gradflux on periodic boxes (NS3D_GF_ROWS)
+    // form of the mixed gradient pass on periodic boxes (NS3D_GF_ROWS): 1 one
+    // row per thread, 2 two rows per thread, 3 two rows with LDS-staged planes
+    int gf_rows = 3;
     // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
     // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
     // faster than eager launches (the kernels outlast the launch cost)
@@ -797,7 +799,10 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     }
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
-    if (const char *env = std::getenv("NS3D_GF_ROWS")) c->gf_rows = std::atoi(env) == 1 ? 1 : 2;
+    if (const char *env = std::getenv("NS3D_GF_ROWS")) {
+        const int v = std::atoi(env);
+        c->gf_rows = v >= 1 && v <= 3 ? v : 2;
+    }
     e = launch_fill_u64(c->stream, c->d_flag, ~0ull, 2);
     if (e == hipSuccess) e = launch_fill_u64(c->stream, c->d_flag + FLAG_STEP, 0ull, 2);   // step counter, time
     if (e == hipSuccess) {

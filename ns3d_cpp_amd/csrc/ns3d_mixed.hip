@@ -895,6 +895,141 @@ __device__ __forceinline__ void gradflux_tile_rows2(const DevParams &P, const MF
     }
 }
 
+// The two-row pass with each entering plane's x / y neighbourhood staged in
+// LDS: a 64 x 4 workgroup (64 x 8 cells) writes its plane tile with an R-cell
+// halo in x (own rows) and y (64 columns) into one of two LDS buffers, one
+// barrier per plane, and takes the x / y derivatives from there.  The own
+// values are the z window's entering plane; the halo costs 2R + 1 loads per
+// thread and plane instead of the 2 (2R + 1) + 2R + 2 stencil loads per
+// quantity of the cached form.  The next plane's values are loaded before the
+// derivative and flux arithmetic of this one, which hides their latency.
+// Needs nx % 64 == 0 and ny % 8 == 0 (no thread leaves before a barrier).
+template <int VIS>
+__global__ __launch_bounds__(256) void km_gradflux_lds(DevParams P, MFields M, int seg_len, int gx, int gy)
+{
+    constexpr int R = VisReach<VIS>::R;
+    constexpr int TR = 8 + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
+    __shared__ float tile[2][4][TR][TC];
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int i0 = P.ng + bx * 64, j0 = P.ng + by * 8;
+    const int i = i0 + tx, j = j0 + 2 * ty;
+    const int k0 = P.ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, P.ng + P.nz);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    const float *const src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
+    // halo cells this thread stages: y-halo rows h = ty, ty + 4, ... < 2R
+    // (tile row h or h + 8), and one x-halo cell (tid < 64 R: quantity
+    // tid / (16 R), own row (tid / 2R) % 8, halo column tid % 2R)
+    constexpr int NYH = (2 * R + 3) / 4;
+    int yh_row[NYH], yh_off[NYH];
+#pragma unroll
+    for (int n = 0; n < NYH; ++n) {
+        const int h = ty + 4 * n;
+        const int tr = h < R ? h : h + 8;
+        yh_row[n] = h < 2 * R ? tr : -1;
+        yh_off[n] = (j0 - R + tr) * sx + i;
+    }
+    const bool xh_on = tid < 64 * R;
+    const int xh_q = xh_on ? tid / (16 * R) : 0;
+    const int xh_r = (tid / (2 * R)) % 8;
+    const int xh_c = tid % (2 * R);
+    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
+    const int xh_off = (j0 + xh_r) * sx + i0 - R + xh_tc;
+    float win[2][4][2 * R + 1];
+    float gxy[R + 1][2][8];
+    float nyh[NYH][4], nxh = 0.0f, nown[2][4];
+    const auto prefetch = [&](int p, bool halo) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nown[r][q] = src[q][p * sxy + col + r * sx];
+        if (halo) {
+#pragma unroll
+            for (int n = 0; n < NYH; ++n)
+                if (yh_row[n] >= 0)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) nyh[n][q] = src[q][p * sxy + yh_off[n]];
+            if (xh_on) nxh = src[xh_q][p * sxy + xh_off];
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int o = 1; o < 2 * R + 1; ++o) win[r][q][o] = src[q][(k0 - R - 1 + o) * sxy + col + r * sx];
+#pragma unroll
+    for (int s = 1; s <= R; ++s)
+        if (k0 + s - 1 < k1) gradxy32_rows2<VIS>(src, (k0 + s - 1) * sxy + col, sx, P.dsc[0], P.dsc[1], gxy[s][0], gxy[s][1]);
+    prefetch(k0 + R, k0 + R < k1);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+        const int p = k + R;   // the entering plane
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) gxy[s][r][q] = gxy[s + 1][r][q];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int o = 0; o < 2 * R; ++o) win[r][q][o] = win[r][q][o + 1];
+                win[r][q][2 * R] = nown[r][q];
+            }
+        const bool stage = p < k1;   // uniform over the workgroup
+        if (stage) {
+            float (*T)[TR][TC] = tile[p & 1];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) T[q][R + 2 * ty + r][R + tx] = nown[r][q];
+#pragma unroll
+            for (int n = 0; n < NYH; ++n)
+                if (yh_row[n] >= 0)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) T[q][yh_row[n]][R + tx] = nyh[n][q];
+            if (xh_on) T[xh_q][R + xh_r][xh_tc] = nxh;
+            __syncthreads();
+        }
+        if (p + 1 < k1 + R) prefetch(p + 1, p + 1 < k1);
+        if (stage) {
+            float (*T)[TR][TC] = tile[p & 1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float wx0[2 * R + 1], wx1[2 * R + 1], wy[2 * R + 2];
+#pragma unroll
+                for (int o = -R; o <= R; ++o) {
+                    wx0[o + R] = T[q][R + 2 * ty][R + tx + o];
+                    wx1[o + R] = T[q][R + 2 * ty + 1][R + tx + o];
+                }
+#pragma unroll
+                for (int o = -R; o <= R + 1; ++o) wy[o + R] = T[q][R + 2 * ty + o][R + tx];
+                gxy[R][0][2 * q + 0] = diff32<VIS>(wx0 + R, P.dsc[0]);
+                gxy[R][1][2 * q + 0] = diff32<VIS>(wx1 + R, P.dsc[0]);
+                gxy[R][0][2 * q + 1] = diff32<VIS>(wy + R, P.dsc[1]);
+                gxy[R][1][2 * q + 1] = diff32<VIS>(wy + R + 1, P.dsc[1]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            float g[12];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                g[3 * q + 0] = gxy[0][r][2 * q];
+                g[3 * q + 1] = gxy[0][r][2 * q + 1];
+                g[3 * q + 2] = diff32<VIS>(&win[r][q][R], P.dsc[2]);
+            }
+            vflux32_store(P, M, id + r * sx, g, win[r][0][R], win[r][1][R], win[r][2][R], win[r][3][R]);
+        }
+    }
+}
+
 // one workgroup per tile of blockDim.x x blockDim.y column segments (1D
 // grid, XCD remap as xcd_block3)
 template <int VIS, int BC>
@@ -1377,8 +1512,10 @@ static void m_gradflux_vis(hipStream_t s, const DevParams &P, const MFields &M, 
     }
 }
 
-// rows 2: two rows per thread where the box is periodic and ny is a
-// multiple of 2 by (else one)
+// rows 3: the LDS-staged two-row form where the box is periodic, the block
+// is 64 x 4 and it tiles x and y exactly; rows 2 (or 3 where that does not
+// hold): two rows per thread where the box is periodic and ny is a multiple
+// of 2 by; else one row per thread
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by, int rows)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
@@ -1386,7 +1523,17 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
     const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
     dim3 b(bx, by, 1);
     const int gx = (P.nx + bx - 1) / bx, gz = (P.nz + seg_len - 1) / seg_len;
-    if (rows == 2 && per && P.ny % (2 * by) == 0) {
+    if (rows == 3 && per && bx == 64 && by == 4 && P.nx % 64 == 0 && P.ny % 8 == 0) {
+        const int gy = P.ny / 8;
+        const int n = gx * gy * gz;
+        switch (P.vis) {
+        case VIS_C8: hipLaunchKernelGGL((km_gradflux_lds<VIS_C8>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        case VIS_C6: hipLaunchKernelGGL((km_gradflux_lds<VIS_C6>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        default: hipLaunchKernelGGL((km_gradflux_lds<VIS_C4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        }
+        return hipGetLastError();
+    }
+    if (rows >= 2 && per && P.ny % (2 * by) == 0) {
         const int gy = P.ny / (2 * by);
         const int n = gx * gy * gz;
         switch (P.vis) {

@@ -158,3 +158,36 @@ def test_mixed_characteristic_trajectory(gpu):
             m.rk3_step(dt)
         errs = [rel_l2(m.interior(a), d.interior(b)) for a, b in zip(m.conserved(), d.conserved())]
         assert max(errs) < 1e-5, (bc, errs)
+
+
+@pytest.mark.parametrize("vis", ["C8th", "C6th", "C4th"])
+def test_gradient_pass_variants_bitwise(gpu, vis, monkeypatch):
+    """The mixed gradient/viscous-flux pass has three forms on periodic boxes:
+    one row per thread with cached stencil loads, two rows per thread sharing
+    the y-neighbour loads, and two rows per thread with each plane's x / y
+    neighbourhood staged in LDS.  They evaluate the same arithmetic on the
+    same values, so the RHS and two RK3 steps must agree bit for bit.  The
+    box is 64 x 32 x 24 (a different spacing per axis) in 2 z segments of
+    unequal length: tile shapes and segment ends that the 256^3 workload
+    does not exercise."""
+    nx, ny, nz = 64, 32, 24
+    rng = np.random.default_rng(7)
+    st, Ma = fields.hit(nz, k0=2, Mt=0.3, seed=11)
+    shape = (nz, ny, nx)
+    x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
+    out = {}
+    for rows in ("1", "2", "3"):
+        monkeypatch.setenv("NS3D_GF_ROWS", rows)
+        monkeypatch.setenv("NS3D_SEG_Z", "16")
+        p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(CFG, vis_scheme=vis))
+        s = ns.Solver(p, g)
+        s.upload_primitive(*x)
+        r = [s.interior(a).copy() for a in s.compute_rhs()]
+        s.rk3_step(1e-3)
+        s.rk3_step(1e-3)
+        u = [s.interior(a).copy() for a in s.conserved()]
+        s.close()
+        out[rows] = (r, u)
+    for rows in ("2", "3"):
+        for a, b in zip(out[rows][0] + out[rows][1], out["1"][0] + out["1"][1]):
+            assert np.array_equal(a, b), rows
