@@ -48,6 +48,7 @@ struct ns3d_ctx {
     // form of the mixed gradient pass on periodic boxes (NS3D_GF_ROWS): 1 one
     // row per thread, 2 two rows per thread, 3 two rows with LDS-staged planes
     int gf_rows = 3;
+    int vd_lds = 1;    // LDS-staged form of the mixed divergence/stage pass on periodic boxes (NS3D_VD_LDS)
     // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
     // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
     // faster than eager launches (the kernels outlast the launch cost)
@@ -281,7 +282,7 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
           launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1], c->gf_rows));
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
           launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, dev_dt, c->m_vd_block[0],
-                        c->m_vd_block[1]));
+                        c->m_vd_block[1], c->vd_lds));
     return NS3D_OK;
 }
 
@@ -799,6 +800,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     }
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_VD_LDS")) c->vd_lds = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GF_ROWS")) {
         const int v = std::atoi(env);
         c->gf_rows = v >= 1 && v <= 3 ? v : 2;

@@ -1050,7 +1050,9 @@ __global__ __launch_bounds__(256) void km_gradflux_rows2(DevParams P, MFields M,
 // RK3 stage update + C2P into U / dU / the shadow (MODE 1..3); with cfl_next
 // the third stage also folds the new state's cell into the CFL maxima, from
 // the shadow values it stores (the values the CFL pass would read)
-template <int MODE>
+// PF: the caller passes the cell's U (u0) and dU (du0) in every mode (else
+// only the third stage does, and the first two load them here)
+template <int MODE, bool PF = false>
 __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M, int id, const double r[5],
                                             const double u0[5], const float du0[5], double dt, unsigned long long *flag,
                                             bool cfl_next, double &cx, double &cy, double &cz)
@@ -1076,10 +1078,10 @@ __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M
         for (int m = 0; m < 5; ++m) {
             float d;
             if constexpr (MODE == 1) d = (float)(dt * r[m]);
-            else if constexpr (MODE == 2) d = (float)(0.25 * ((double)M.dU[m][id] + dt * r[m]));
+            else if constexpr (MODE == 2) d = (float)(0.25 * ((double)(PF ? du0[m] : M.dU[m][id]) + dt * r[m]));
             else d = (float)((2.0 / 3.0) * ((double)du0[m] + dt * r[m]));
             M.dU[m][id] = d;
-            Un[m] = (MODE == 3 ? u0[m] : M.U[m][id]) + (double)d;
+            Un[m] = (MODE == 3 || PF ? u0[m] : M.U[m][id]) + (double)d;
         }
         double rn = Un[0];
         if (rn <= 0.0) rn = 1e-12;
@@ -1199,6 +1201,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             r[m + 1] = (double)vd - (double)an[m + 1];
         }
         vdiv_finish<MODE>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
+    }
+    if (cfl_next) {
+        const double cm = wave_min(cfl_bound(P, cx, cy, cz));
+        if (__lane_id() == 0) atomicMin(flag + FLAG_CFL_NEXT, (unsigned long long)__double_as_longlong(cm));
+    }
+}
+
+// km_vdiv_z on a periodic box with each plane's x / y stencil fluxes staged
+// in LDS: a 64 x 4 workgroup writes its plane's x fluxes with an R-cell x
+// halo and its y fluxes with an R-row y halo (periodic images of interior
+// cells: the flux arrays hold no ghosts) into one of two buffers, one
+// barrier per plane, and takes both derivatives from there.  Everything the
+// next plane needs (own fluxes, halos, the entering z-flux plane, A, U, dU)
+// is loaded before this plane's arithmetic, so the loads of a plane overlap
+// the arithmetic of the previous one: about 35 loads per cell instead of 91.
+// The same arithmetic in the same order as km_vdiv_z<VIS, MODE, 0>: equal bit
+// for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before a
+// barrier).
+template <int VIS, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
+    DevParams P, MFields M, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
+{
+    constexpr int R = VisReach<VIS>::R;
+    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
+    __shared__ float fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
+    __shared__ float fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
+    const int ng = P.ng;
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int i0 = ng + bx * 64, j0 = ng + by * 4;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, ng + P.nz);
+    if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    // the x-halo cell of this thread (tid < 32 R): flux tid / 8R, tile row
+    // (tid / 2R) % 4, halo column tid % 2R
+    const bool xh_on = tid < 32 * R;
+    const int xh_q = xh_on ? tid / (8 * R) : 0;
+    const int xh_r = (tid / (2 * R)) % 4;
+    const int xh_c = tid % (2 * R);
+    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
+    const int xh_off = (j0 + xh_r) * sx + wrapi_m(i0 - R + xh_tc, ng, P.nx);
+    // the y-halo rows of this thread: h = ty, ty + 4, ... < 2R, tile row h
+    // (below the tile) or h + 4 (above it)
+    constexpr int NYH = (2 * R + 3) / 4;
+    int yh_row[NYH], yh_off[NYH];
+#pragma unroll
+    for (int n = 0; n < NYH; ++n) {
+        const int h = ty + 4 * n;
+        const int tr = h < R ? h : h + 4;
+        yh_row[n] = h < 2 * R ? tr : -1;
+        yh_off[n] = wrapi_m(j0 - R + tr, ng, P.ny) * sx + i;
+    }
+    // the y fluxes are VF[4..7] with VF[4] = VF[1]: comps 1..3 from VF[5..7]
+    float pfx[4], pfy[3], pxh = 0.0f, pyh[NYH][4], pz[4], pa[5], pdu[5];
+    double pu[5];
+    const auto prefetch = [&](int kp) {
+        const int pl = kp * sxy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pfx[q] = M.VF[q][pl + col];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) pfy[q] = M.VF[5 + q][pl + col];
+        if (xh_on) pxh = M.VF[xh_q][pl + xh_off];
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0) {
+                pyh[n][0] = M.VF[1][pl + yh_off[n]];
+#pragma unroll
+                for (int q = 1; q < 4; ++q) pyh[n][q] = M.VF[4 + q][pl + yh_off[n]];
+            }
+        const int zn = wrapi_m(kp + R, ng, P.nz) * sxy + col;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pz[m] = M.VF[8 + m][zn];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            pa[m] = M.A[m][pl + col];
+            if constexpr (MODE != 0) pu[m] = M.U[m][pl + col];
+            if constexpr (MODE >= 2) pdu[m] = M.dU[m][pl + col];
+        }
+    };
+    float win[4][2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o)
+            win[m][o] = M.VF[8 + m][wrapi_m(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    prefetch(k0);
+    const bool cfl_next = MODE == 3 && dev_dt == 2;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
+            win[m][2 * R] = pz[m];
+        }
+        float (*FX)[4][TC] = fxt[k & 1];
+        float (*FY)[TYR][64] = fyt[k & 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) FX[q][ty][R + tx] = pfx[q];
+        if (xh_on) FX[xh_q][xh_r][xh_tc] = pxh;
+        FY[0][R + ty][tx] = pfx[1];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) FY[q][R + ty][tx] = pfy[q - 1];
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) FY[q][yh_row[n]][tx] = pyh[n][q];
+        float an[5], du0[5];
+        double u0[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            an[m] = pa[m];
+            u0[m] = MODE != 0 ? pu[m] : 0.0;
+            du0[m] = MODE >= 2 ? pdu[m] : 0.0f;
+        }
+        __syncthreads();
+        if (k + 1 < k1) prefetch(k + 1);
+        double r[5];
+        r[0] = -(double)an[0];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            float wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+            for (int o = -R; o <= R; ++o) {
+                wx[o + R] = FX[m][ty][R + tx + o];
+                wy[o + R] = FY[m][R + ty + o][tx];
+            }
+            const float vd = diff32<VIS>(wx + R, P.dsc[0]) + diff32<VIS>(wy + R, P.dsc[1]) +
+                             diff32<VIS>(&win[m][R], P.dsc[2]);
+            r[m + 1] = (double)vd - (double)an[m + 1];
+        }
+        vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
     }
     if (cfl_next) {
         const double cm = wave_min(cfl_bound(P, cx, cy, cz));
@@ -1581,12 +1721,38 @@ bool m_vdiv_cfl_next(const DevParams &P, int bx, int by)
     return (bx * by) % 64 == 0 && P.nx % bx == 0 && P.ny % by == 0;
 }
 
+template <int VIS>
+static void m_vdiv_lds(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                       unsigned long long *flag, int dev_dt)
+{
+    const int gx = P.nx / 64, gy = P.ny / 4, n = gx * gy * ((P.nz + seg - 1) / seg);
+    dim3 b(64, 4, 1);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((km_vdiv_lds<VIS, 0>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_lds<VIS, 1>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_lds<VIS, 2>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((km_vdiv_lds<VIS, 3>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    }
+}
+
+// lds: the LDS-staged form where the box is periodic, the block is 64 x 4
+// and it tiles x and y exactly
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, int dev_dt, int bx, int by)
+                         unsigned long long *flag, int dev_dt, int bx, int by, int lds)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
     if (dev_dt == 2 && !m_vdiv_cfl_next(P, bx, by)) return hipErrorInvalidValue;   // partial waves
-    if (P.periodic[0] && P.periodic[1] && P.periodic[2])
+    const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
+    if (lds && per && bx == 64 && by == 4 && P.nx % 64 == 0 && P.ny % 4 == 0) {
+        switch (P.vis) {
+        case VIS_C8: m_vdiv_lds<VIS_C8>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C6: m_vdiv_lds<VIS_C6>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C4: m_vdiv_lds<VIS_C4>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    if (per)
         return m_vdiv_vis<0>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);
     return m_vdiv_vis<1>(s, P, M, mode, dt, seg, flag, dev_dt, bx, by);
 }

@@ -161,12 +161,15 @@ def test_mixed_characteristic_trajectory(gpu):
 
 
 @pytest.mark.parametrize("vis", ["C8th", "C6th", "C4th"])
-def test_gradient_pass_variants_bitwise(gpu, vis, monkeypatch):
-    """The mixed gradient/viscous-flux pass has three forms on periodic boxes:
-    one row per thread with cached stencil loads, two rows per thread sharing
-    the y-neighbour loads, and two rows per thread with each plane's x / y
-    neighbourhood staged in LDS.  They evaluate the same arithmetic on the
-    same values, so the RHS and two RK3 steps must agree bit for bit.  The
+def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
+    """The mixed viscous passes have several forms on periodic boxes.  The
+    gradient/flux pass has one row per thread with cached stencil loads, two
+    rows per thread sharing the y-neighbour loads, and two rows per thread
+    with each plane's x / y neighbourhood staged in LDS.  The divergence and
+    stage pass has cached stencil loads or an LDS-staged plane.  All of them
+    evaluate the same arithmetic on the same values, so the RHS, two RK3
+    steps at a fixed dt and two CFL steps with the device time step (the
+    third stage reducing the next CFL minimum) must agree bit for bit.  The
     box is 64 x 32 x 24 (a different spacing per axis) in 2 z segments of
     unequal length: tile shapes and segment ends that the 256^3 workload
     does not exercise."""
@@ -176,8 +179,10 @@ def test_gradient_pass_variants_bitwise(gpu, vis, monkeypatch):
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    for rows in ("1", "2", "3"):
+    forms = [("1", "0"), ("2", "0"), ("3", "0"), ("1", "1"), ("3", "1")]
+    for rows, lds in forms:
         monkeypatch.setenv("NS3D_GF_ROWS", rows)
+        monkeypatch.setenv("NS3D_VD_LDS", lds)
         monkeypatch.setenv("NS3D_SEG_Z", "16")
         p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(CFG, vis_scheme=vis))
         s = ns.Solver(p, g)
@@ -186,8 +191,12 @@ def test_gradient_pass_variants_bitwise(gpu, vis, monkeypatch):
         s.rk3_step(1e-3)
         s.rk3_step(1e-3)
         u = [s.interior(a).copy() for a in s.conserved()]
+        t = s.advance_cfl(2)
+        u2 = [s.interior(a).copy() for a in s.conserved()]
         s.close()
-        out[rows] = (r, u)
-    for rows in ("2", "3"):
-        for a, b in zip(out[rows][0] + out[rows][1], out["1"][0] + out["1"][1]):
-            assert np.array_equal(a, b), rows
+        out[(rows, lds)] = (r + u + u2, t)
+    ref = out[forms[0]]
+    for f in forms[1:]:
+        assert out[f][1] == ref[1], f
+        for a, b in zip(out[f][0], ref[0]):
+            assert np.array_equal(a, b), f
