@@ -48,9 +48,9 @@ def kind_of(name):
         return ["sweep_x", "sweep_y", "sweep_z"][int(m.group(1))]
     if "gradflux" in n:
         return "gradflux"
-    m = re.search(r"km?_vdiv(_z)?<\d+, (\d)(?:, \d)?>", n)
+    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, \d)?>", n)
     if m:
-        return "vdiv_rhs" if m.group(2) == "0" else "vdiv_stage"
+        return "vdiv_rhs" if m.group(1) == "0" else "vdiv_stage"
     if "fill_periodic" in n or "fill_bc" in n:
         return "fill_ghosts"
     if "k_cfl" in n or "km_cfl" in n:
@@ -64,8 +64,8 @@ def kind_of(name):
 
 def stage_of(name):
     """RK3 stage (1..3) of a stage-kernel instantiation, else None."""
-    m = re.search(r"km?_vdiv(_z)?<\d+, (\d)(?:, \d)?>", name.replace("ns3d::", ""))
-    return int(m.group(2)) if m and m.group(2) != "0" else None
+    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, \d)?>", name.replace("ns3d::", ""))
+    return int(m.group(1)) if m and m.group(1) != "0" else None
 
 
 def short(name):
