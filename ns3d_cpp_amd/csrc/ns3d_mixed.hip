@@ -263,16 +263,23 @@ __global__ __launch_bounds__(64, 2) void km_sweep64(DevParams P, MFields M, int 
 // of the candidate polynomials goes into the reciprocal's argument: a float
 // 1/6 applied after it scaled every reconstructed difference of a run by
 // the same factor 1 + 3.0e-8, and that bias accumulated over the steps.
-__device__ __forceinline__ float rel32_weno5(const float *D)
+// The first term of a WENO5 smoothness indicator, scaled by 4 (the weights
+// see ratios only): 4 eps + (13/3) s^2, s the stencil's second difference.
+// A three-cell stencil's s is the same at the three faces that use it, so the
+// marching sweeps carry it from face to face.
+__device__ __forceinline__ float weno5_u(float s)
+{
+    return fmaf((13.0f / 3.0f) * s, s, 4e-6f);
+}
+
+// rel32_weno5 with the smoothness indicators' first terms u0, u1, u2 given
+__device__ __forceinline__ float rel32_weno5_u(const float *D, float u0, float u1, float u2)
 {
     const float D1 = D[1], D2 = D[2], D3 = D[3], D4 = D[4];
-    const float c13 = 13.0f / 3.0f;
-    const float s0 = D2 - D1, r0 = 3.0f * D2 - D1;
-    const float s1 = D3 - D2, r1 = D2 + D3;
-    const float s2 = D4 - D3, r2 = D4 - 3.0f * D3;
-    const float t0 = 4e-6f + c13 * s0 * s0 + r0 * r0;
-    const float t1 = 4e-6f + c13 * s1 * s1 + r1 * r1;
-    const float t2 = 4e-6f + c13 * s2 * s2 + r2 * r2;
+    const float r0 = 3.0f * D2 - D1, r1 = D2 + D3, r2 = D4 - 3.0f * D3;
+    const float t0 = fmaf(r0, r0, u0);
+    const float t1 = fmaf(r1, r1, u1);
+    const float t2 = fmaf(r2, r2, u2);
     float T0 = t0 * t0, T1 = t1 * t1, T2 = t2 * t2;
     // scale by the power of two 2^(127 - e), e the biased exponent of the
     // largest T (T >= 1.6e-11 > 0, so e is in 91..254): the largest scaled T
@@ -286,6 +293,11 @@ __device__ __forceinline__ float rel32_weno5(const float *D)
     const float q1 = D2 + 2.0f * D3;
     const float q2 = 4.0f * D3 - D4;
     return (a0 * q0 + a1 * q1 + a2 * q2) * rcpf(6.0f * (a0 + a1 + a2));
+}
+
+__device__ __forceinline__ float rel32_weno5(const float *D)
+{
+    return rel32_weno5_u(D, weno5_u(D[2] - D[1]), weno5_u(D[3] - D[2]), weno5_u(D[4] - D[3]));
 }
 
 // coefficients e[t] of a linear scheme in difference form:
@@ -528,6 +540,11 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
     constexpr int L = S - T0;
     constexpr int R0 = 5;                     // components 5..9 (minus side) are in the ring
     __shared__ float ring[L][5][64];
+    // WENO5: the minus side's smoothness first terms of the last two faces,
+    // slot f & 1 for face f (weno5_u).  Not with the eps split, whose extra
+    // registers would leave the carry at 146 VGPRs (3 waves per SIMD)
+    constexpr bool CARRY = RECON == R_WENO5 && FVS != FVS_SWE;
+    __shared__ float uring[CARRY ? 2 : 1][5][64];
     int bs, bt, bb;
     xcd_block3(bs, bt, bb);   // grid (nseg, lane tiles, b), as the launch
     const int lane = threadIdx.x;
@@ -601,6 +618,24 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
         for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
     }
     float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // WENO5: each face's three-cell stencils share their second difference
+    // with the neighbouring faces.  After the push of face f the plus side's
+    // stencils are positions (1,2,3), (2,3,4), (3,4,5) and only the last is
+    // new; the minus side's are (5,4,3), (4,3,2), (3,2,1) read backwards and
+    // only the first is new.  The plus side carries the older two terms in
+    // up0 / up1, the minus side in uring.  Before the first face's push the
+    // window holds positions 2..5 of face -1's numbering shifted down by one.
+    float up0[5], up1[5];
+    if constexpr (CARRY) {
+        const int f0 = c0 - 1;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            up0[m] = weno5_u(wp[m][3] - wp[m][2]);
+            up1[m] = weno5_u(wp[m][4] - wp[m][3]);
+            uring[(f0 + 1) & 1][m][lane] = weno5_u(get(5 + m, 5) - get(5 + m, 4));   // s1 of face f0
+            uring[f0 & 1][m][lane] = weno5_u(get(5 + m, 4) - get(5 + m, 3));         // s2 of face f0
+        }
+    }
     // the accumulator of the next cell written, loaded an iteration ahead
     // (a load at the point of the update left its latency exposed)
     float an[5];
@@ -634,7 +669,18 @@ __global__ __launch_bounds__(64) void km_sweep(DevParams P, MFields M, int seg_l
                 D[t] = get(m, t);
                 Dm[t] = -get(5 + m, S - t);
             }
-            rel[m] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
+            if constexpr (CARRY) {
+                const float up2 = weno5_u(D[4] - D[3]);
+                const float um0 = weno5_u(Dm[2] - Dm[1]);
+                const float um1 = uring[(f + 1) & 1][m][lane];
+                const float um2 = uring[f & 1][m][lane];
+                uring[f & 1][m][lane] = um0;
+                rel[m] = rel32_weno5_u(D, up0[m], up1[m], up2) + rel32_weno5_u(Dm, um0, um1, um2);
+                up0[m] = up1[m];
+                up1[m] = up2;
+            } else {
+                rel[m] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
+            }
         }
         if (f >= c0) {
             const int idc = base + f * st;

@@ -200,3 +200,48 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
         assert out[f][1] == ref[1], f
         for a, b in zip(out[f][0], ref[0]):
             assert np.array_equal(a, b), f
+
+
+@pytest.mark.parametrize("recon,fvs,eps", [("WENO5", "StagerWarming", 0.0), ("WENO5", "LaxFriedrichs", 0.0),
+                                           ("WENO5", "StagerWarming", 0.1), ("UPWIND_7TH", "StagerWarming", 0.0)])
+def test_one_dimensional_flow_is_the_same_along_each_axis(gpu, recon, fvs, eps):
+    """A flow that varies along one axis only, with velocity along that axis
+    only, on a periodic 64^3 cube: the x sweep (whole rows in LDS, WENO5's
+    smoothness terms computed once per slot), the y and z sweeps (marching
+    columns, the terms carried from face to face), the LDS-staged viscous
+    passes and the stage update must give the same numbers along each axis,
+    bit for bit: the RHS and three CFL steps (the tangential velocities stay
+    exactly zero, so every sum meets the same nonzero terms in the same
+    order)."""
+    n = 64
+    x = np.arange(n) * (2 * np.pi / n)
+    rng = np.random.default_rng(3)
+    r = 1.0 + 0.05 * np.sin(x) + 0.01 * rng.standard_normal(n)
+    un = 0.3 * np.cos(2 * x) + 0.02 * rng.standard_normal(n)
+    pr = (1.0 + 0.1 * np.sin(3 * x + 0.4)) / (1.4 * 0.3 ** 2)
+    out = []
+    for axis in range(3):
+        shape = [1, 1, 1]
+        shape[2 - axis] = n      # arrays are (k, j, i)
+        b = lambda a: np.broadcast_to(a.reshape(shape), (n, n, n)).copy()  # noqa: E731
+        vel = [np.zeros((n, n, n)) for _ in range(3)]
+        vel[axis] = b(un)
+        p, g = ns.make_params(n, precision="mixed", Ma=0.3, Re=500.0, Pr=0.72, recon=recon, fvs_type=fvs,
+                              vis_scheme="C8th", sutherland_s=110.4 / 273.15, cfl=0.5, sw_eps=eps)
+        s = ns.Solver(p, g)
+        s.upload_primitive(b(r), vel[0], vel[1], vel[2], b(pr))
+        rhs = [s.interior(a).copy() for a in s.compute_rhs()]
+        t = s.advance_cfl(3)
+        u = [s.interior(a).copy() for a in s.conserved()]
+        s.close()
+        out.append((rhs, u, t))
+    perm = {1: (0, 2, 1), 2: (2, 1, 0)}
+    for axis in (1, 2):
+        assert out[axis][2] == out[0][2], axis
+        for ref, got in ((out[0][0], out[axis][0]), (out[0][1], out[axis][1])):
+            # the x run's normal momentum (component 1) is this run's 1 + axis
+            order = list(range(5))
+            order[1], order[1 + axis] = order[1 + axis], order[1]
+            for m in range(5):
+                want = np.transpose(ref[order[m]], perm[axis])
+                assert np.array_equal(got[m], want), (axis, m)
