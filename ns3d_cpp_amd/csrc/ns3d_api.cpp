@@ -49,6 +49,7 @@ struct ns3d_ctx {
     // row per thread, 2 two rows per thread, 3 two rows with LDS-staged planes
     int gf_rows = 3;
     int vd_lds = 1;    // LDS-staged form of the mixed divergence/stage pass on periodic boxes (NS3D_VD_LDS)
+    int vd_img = 1;    // its stages write the shadow's periodic ghost images, no fill pass (NS3D_VD_IMG)
     // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
     // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
     // faster than eager launches (the kernels outlast the launch cost)
@@ -282,7 +283,7 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
           launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1], c->gf_rows));
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
           launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, dev_dt, c->m_vd_block[0],
-                        c->m_vd_block[1], c->vd_lds));
+                        c->m_vd_block[1], c->vd_lds, c->vd_img));
     return NS3D_OK;
 }
 
@@ -341,11 +342,16 @@ int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0, bool first = true)
         int st = cfl_enqueue(c, dev_dt == 2);
         if (st) return st;
     }
+    // the mixed LDS-staged stage kernel writes the periodic ghost images
+    // itself (vd_img); otherwise each stage is followed by the fill pass
+    const bool images = c->mixed && c->vd_img && m_vdiv_lds_form(c->dp, c->m_vd_block[0], c->m_vd_block[1], c->vd_lds);
     for (int stage = 1; stage <= 3; ++stage) {
         int st = rhs_pass(c, stage, dt, dev_dt);
         if (st) return st;
-        st = fill_ghosts(c);
-        if (st) return st;
+        if (!images) {
+            st = fill_ghosts(c);
+            if (st) return st;
+        }
     }
     return step_end(c, dev_dt);
 }
@@ -801,6 +807,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_VD_LDS")) c->vd_lds = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_VD_IMG")) c->vd_img = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GF_ROWS")) {
         const int v = std::atoi(env);
         c->gf_rows = v >= 1 && v <= 3 ? v : 2;

@@ -101,7 +101,8 @@ hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, i
 hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M, int seg_len, int bx, int by,
                              int rows);
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, int dev_dt, int bx = 64, int by = 4, int lds = 0);
+                         unsigned long long *flag, int dev_dt, int bx = 64, int by = 4, int lds = 0, int img = 0);
+bool m_vdiv_lds_form(const DevParams &P, int bx, int by, int lds);
 // the workgroup shape tiles the interior with whole waves: km_vdiv_z can
 // reduce the next step's CFL minimum (dev_dt 2)
 bool m_vdiv_cfl_next(const DevParams &P, int bx, int by);

@@ -1098,10 +1098,11 @@ __global__ __launch_bounds__(256) void km_gradflux_rows2(DevParams P, MFields M,
 // the shadow values it stores (the values the CFL pass would read)
 // PF: the caller passes the cell's U (u0) and dU (du0) in every mode (else
 // only the third stage does, and the first two load them here)
+// sh: where given (MODE 1..3), also receives the six shadow values stored
 template <int MODE, bool PF = false>
 __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M, int id, const double r[5],
                                             const double u0[5], const float du0[5], double dt, unsigned long long *flag,
-                                            bool cfl_next, double &cx, double &cy, double &cz)
+                                            bool cfl_next, double &cx, double &cy, double &cz, float *sh = nullptr)
 {
     if constexpr (MODE == 0) {
 #pragma unroll
@@ -1148,6 +1149,10 @@ __device__ __forceinline__ void vdiv_finish(const DevParams &P, const MFields &M
         M.S[3][id] = s3;
         M.S[4][id] = s4;
         M.S[5][id] = (float)(Tn - 1.0);
+        if (sh) {
+            sh[0] = s0; sh[1] = s1; sh[2] = s2; sh[3] = s3; sh[4] = s4;
+            sh[5] = (float)(Tn - 1.0);
+        }
         if constexpr (MODE == 3) {
             if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
                 atomicMin(flag + FLAG_BAD, nonphysical_key(flag, id));
@@ -1265,7 +1270,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // The same arithmetic in the same order as km_vdiv_z<VIS, MODE, 0>: equal bit
 // for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before a
 // barrier).
-template <int VIS, int MODE>
+// IMG (MODE 1..3): the stage also writes the new shadow's periodic ghost
+// images (the face layers of the fill pass), so that pass can be skipped
+template <int VIS, int MODE, bool IMG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
     DevParams P, MFields M, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
@@ -1384,7 +1391,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                              diff32<VIS>(&win[m][R], P.dsc[2]);
             r[m + 1] = (double)vd - (double)an[m + 1];
         }
-        vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
+        if constexpr (IMG && MODE != 0) {
+            float sh[6];
+            vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz, sh);
+            // the cell's images in the ghost layers of each axis: i < 2 ng is
+            // the right ghost i + n, i >= n the left ghost i - n
+            const int im[3] = {i, j, k};
+            const int nn[3] = {P.nx, P.ny, P.nz};
+            const int str[3] = {1, sx, sxy};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (im[a] < 2 * ng) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) M.S[q][id + nn[a] * str[a]] = sh[q];
+                }
+                if (im[a] >= nn[a]) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) M.S[q][id - nn[a] * str[a]] = sh[q];
+                }
+            }
+        } else {
+            vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
+        }
     }
     if (cfl_next) {
         const double cm = wave_min(cfl_bound(P, cx, cy, cz));
@@ -1767,33 +1795,44 @@ bool m_vdiv_cfl_next(const DevParams &P, int bx, int by)
     return (bx * by) % 64 == 0 && P.nx % bx == 0 && P.ny % by == 0;
 }
 
-template <int VIS>
+template <int VIS, bool IMG>
 static void m_vdiv_lds(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                        unsigned long long *flag, int dev_dt)
 {
     const int gx = P.nx / 64, gy = P.ny / 4, n = gx * gy * ((P.nz + seg - 1) / seg);
     dim3 b(64, 4, 1);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((km_vdiv_lds<VIS, 0>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    case 1: hipLaunchKernelGGL((km_vdiv_lds<VIS, 1>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    case 2: hipLaunchKernelGGL((km_vdiv_lds<VIS, 2>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    default: hipLaunchKernelGGL((km_vdiv_lds<VIS, 3>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 0: hipLaunchKernelGGL((km_vdiv_lds<VIS, 0, false>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_lds<VIS, 1, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_lds<VIS, 2, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((km_vdiv_lds<VIS, 3, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
     }
 }
 
+// whether launch_m_vdiv takes the LDS-staged form for these arguments
+bool m_vdiv_lds_form(const DevParams &P, int bx, int by, int lds)
+{
+    const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
+    return lds && per && bx == 64 && by == 4 && P.nx % 64 == 0 && P.ny % 4 == 0;
+}
+
 // lds: the LDS-staged form where the box is periodic, the block is 64 x 4
-// and it tiles x and y exactly
+// and it tiles x and y exactly; with img the stages (mode 1..3) of that
+// form also write the shadow's periodic ghost images
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
-                         unsigned long long *flag, int dev_dt, int bx, int by, int lds)
+                         unsigned long long *flag, int dev_dt, int bx, int by, int lds, int img)
 {
     if (bx * by > 256 || bx % 32) return hipErrorInvalidValue;
     if (dev_dt == 2 && !m_vdiv_cfl_next(P, bx, by)) return hipErrorInvalidValue;   // partial waves
     const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
-    if (lds && per && bx == 64 && by == 4 && P.nx % 64 == 0 && P.ny % 4 == 0) {
+    if (m_vdiv_lds_form(P, bx, by, lds)) {
         switch (P.vis) {
-        case VIS_C8: m_vdiv_lds<VIS_C8>(s, P, M, mode, dt, seg, flag, dev_dt); break;
-        case VIS_C6: m_vdiv_lds<VIS_C6>(s, P, M, mode, dt, seg, flag, dev_dt); break;
-        case VIS_C4: m_vdiv_lds<VIS_C4>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C8: img ? m_vdiv_lds<VIS_C8, true>(s, P, M, mode, dt, seg, flag, dev_dt)
+                         : m_vdiv_lds<VIS_C8, false>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C6: img ? m_vdiv_lds<VIS_C6, true>(s, P, M, mode, dt, seg, flag, dev_dt)
+                         : m_vdiv_lds<VIS_C6, false>(s, P, M, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C4: img ? m_vdiv_lds<VIS_C4, true>(s, P, M, mode, dt, seg, flag, dev_dt)
+                         : m_vdiv_lds<VIS_C4, false>(s, P, M, mode, dt, seg, flag, dev_dt); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

@@ -166,10 +166,12 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
     gradient/flux pass has one row per thread with cached stencil loads, two
     rows per thread sharing the y-neighbour loads, and two rows per thread
     with each plane's x / y neighbourhood staged in LDS.  The divergence and
-    stage pass has cached stencil loads or an LDS-staged plane.  All of them
-    evaluate the same arithmetic on the same values, so the RHS, two RK3
-    steps at a fixed dt and two CFL steps with the device time step (the
-    third stage reducing the next CFL minimum) must agree bit for bit.  The
+    stage pass has cached stencil loads or an LDS-staged plane, the latter
+    writing the new shadow's periodic ghost images itself (no fill pass) or
+    not.  All of them evaluate the same arithmetic on the same values, so the
+    RHS, two RK3 steps at a fixed dt, two CFL steps with the device time step
+    (the third stage reducing the next CFL minimum) and the downloaded
+    primitives, ghost layers included, must agree bit for bit.  The
     box is 64 x 32 x 24 (a different spacing per axis) in 2 z segments of
     unequal length: tile shapes and segment ends that the 256^3 workload
     does not exercise."""
@@ -179,10 +181,11 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    forms = [("1", "0"), ("2", "0"), ("3", "0"), ("1", "1"), ("3", "1")]
-    for rows, lds in forms:
+    forms = [("1", "0", "0"), ("2", "0", "0"), ("3", "0", "0"), ("1", "1", "0"), ("3", "1", "0"), ("3", "1", "1")]
+    for rows, lds, img in forms:
         monkeypatch.setenv("NS3D_GF_ROWS", rows)
         monkeypatch.setenv("NS3D_VD_LDS", lds)
+        monkeypatch.setenv("NS3D_VD_IMG", img)
         monkeypatch.setenv("NS3D_SEG_Z", "16")
         p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(CFG, vis_scheme=vis))
         s = ns.Solver(p, g)
@@ -193,8 +196,9 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
         u = [s.interior(a).copy() for a in s.conserved()]
         t = s.advance_cfl(2)
         u2 = [s.interior(a).copy() for a in s.conserved()]
+        w = list(s.primitive().values())
         s.close()
-        out[(rows, lds)] = (r + u + u2, t)
+        out[(rows, lds, img)] = (r + u + u2 + w, t)
     ref = out[forms[0]]
     for f in forms[1:]:
         assert out[f][1] == ref[1], f
