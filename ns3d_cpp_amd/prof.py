@@ -48,7 +48,7 @@ def kind_of(name):
         return ["sweep_x", "sweep_y", "sweep_z"][int(m.group(1))]
     if "gradflux" in n:
         return "gradflux"
-    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, \d)?>", n)
+    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, (?:\d|true|false))?>", n)
     if m:
         return "vdiv_rhs" if m.group(1) == "0" else "vdiv_stage"
     if "fill_periodic" in n or "fill_bc" in n:
@@ -64,7 +64,7 @@ def kind_of(name):
 
 def stage_of(name):
     """RK3 stage (1..3) of a stage-kernel instantiation, else None."""
-    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, \d)?>", name.replace("ns3d::", ""))
+    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, (?:\d|true|false))?>", name.replace("ns3d::", ""))
     return int(m.group(1)) if m and m.group(1) != "0" else None
 
 
