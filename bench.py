@@ -235,6 +235,43 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
     store_all = {"ckpt_every": nsteps, "adj_forward_ms": round(a_f.value, 3), "adj_backward_ms": round(a_b.value, 3),
                  "cost_vs_forward": round((a_f.value + a_b.value) / fwd.value, 3),
                  "device_GB": round((s.adj_memory() + base) / 1e9, 3), "gradient_bitwise_equal": bool(same)}
+    # the same gradient from the context switched to mixed precision: mixed
+    # steps forward and recomputed, the FP64 scheme's derivatives at each
+    # stage state (checkpoint every ckpt steps as above)
+    s.set_precision("mixed")
+    s.upload_conserved(*U0)
+    s.advance(2, dt)
+    s.upload_conserved(*U0)
+    s.adj_forward(2, dt, 1)
+    s.adj_backward()
+    s.upload_conserved(*U0)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    s.advance(nsteps, dt)
+    H.hipEventRecord(ev[1], s.stream())
+    H.hipEventSynchronize(ev[1])
+    m_fwd = C.c_float()
+    H.hipEventElapsedTime(C.byref(m_fwd), ev[0], ev[1])
+    s.upload_conserved(*U0)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    Jm = s.adj_forward(nsteps, dt, ckpt)
+    H.hipEventRecord(ev[1], s.stream())
+    lamm, thm = s.adj_backward()
+    H.hipEventRecord(ev[2], s.stream())
+    H.hipEventSynchronize(ev[2])
+    m_f, m_b = C.c_float(), C.c_float()
+    H.hipEventElapsedTime(C.byref(m_f), ev[0], ev[1])
+    H.hipEventElapsedTime(C.byref(m_b), ev[1], ev[2])
+    gdiff = float(np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(lamm, lam0))))
+    m_tot = m_f.value + m_b.value
+    mixed = {"forward_ms": round(m_fwd.value, 3), "adj_forward_ms": round(m_f.value, 3),
+             "adj_backward_ms": round(m_b.value, 3), "adjoint_total_ms": round(m_tot, 3),
+             "cost_vs_mixed_forward": round(m_tot / m_fwd.value, 3),
+             "cost_vs_fp64_forward": round(m_tot / fwd.value, 3),
+             "J_rel_diff_vs_fp64": abs(Jm - J) / abs(J), "dJ_dinvRe_rel_diff_vs_fp64": abs(thm - dJ_dinvRe) / abs(dJ_dinvRe),
+             "grad_rel_l2_vs_fp64": gdiff / gnorm,
+             "device_GB": round((s.adj_memory() + base + ns.lib().ns3d_cells(s.ctx) * 20 * 4) / 1e9, 3)}
     s.close()
     total = t_f.value + t_b.value
     return {"config": f"isotropic turbulence {n}^3 FP64, k0=4, Mt=0.3, Re=100, WENO5+SW+C8, {nsteps} RK3 steps, "
@@ -244,7 +281,7 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
             "adj_backward_ms": round(t_b.value, 3), "adjoint_total_ms": round(total, 3),
             "cost_vs_forward": round(total / fwd.value, 3),
             "device_bytes": int(mem + base), "device_GB": round((mem + base) / 1e9, 3),
-            "store_all_stages": store_all,
+            "store_all_stages": store_all, "mixed": mixed,
             "kernels_ms": {k: round(v[0], 2) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}}
 
 

@@ -269,12 +269,16 @@ int ns3d_cfl_dt_noalloc(const ns3d_params *p, const ns3d_grid *g, const double *
                         const double *v, const double *w, const double *pr, double *dt_dev, void *stream);
 
 /* ---- Differentiation (no counterpart in the reference, which has no
- * derivative code).  FP64 contexts on a periodic box with a component-wise
- * WENO5 / linear / MDCD / UD7 / UD7L reconstruction and every grid extent
- * >= 8; other settings return NS3D_ERR_UNSUPPORTED.  Host arrays are full
- * sx*sy*sz layouts; only interior cells are read or written.  |x| in the
- * plain SW/LF splits is differentiated as sign(x); sw_eps > 0 makes the
- * split smooth. */
+ * derivative code).  A periodic box with a component-wise WENO5 / linear /
+ * MDCD / UD7 / UD7L reconstruction and every grid extent >= 8; other
+ * settings return NS3D_ERR_UNSUPPORTED.  Host arrays are full sx*sy*sz
+ * layouts; only interior cells are read or written.  |x| in the plain SW/LF
+ * splits is differentiated as sign(x); sw_eps > 0 makes the split smooth.
+ * Mixed-precision contexts: the forward and recomputed steps are the mixed
+ * steps (checkpoints hold the FP64 state and its FP32 shadow, so the
+ * recomputation is bitwise), and the derivatives are those of the FP64
+ * scheme taken at the FP64 primitives of each stage's state (U, or U plus
+ * the stage increment). */
 
 /* Tangent-linear RHS at the current state: out = dR/dU * dU. */
 int ns3d_tlm_rhs(ns3d_ctx *ctx, const double *const dU[5], double *const out[5]);

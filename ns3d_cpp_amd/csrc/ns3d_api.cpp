@@ -333,6 +333,18 @@ bool cfl_fused(const ns3d_ctx *c)
 // mixed: the stages' FP32 increments over the step's initial state); with
 // dev_dt the step's dt is cfl * the CFL reduction of its initial state,
 // taken on the device: by a CFL pass at the step's start, or, where the
+// one RK3 stage: the RHS pass finished by the stage update, then the ghost
+// layers of the new state (the mixed LDS-staged stage kernel writes the
+// periodic ghost images itself, vd_img; otherwise the fill pass runs)
+int stage_enqueue(ns3d_ctx *c, int stage, double dt, int dev_dt = 0)
+{
+    int st = rhs_pass(c, stage, dt, dev_dt);
+    if (st) return st;
+    if (c->mixed && c->vd_img && m_vdiv_lds_form(c->dp, c->m_vd_block[0], c->m_vd_block[1], c->vd_lds))
+        return NS3D_OK;
+    return fill_ghosts(c);
+}
+
 // third stage reduces it (cfl_fused), by the previous step of the same call
 // (first: the first step of a call, which runs the pass)
 int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0, bool first = true)
@@ -342,16 +354,9 @@ int rk3_enqueue(ns3d_ctx *c, double dt, int dev_dt = 0, bool first = true)
         int st = cfl_enqueue(c, dev_dt == 2);
         if (st) return st;
     }
-    // the mixed LDS-staged stage kernel writes the periodic ghost images
-    // itself (vd_img); otherwise each stage is followed by the fill pass
-    const bool images = c->mixed && c->vd_img && m_vdiv_lds_form(c->dp, c->m_vd_block[0], c->m_vd_block[1], c->vd_lds);
     for (int stage = 1; stage <= 3; ++stage) {
-        int st = rhs_pass(c, stage, dt, dev_dt);
+        int st = stage_enqueue(c, stage, dt, dev_dt);
         if (st) return st;
-        if (!images) {
-            st = fill_ghosts(c);
-            if (st) return st;
-        }
     }
     return step_end(c, dev_dt);
 }
@@ -489,7 +494,7 @@ int alloc_mixed(ns3d_ctx *c)
 
 int adj_check(ns3d_ctx *c)
 {
-    if (c->mixed || c->P.char_recon || !ad_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
+    if (c->P.char_recon || !ad_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
     if (c->dp.nx < 8 || c->dp.ny < 8 || c->dp.nz < 8) return NS3D_ERR_GRID_TOO_SMALL;
     return NS3D_OK;
 }
@@ -574,14 +579,72 @@ int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot, 
     return NS3D_OK;
 }
 
-// 6-array primitive state (W incl. T, all cells) of checkpoint slot q
-double *ck_state(ns3d_ctx *c, long long q) { return c->ck_slab + q * 6 * c->ncell; }
+// Checkpoint slots of the differentiation, all cells.  FP64 contexts: the
+// 6-array primitive state W (incl. T).  Mixed contexts: the FP64 conserved
+// state U (5 arrays) and the FP32 shadow (6 arrays, in 3 slots of doubles),
+// which restart the mixed step bit for bit; their stage snapshots hold the
+// FP64 primitives of the stage input (state_prims64) in the first 6 arrays.
+int ck_width(const ns3d_ctx *c) { return c->mixed ? 8 : 6; }
+double *ck_state(ns3d_ctx *c, long long q) { return c->ck_slab + q * ck_width(c) * c->ncell; }
 
 int copy_state(ns3d_ctx *c, double *dst, const double *src)
 {
     HIPCALL(c, hipMemcpyAsync(dst, src, (size_t)c->ncell * 6 * sizeof(double), hipMemcpyDeviceToDevice,
                               c->stream));
     return NS3D_OK;
+}
+
+// the context's state into slot dst (save) or from slot src (load)
+int save_state(ns3d_ctx *c, double *dst)
+{
+    if (!c->mixed) return copy_state(c, dst, c->f.W[0]);
+    const size_t b = (size_t)c->ncell * sizeof(double);
+    for (int m = 0; m < 5; ++m)
+        HIPCALL(c, hipMemcpyAsync(dst + (size_t)m * c->ncell, c->m.U[m], b, hipMemcpyDeviceToDevice, c->stream));
+    for (int m = 0; m < 6; ++m)
+        HIPCALL(c, hipMemcpyAsync(reinterpret_cast<float *>(dst + 5 * (size_t)c->ncell) + (size_t)m * c->ncell,
+                                  c->m.S[m], b / 2, hipMemcpyDeviceToDevice, c->stream));
+    return NS3D_OK;
+}
+
+int load_state(ns3d_ctx *c, const double *src)
+{
+    if (!c->mixed) return copy_state(c, c->f.W[0], src);
+    const size_t b = (size_t)c->ncell * sizeof(double);
+    for (int m = 0; m < 5; ++m)
+        HIPCALL(c, hipMemcpyAsync(c->m.U[m], src + (size_t)m * c->ncell, b, hipMemcpyDeviceToDevice, c->stream));
+    for (int m = 0; m < 6; ++m)
+        HIPCALL(c, hipMemcpyAsync(c->m.S[m],
+                                  reinterpret_cast<const float *>(src + 5 * (size_t)c->ncell) + (size_t)m * c->ncell,
+                                  b / 2, hipMemcpyDeviceToDevice, c->stream));
+    return NS3D_OK;
+}
+
+// The FP64 primitives the derivative kernels read (W with ghost layers):
+// the context's own W in FP64 mode; in mixed mode C2P of the FP64 state
+// U (+ the stage increment dU, add_du) into out (default: the FP64 scratch
+// arrays G[6..11], unused by the mixed step outside char_recon, which the
+// derivative code does not support), then the FP64 ghost fill.
+int fields64(ns3d_ctx *c, Fields &F, int add_du, double *out = nullptr)
+{
+    F = c->f;
+    if (!c->mixed) return NS3D_OK;
+    Prims64 w;
+    for (int m = 0; m < 6; ++m) w.p[m] = out ? out + (size_t)m * c->ncell : c->f.G[6 + m];
+    for (int m = 0; m < 6; ++m) F.W[m] = w.p[m];
+    TIMED(c, "state_prims64", launch_m_state_prims64(c->stream, c->dp, c->m, w, add_du));
+    TIMED(c, "fill_ghosts", launch_fill_ghosts(c->stream, c->dp, F));
+    return NS3D_OK;
+}
+
+// the FP64 primitives of a stage's input state into snapshot slot dst: the
+// context's W (FP64), or C2P of U (stage 1) / U + dU (stages 2, 3: dU holds
+// the previous stage's increment) in mixed mode
+int snapshot_stage(ns3d_ctx *c, double *dst, int stage)
+{
+    if (!c->mixed) return copy_state(c, dst, c->f.W[0]);
+    Fields F;
+    return fields64(c, F, stage > 1 ? 1 : 0, dst);
 }
 
 
@@ -1063,6 +1126,8 @@ int ns3d_set_precision(ns3d_ctx *c, int precision)
     const bool to_mixed = precision == NS3D_PREC_MIXED;
     if (to_mixed == c->mixed) return NS3D_OK;
     c->fw_valid = false;
+    // the checkpoint slots change width with the precision
+    if (c->ck_slab) { HIPCALL(c, hipFree(c->ck_slab)); c->ck_slab = nullptr; c->ck_states = 0; }
     if (to_mixed) {
         if (!c->P.char_recon && !m_xrow_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
         int st = alloc_mixed(c);
@@ -1367,8 +1432,10 @@ int ns3d_tlm_rhs(ns3d_ctx *c, const double *const dU[5], double *const out[5])
     int st = adj_check(c);
     if (!st) st = ensure_adj(c);
     if (!st) st = upload_set(c, c->adj.X, dU);
+    Fields F;
+    if (!st) st = fields64(c, F, 0);
     if (st) return st;
-    TIMED(c, "tlm_rhs", launch_tlm_rhs(c->stream, c->dp, c->f, c->adj));
+    TIMED(c, "tlm_rhs", launch_tlm_rhs(c->stream, c->dp, F, c->adj));
     st = download_set(c, out, c->adj.Z);
     if (st) return st;
     return stream_sync(c);
@@ -1381,7 +1448,9 @@ int ns3d_adj_rhs(ns3d_ctx *c, const double *const lam[5], double w_obj, double *
     if (!st) st = ensure_adj(c);
     if (!st) st = ensure_sums(c, 1);
     if (!st) st = upload_set(c, c->adj.X, lam);
-    if (!st) st = adj_rhs_enqueue(c, c->f, w_obj, 0, 0.0, 0.0, 1.0, 1, 0);
+    Fields F;
+    if (!st) st = fields64(c, F, 0);
+    if (!st) st = adj_rhs_enqueue(c, F, w_obj, 0, 0.0, 0.0, 1.0, 1, 0);
     if (st) return st;
     st = download_set(c, out, c->adj.Z);
     if (st) return st;
@@ -1396,10 +1465,11 @@ int ns3d_adj_rhs(ns3d_ctx *c, const double *const lam[5], double w_obj, double *
 int ns3d_dissipation(ns3d_ctx *c, double *eps)
 {
     if (!c || !eps) return NS3D_ERR_INVALID_ARG;
-    if (c->mixed) return NS3D_ERR_UNSUPPORTED;
     int st = ensure_adj(c);
+    Fields F;
+    if (!st) st = fields64(c, F, 0);
     if (st) return st;
-    TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, c->f, c->adj_part));
+    TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, F, c->adj_part));
     std::vector<double> h((size_t)ad_blocks(c->dp));
     HIPCALL(c, hipMemcpyAsync(h.data(), c->adj_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     st = stream_sync(c);
@@ -1421,7 +1491,7 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     const long long states = ncp + 3LL * ckpt_every + 1;
     if (states > c->ck_states) {
         if (c->ck_slab) { HIPCALL(c, hipFree(c->ck_slab)); c->ck_slab = nullptr; c->ck_states = 0; }
-        HIPCALL(c, hipMalloc((void **)&c->ck_slab, (size_t)states * 6 * c->ncell * sizeof(double)));
+        HIPCALL(c, hipMalloc((void **)&c->ck_slab, (size_t)states * ck_width(c) * c->ncell * sizeof(double)));
         c->ck_states = states;
     }
     st = ensure_sums(c, 4LL * nsteps);
@@ -1434,16 +1504,18 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     const int last0 = (int)((ncp - 1) * ckpt_every);
     for (int n = 0; n < nsteps; ++n) {
         if (n % ckpt_every == 0) {
-            st = copy_state(c, ck_state(c, n / ckpt_every), c->f.W[0]);
+            st = save_state(c, ck_state(c, n / ckpt_every));
             if (st) return st;
         }
-        TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, c->f, c->adj_part));
+        Fields F;
+        st = fields64(c, F, 0);
+        if (st) return st;
+        TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, F, c->adj_part));
         TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), jsum + n));
         if (n >= last0) {
             for (int stage = 1; stage <= 3; ++stage) {
-                st = copy_state(c, ck_state(c, snap0 + 3LL * (n - last0) + stage - 1), c->f.W[0]);
-                if (!st) st = rhs_pass(c, stage, dt);
-                if (!st) st = fill_ghosts(c);
+                st = snapshot_stage(c, ck_state(c, snap0 + 3LL * (n - last0) + stage - 1), stage);
+                if (!st) st = stage_enqueue(c, stage, dt);
                 if (st) return st;
             }
             st = step_end(c, 0);
@@ -1453,7 +1525,7 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
             if (st) return st;
         }
     }
-    st = copy_state(c, ck_state(c, c->ck_states - 1), c->f.W[0]);
+    st = save_state(c, ck_state(c, c->ck_states - 1));
     if (st) return st;
     st = check_flag(c);
     if (st) return st;
@@ -1489,16 +1561,15 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
         int n1r;
         // recompute the segment from its checkpoint, keeping every stage input
         // (the last segment's inputs are still there from the forward pass)
-        st = copy_state(c, c->f.W[0], ck_state(c, seg));
+        st = load_state(c, ck_state(c, seg));
         if (st) return st;
         if (seg == (N - 1) / K && c->fw_last_valid) n1r = n0;
         else n1r = n1;
         c->fw_last_valid = false;
         for (int n = n0; n < n1r; ++n)
             for (int stage = 1; stage <= 3; ++stage) {
-                st = copy_state(c, ck_state(c, snap0 + 3LL * (n - n0) + stage - 1), c->f.W[0]);
-                if (!st) st = rhs_pass(c, stage, dt);
-                if (!st) st = fill_ghosts(c);
+                st = snapshot_stage(c, ck_state(c, snap0 + 3LL * (n - n0) + stage - 1), stage);
+                if (!st) st = stage_enqueue(c, stage, dt);
                 if (st) return st;
             }
         for (int n = n1 - 1; n >= n0; --n) {
@@ -1516,7 +1587,7 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
         }
     }
     // leave the context at the final state of the forward pass
-    st = copy_state(c, c->f.W[0], ck_state(c, c->ck_states - 1));
+    st = load_state(c, ck_state(c, c->ck_states - 1));
     if (!st) st = download_set(c, lam0, c->adj.X);
     if (st) return st;
     std::vector<double> h(3 * (size_t)N);
@@ -1537,7 +1608,7 @@ int ns3d_adj_memory(const ns3d_ctx *c, long long *bytes)
     if (!c || !bytes) return NS3D_ERR_INVALID_ARG;
     long long b = 0;
     if (c->adj_slab) b += c->ncell * kAdjArrays * (long long)sizeof(double);
-    b += c->ck_states * 6 * c->ncell * (long long)sizeof(double);
+    b += c->ck_states * ck_width(c) * c->ncell * (long long)sizeof(double);
     b += c->adj_nsums * (long long)sizeof(double);
     *bytes = b;
     return NS3D_OK;

@@ -95,6 +95,7 @@ hipError_t launch_m_to_prims(hipStream_t s, const DevParams &P, const MFields &M
 hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields &M);
 hipError_t launch_m_cfl(hipStream_t s, const DevParams &P, const MFields &M, unsigned long long *out);
 hipError_t launch_m_shadow_to_prims64(hipStream_t s, const DevParams &P, const MFields &M, const Prims64 &out);
+hipError_t launch_m_state_prims64(hipStream_t s, const DevParams &P, const MFields &M, const Prims64 &out, int add_du);
 hipError_t launch_m_face_diff(hipStream_t s, const DevParams &P, const MFields &M, const FaceArrays &ff);
 bool m_xrow_supported(const DevParams &P);
 hipError_t launch_m_sweep(hipStream_t s, const DevParams &P, const MFields &M, int dir, int seg_len);

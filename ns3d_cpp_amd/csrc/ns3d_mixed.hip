@@ -1463,6 +1463,32 @@ __global__ void km_to_prims(DevParams P, MFields M, double *const *dst, int ncel
     dst[5][id] = p / (rr * P.Rgas);
 }
 
+// FP64 primitives (rho, u, v, w, p, T) of the FP64 state U + dU (add_du) or
+// U at the interior cells: C2P of the state whose FP32 rounding is the
+// shadow (the stage kernel's C2P).  The FP64 derivative kernels take them.
+__global__ void km_state_prims64(DevParams P, MFields M, Prims64 out, int add_du)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx) return;
+    const int id = (k * P.sy + j) * P.sx + i;
+    double Un[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) Un[m] = M.U[m][id] + (add_du ? (double)M.dU[m][id] : 0.0);
+    double rn = Un[0];
+    if (rn <= 0.0) rn = 1e-12;
+    const double un = Un[1] / rn, vn = Un[2] / rn, wn = Un[3] / rn;
+    const double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+    const double pn = (P.gamma - 1.0) * rn * (Un[4] / rn - kinetic);
+    out.p[0][id] = rn;
+    out.p[1][id] = un;
+    out.p[2][id] = vn;
+    out.p[3][id] = wn;
+    out.p[4][id] = pn;
+    out.p[5][id] = pn / (rn * P.Rgas);
+}
+
 // periodic ghost fill of the six FP32 shadow arrays (faces only)
 __global__ void km_fill_periodic(DevParams P, MFields M)
 {
@@ -1600,6 +1626,12 @@ hipError_t launch_m_fill_ghosts(hipStream_t s, const DevParams &P, const MFields
         const int n = P.ng * (axis == 0 ? P.sy : P.sx) * (axis == 2 ? P.sy : P.sz);
         hipLaunchKernelGGL(km_fill_bc, dim3((n + 255) / 256), dim3(256), 0, s, P, M, face, P.bc[face]);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_m_state_prims64(hipStream_t s, const DevParams &P, const MFields &M, const Prims64 &out, int add_du)
+{
+    hipLaunchKernelGGL(km_state_prims64, dim3((P.nx + 255) / 256, P.ny, P.nz), dim3(256), 0, s, P, M, out, add_du);
     return hipGetLastError();
 }
 

@@ -10,7 +10,11 @@ Checks that pin them independently of the code under test:
     evaluated by the GPU forward path and by the CPU oracle's RK3 loop, the
     latter with eps computed in numpy from the oracle's gradients;
   * d/d(1/Re) against finite differences in Re;
-  * checkpoint spacing changes nothing (the recomputation is bitwise).
+  * checkpoint spacing changes nothing (the recomputation is bitwise);
+  * in mixed-precision contexts (mixed steps forward, the FP64 scheme's
+    derivatives at each stage state's FP64 primitives): the RHS derivatives
+    against the FP64 context's, the gradient against the FP64 gradient and
+    FD of the FP64 J, and bitwise checkpoint-spacing invariance.
 The split uses sw_eps > 0 where finite differences are taken, so the flux is
 smooth; the dot-product test also covers the plain |lambda| splits.
 """
@@ -361,11 +365,113 @@ def test_gradient_against_oracle_finite_differences(gpu):
     assert abs(fd - ad) <= 1e-6 * abs(fd)
 
 
+# ---- mixed-precision contexts: the mixed steps forward and recomputed, the
+# FP64 scheme's derivatives at the FP64 primitives of each stage state
+
+MIXED_CASES = [DOT_CASES[0], DOT_CASES[1], DOT_CASES[5]]
+
+
+def _rel_close(s, a, b, tol):
+    for x, y in zip(a, b):
+        ref = np.max(np.abs(s.interior(x)))
+        assert np.max(np.abs(s.interior(x) - s.interior(y))) <= tol * ref
+
+
+@pytest.mark.parametrize("cfg", MIXED_CASES, ids=["weno5-sw", "weno5-lf", "ud7"])
+def test_mixed_rhs_derivatives_match_fp64(gpu, cfg):
+    """A mixed context holds U = P2C(W) in FP64; its TLM / adjoint RHS,
+    d/d(1/Re) and eps are the FP64 scheme's at C2P(U), so they equal the
+    FP64 context's to rounding, and the pair passes the dot-product test."""
+    n = 16
+    prims, Ma = state(n)
+    s64 = solver(n, prims, Ma, **cfg)
+    sm = solver(n, prims, Ma, precision="mixed", **cfg)
+    rng = np.random.default_rng(11)
+    x = [s64._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    y = [s64._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    Rx64, Rxm = s64.tlm_rhs(x), sm.tlm_rhs(x)
+    _rel_close(s64, Rx64, Rxm, 1e-11)
+    (g64, th64), (gm, thm) = s64.adj_rhs(y, 0.3), sm.adj_rhs(y, 0.3)
+    _rel_close(s64, g64, gm, 1e-11)
+    assert abs(th64 - thm) <= 1e-11 * abs(th64)
+    e64, em = s64.dissipation(), sm.dissipation()
+    assert abs(e64 - em) <= 1e-12 * abs(e64)
+    RTy, _ = sm.adj_rhs(y, 0.0)
+    lhs, rhs = interior_dot(sm, y, Rxm), interior_dot(sm, RTy, x)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_mixed_gradient_and_checkpoint_spacing(gpu):
+    """The mixed forward/backward: J within the mixed step's own error of
+    the FP64 J; the gradient against the FP64 adjoint and against central
+    differences of the FP64 J; any checkpoint spacing gives the same bits;
+    the context is left at the state a plain mixed advance reaches."""
+    n, nsteps, dt = 16, 7, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = state(n)
+    s64 = solver(n, prims, Ma, **cfg)
+    U = [s64._full(a) for a in conserved_of(prims)]
+    s64.upload_conserved(*U)
+    J64 = s64.adj_forward(nsteps, dt, 3)
+    lam64, th64 = s64.adj_backward()
+    sm = solver(n, prims, Ma, precision="mixed", **cfg)
+    res = []
+    for ck in (3, 1, 7):
+        sm.upload_conserved(*U)
+        J = sm.adj_forward(nsteps, dt, ck)
+        res.append((J,) + sm.adj_backward())
+    Jm, lamm, thm = res[0]
+    for J, lam, th in res[1:]:
+        assert J == Jm and th == thm
+        for a, b in zip(lam, lamm):
+            assert np.array_equal(sm.interior(a), sm.interior(b))
+    UN = sm.conserved()
+    s2 = solver(n, prims, Ma, precision="mixed", **cfg)
+    s2.upload_conserved(*U)
+    s2.advance(nsteps, dt)
+    for a, b in zip(UN, s2.conserved()):
+        assert np.array_equal(sm.interior(a), sm.interior(b))
+    print("J fp64", J64, "mixed", Jm, "theta", th64, thm)
+    assert abs(Jm - J64) <= 1e-7 * abs(J64)
+    assert abs(thm - th64) <= 1e-7 * abs(th64)
+    num = np.sqrt(sum(np.sum((s64.interior(a) - s64.interior(b)) ** 2) for a, b in zip(lamm, lam64)))
+    den = np.sqrt(sum(np.sum(s64.interior(a) ** 2) for a in lam64))
+    print("gradient rel L2 mixed vs fp64", num / den)
+    assert num <= 1e-6 * den
+    dU = [s64._full(a) for a in smooth_dir(n, 21, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    h = 1e-5
+    fd = (_J_gpu(n, prims, Ma, cfg, U, dU, +h, nsteps, dt) - _J_gpu(n, prims, Ma, cfg, U, dU, -h, nsteps, dt)) / (2 * h)
+    ad = interior_dot(sm, lamm, dU)
+    print("fd (fp64 J)", fd, "mixed adjoint", ad)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+def test_mixed_adjoint_memory_and_precision_switch(gpu):
+    """Mixed checkpoints hold 8 doubles per cell (U + the FP32 shadow);
+    switching precision drops the recorded pass."""
+    n, nsteps, dt = 16, 4, 0.01
+    prims, Ma = state(n)
+    sm = solver(n, prims, Ma, precision="mixed", **DOT_CASES[0])
+    base = sm.adj_memory()
+    sm.adj_forward(nsteps, dt, 2)
+    states = 2 + 3 * 2 + 1     # checkpoints, one segment's stage inputs, the final state
+    assert sm.adj_memory() - base >= states * 8 * int(np.prod(sm.shape)) * 8
+    sm.set_precision("fp64")
+    with pytest.raises(ns.NS3DError):
+        sm.adj_backward()
+    sm.adj_forward(nsteps, dt, 2)
+    sm.adj_backward()
+    sm.set_precision("mixed")
+    sm.adj_forward(nsteps, dt, 2)
+    sm.adj_backward()
+
+
 def test_unsupported_settings(gpu):
     n = 16
     prims, Ma = state(n)
-    for cfg in (dict(precision="mixed"), dict(char_recon=1),
-                dict(bc=["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])):
+    walls = ["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"]
+    for cfg in (dict(char_recon=1), dict(bc=walls), dict(precision="mixed", char_recon=1),
+                dict(precision="mixed", bc=walls)):
         s = solver(n, prims, Ma, **cfg)
         with pytest.raises(ns.NS3DError):
             s.tlm_rhs([np.zeros((n, n, n))] * 5)
