@@ -199,6 +199,48 @@ __device__ __forceinline__ void split_t(const DevParams &P, T rho, T u, T v, T w
     }
 }
 
+// computeRoeAveragedState_new and build_eigen_matrices_new
+// (InviscousFlux_new.cpp:532-659) for a generic scalar: the Roe state of the
+// primitive states ul, ur of a face with unit normal (nx, ny, nz), the left
+// (Lm) and right (Rm) eigenvector matrices of the characteristic
+// reconstruction, with the tangential pair of the face kernels
+template <class T>
+__device__ __forceinline__ void char_matrices_t(double gamma, const T ul[5], const T ur[5], double nx, double ny,
+                                                double nz, T Lm[5][5], T Rm[5][5])
+{
+    const T E_L = ul[0] * (0.5 * (ul[1] * ul[1] + ul[2] * ul[2] + ul[3] * ul[3]) + ul[4] / ((gamma - 1.0) * ul[0]));
+    const T E_R = ur[0] * (0.5 * (ur[1] * ur[1] + ur[2] * ur[2] + ur[3] * ur[3]) + ur[4] / ((gamma - 1.0) * ur[0]));
+    const T sl = sqrt_t(ul[0]), sr = sqrt_t(ur[0]);
+    const T den = 1.0 / (sl + sr);
+    const T u0 = (sl * ul[1] + sr * ur[1]) * den;
+    const T v0 = (sl * ul[2] + sr * ur[2]) * den;
+    const T w0 = (sl * ul[3] + sr * ur[3]) * den;
+    const T h0 = (sl * ((E_L + ul[4]) / ul[0]) + sr * ((E_R + ur[4]) / ur[0])) * den;
+    const T V = 0.5 * (u0 * u0 + v0 * v0 + w0 * w0);
+    const T un = u0 * nx + v0 * ny + w0 * nz;
+    const T c = sqrt_t((gamma - 1.0) * (h0 - V));
+    const T one = 1.0 + 0.0 * c, zero = 0.0 * c;
+    Rm[0][0] = one; Rm[1][0] = u0 - c * nx; Rm[2][0] = v0 - c * ny; Rm[3][0] = w0 - c * nz; Rm[4][0] = h0 - un * c;
+    Rm[0][1] = one; Rm[1][1] = u0; Rm[2][1] = v0; Rm[3][1] = w0; Rm[4][1] = V;
+    Rm[0][2] = zero; Rm[1][2] = zero + nz; Rm[2][2] = zero + nx; Rm[3][2] = zero + ny;
+    Rm[4][2] = u0 * nz + v0 * nx + w0 * ny;
+    Rm[0][3] = zero; Rm[1][3] = zero + ny; Rm[2][3] = zero + nz; Rm[3][3] = zero + nx;
+    Rm[4][3] = u0 * ny + v0 * nz + w0 * nx;
+    Rm[0][4] = one; Rm[1][4] = u0 + c * nx; Rm[2][4] = v0 + c * ny; Rm[3][4] = w0 + c * nz; Rm[4][4] = h0 + un * c;
+    const T chi = 0.5 / c;
+    const T K = (gamma - 1.0) / (c * c);
+    const T K1 = K * 0.5;
+    Lm[0][0] = K1 * V + chi * un; Lm[0][1] = -(chi * nx) - K1 * u0; Lm[0][2] = -(chi * ny) - K1 * v0;
+    Lm[0][3] = -(chi * nz) - K1 * w0; Lm[0][4] = K1;
+    Lm[1][0] = 1.0 - K * V; Lm[1][1] = K * u0; Lm[1][2] = K * v0; Lm[1][3] = K * w0; Lm[1][4] = -K;
+    Lm[2][0] = -(u0 * nz + v0 * nx + w0 * ny); Lm[2][1] = zero + nz; Lm[2][2] = zero + nx; Lm[2][3] = zero + ny;
+    Lm[2][4] = zero;
+    Lm[3][0] = -(u0 * ny + v0 * nz + w0 * nx); Lm[3][1] = zero + ny; Lm[3][2] = zero + nz; Lm[3][3] = zero + nx;
+    Lm[3][4] = zero;
+    Lm[4][0] = K1 * V - chi * un; Lm[4][1] = chi * nx - K1 * u0; Lm[4][2] = chi * ny - K1 * v0;
+    Lm[4][3] = chi * nz - K1 * w0; Lm[4][4] = K1;
+}
+
 template <class T>
 __device__ __forceinline__ T recon_weno5_t(const T *s)
 {

@@ -1,5 +1,6 @@
 // ns3d_adjoint.hip -- tangent-linear (TLM) and discrete adjoint of the FP64
-// RHS on a periodic box, and the dissipation objective.
+// RHS on a periodic box (component-wise or characteristic reconstruction),
+// and the dissipation objective.
 //
 // Notation: W = (rho, u, v, w, p) with T = p / (rho R); U = conserved.
 // R(U) = R_inv + R_vis as in range_kutta.cpp:11-99.  With periodic ghosts
@@ -106,8 +107,12 @@ __global__ void kt_c2p(DevParams P, Fields F, AdjArrays A)
 // ------------------------------------------------------------------ TLM
 
 // tangent of the face flux of the right face of every interior cell along
-// DIR, from W (ghost-filled) and dW (interior, wrapped): dF[m] arrays
-template <int DIR, int RECON, int FVS>
+// DIR, from W (ghost-filled) and dW (interior, wrapped): dF[m] arrays.
+// CHAR: the characteristic form (du_inviscous_character,
+// InviscousFlux_new.cpp:165-437): the split fluxes projected on the left
+// eigenvectors of the face's Roe state, reconstructed per characteristic
+// field and projected back, with the Roe state's tangent included
+template <int DIR, int RECON, int FVS, int CHAR>
 __global__ __launch_bounds__(256) void kt_faces(DevParams P, Fields F, AdjArrays A)
 {
     constexpr int S = Stencil<RECON>::S;
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void kt_faces(DevParams P, Fields F, AdjArrays
     const int c = DIR == 0 ? i : DIR == 1 ? j : k;
     const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
     using D1 = Dual<1>;
-    D1 fp[S][5], fm[S][5];
+    D1 fp[S][5], fm[S][5], wl[5], wr[5];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = awrap(c - MID + s, P.ng, n);
@@ -129,15 +134,46 @@ __global__ __launch_bounds__(256) void kt_faces(DevParams P, Fields F, AdjArrays
 #pragma unroll
         for (int m = 0; m < 5; ++m) { x[m].v = F.W[m][id]; x[m].d[0] = A.Y[m][id]; }
         split_t<FVS, DIR>(P, x[0], x[1], x[2], x[3], x[4], fp[s], fm[s]);
+        if (CHAR && s == MID)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) wl[m] = x[m];
+        if (CHAR && s == MID + 1)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) wr[m] = x[m];
     }
     const int id = (k * P.sy + j) * P.sx + i;
+    if constexpr (CHAR) {
+        D1 Lm[5][5], Rm[5][5], wc[5];
+        char_matrices_t(P.gamma, wl, wr, DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0, Lm, Rm);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        D1 sp[S], sm[S];
+        for (int e = 0; e < 5; ++e) {
+            D1 sp[S], sm[S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[S - 1 - s] = fm[s][m]; }
-        D1 r = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
-        A.T[m][id] = r.d[0];
+            for (int s = 0; s < S; ++s) {
+                D1 ap = dconst<1>(0.0), am = dconst<1>(0.0);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) { ap = ap + Lm[e][r] * fp[s][r]; am = am + Lm[e][r] * fm[s][r]; }
+                sp[s] = ap;
+                sm[S - 1 - s] = am;
+            }
+            wc[e] = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            D1 r = dconst<1>(0.0);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) r = r + Rm[m][e] * wc[e];
+            A.T[m][id] = r.d[0];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            D1 sp[S], sm[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[S - 1 - s] = fm[s][m]; }
+            D1 r = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
+            A.T[m][id] = r.d[0];
+        }
     }
 }
 
@@ -458,6 +494,181 @@ __global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, Adj
     }
 }
 
+// index of the cell (i,j,k) with its coordinate along DIR replaced by q
+template <int DIR>
+__device__ __forceinline__ int along(const DevParams &P, int i, int j, int k, int q)
+{
+    return DIR == 0 ? (k * P.sy + j) * P.sx + q : DIR == 1 ? (k * P.sy + q) * P.sx + i : (q * P.sy + j) * P.sx + i;
+}
+
+// Characteristic inviscid adjoint along DIR (du_inviscous_character,
+// InviscousFlux_new.cpp:165-437), per face: the right face of every
+// interior cell.  The face is recomputed (Roe state, eigenvector matrices,
+// the split fluxes projected on the left eigenvectors, the reconstruction of
+// each characteristic field), then reversed: lambda_F = (X(c+1) - X(c)) / h,
+// lambda_wc = Rm^T lambda_F, the reconstructions' reverse rules field by
+// field, and the projections' reverse.  Written per face: for every stencil
+// slot s the adjoints of F+ and F- at that slot's cell (arrays 5 s + m and
+// 5 (S + s) + m of B), and the adjoint through the Roe state of the two
+// cells' primitives (arrays 10 S + t for the left cell, 10 S + 5 + t for the
+// right), by forward mode over those ten inputs.  ka_char_gather sums them
+// per cell in a fixed order.
+template <int DIR, int RECON, int FVS>
+__global__ __launch_bounds__(256) void ka_char_face(DevParams P, Fields F, AdjArrays A, CharBuf B)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int c = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const double nv[3] = {DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0};
+    int ids[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) ids[s] = along<DIR>(P, i, j, k, awrap(c - MID + s, P.ng, n));
+    const int id = ids[MID];
+    double wl[5], wr[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { wl[m] = F.W[m][id]; wr[m] = F.W[m][ids[MID + 1]]; }
+    double Lm[5][5], Rm[5][5];
+    char_matrices_t(P.gamma, wl, wr, nv[0], nv[1], nv[2], Lm, Rm);
+    // the projected split fluxes: pp[e] in stencil order, pm[e] reversed
+    double pp[5][S], pm[5][S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double fp[5], fm[5];
+        split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp, fm);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            double ap = 0.0, am = 0.0;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) { ap += Lm[e][r] * fp[r]; am += Lm[e][r] * fm[r]; }
+            pp[e][s] = ap;
+            pm[e][S - 1 - s] = am;
+        }
+    }
+    double wc[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) wc[e] = recon_t<RECON>(P, pp[e]) + recon_t<RECON>(P, pm[e]);
+    const int idn = along<DIR>(P, i, j, k, awrap(c + 1, P.ng, n));
+    const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
+    double lf[5], lwc[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) lf[m] = (A.X[m][idn] - A.X[m][id]) * inv_d;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        double a = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a += Rm[m][e] * lf[m];
+        lwc[e] = a;
+    }
+    // reverse rules of the reconstructions, in place: pp / pm now hold the
+    // adjoints of the projected stencil values
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        double g[S];
+        recon_vjp<RECON>(P, pp[e], lwc[e], g);
+#pragma unroll
+        for (int s = 0; s < S; ++s) pp[e][s] = g[s];
+        recon_vjp<RECON>(P, pm[e], lwc[e], g);
+#pragma unroll
+        for (int s = 0; s < S; ++s) pm[e][s] = g[s];
+    }
+    // adjoint of Lm, and of F+ / F- at every slot
+    double lL[5][5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e)
+#pragma unroll
+        for (int r = 0; r < 5; ++r) lL[e][r] = 0.0;
+    const size_t N = (size_t)B.n;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double fp[5], fm[5];
+        split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp, fm);
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+#pragma unroll
+            for (int r = 0; r < 5; ++r) lL[e][r] += pp[e][s] * fp[r] + pm[e][S - 1 - s] * fm[r];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double bp = 0.0, bm = 0.0;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) { bp += Lm[e][m] * pp[e][s]; bm += Lm[e][m] * pm[e][S - 1 - s]; }
+            B.base[(size_t)(5 * s + m) * N + id] = bp;
+            B.base[(size_t)(5 * (S + s) + m) * N + id] = bm;
+        }
+    }
+    // the Roe state's eigenvector matrices, forward mode over the ten inputs
+    for (int q = 0; q < 10; ++q) {
+        Dual<1> dl[5], dr[5], dL[5][5], dR[5][5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            dl[m] = dconst<1>(wl[m]);
+            dr[m] = dconst<1>(wr[m]);
+            if (q == m) dl[m].d[0] = 1.0;
+            if (q == m + 5) dr[m].d[0] = 1.0;
+        }
+        char_matrices_t(P.gamma, dl, dr, nv[0], nv[1], nv[2], dL, dR);
+        double a = 0.0;
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+#pragma unroll
+            for (int r = 0; r < 5; ++r) a += lL[e][r] * dL[e][r].d[0] + lf[e] * wc[r] * dR[e][r].d[0];
+        B.base[(size_t)(10 * S + q) * N + id] = a;
+    }
+}
+
+// Per cell, after ka_char_face: the slot adjoints of the faces whose
+// stencils hold the cell (face f = q + MID - s at slot s, periodic), the
+// reverse rule of the split at the cell, and the Roe-state terms of its two
+// faces (left cell of face q, right cell of face q - 1); into Y (store: Y =
+// instead of Y +=).
+template <int DIR, int RECON, int FVS>
+__global__ __launch_bounds__(256) void ka_char_gather(DevParams P, Fields F, AdjArrays A, CharBuf B, int store)
+{
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int q = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const int id = (k * P.sy + j) * P.sx + i;
+    const size_t N = (size_t)B.n;
+    double lp[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, lm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int fid = along<DIR>(P, i, j, k, awrap(q + MID - s, P.ng, n));
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            lp[m] += B.base[(size_t)(5 * s + m) * N + fid];
+            lm[m] += B.base[(size_t)(5 * (S + s) + m) * N + fid];
+        }
+    }
+    const double w0 = F.W[0][id], w1 = F.W[1][id], w2 = F.W[2][id], w3 = F.W[3][id], w4 = F.W[4][id];
+    double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        double gv[5];
+        split_comp_vjp<FVS, DIR>(P, w0, w1, w2, w3, w4, m, lp[m], lm[m], gv);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) g[t] += gv[t];
+    }
+    const int fl = along<DIR>(P, i, j, k, awrap(q - 1, P.ng, n));
+#pragma unroll
+    for (int t = 0; t < 5; ++t) g[t] += B.base[(size_t)(10 * S + t) * N + id] + B.base[(size_t)(10 * S + 5 + t) * N + fl];
+    if (store) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) A.Y[t][id] = g[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) A.Y[t][id] += g[t];
+    }
+}
+
 // Viscous adjoint, pointwise part.  Recomputes the gradients and viscous
 // flux at the cell, forms lambda_VF = -D(L) (periodic), applies the reverse
 // rule of compute_viscous_flux (ViscousFlux.cpp:282-339) into the gradient
@@ -698,24 +909,31 @@ bool ad_supported(const DevParams &P)
            P.periodic[0] && P.periodic[1] && P.periodic[2];
 }
 
-template <int RECON, int FVS>
-static void tlm_faces_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir)
+template <int RECON, int FVS, int CHAR>
+static void tlm_faces_c(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir)
 {
-    if (dir == 0) hipLaunchKernelGGL((kt_faces<0, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A);
-    else if (dir == 1) hipLaunchKernelGGL((kt_faces<1, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A);
-    else hipLaunchKernelGGL((kt_faces<2, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A);
+    if (dir == 0) hipLaunchKernelGGL((kt_faces<0, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A);
+    else if (dir == 1) hipLaunchKernelGGL((kt_faces<1, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A);
+    else hipLaunchKernelGGL((kt_faces<2, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A);
+}
+
+template <int RECON, int FVS>
+static void tlm_faces_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int ch)
+{
+    if (ch) tlm_faces_c<RECON, FVS, 1>(s, P, F, A, dir);
+    else tlm_faces_c<RECON, FVS, 0>(s, P, F, A, dir);
 }
 
 template <int FVS>
-static void tlm_faces_f(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir)
+static void tlm_faces_f(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int ch)
 {
     switch (P.recon) {
-    case R_WENO5: tlm_faces_r<R_WENO5, FVS>(s, P, F, A, dir); break;
-    case R_LINEAR: tlm_faces_r<R_LINEAR, FVS>(s, P, F, A, dir); break;
-    case R_MDCD: tlm_faces_r<R_MDCD, FVS>(s, P, F, A, dir); break;
-    case R_MDCD_HYBRID: tlm_faces_r<R_MDCD_HYBRID, FVS>(s, P, F, A, dir); break;
-    case R_UD7: tlm_faces_r<R_UD7, FVS>(s, P, F, A, dir); break;
-    default: tlm_faces_r<R_UD7L, FVS>(s, P, F, A, dir); break;
+    case R_WENO5: tlm_faces_r<R_WENO5, FVS>(s, P, F, A, dir, ch); break;
+    case R_LINEAR: tlm_faces_r<R_LINEAR, FVS>(s, P, F, A, dir, ch); break;
+    case R_MDCD: tlm_faces_r<R_MDCD, FVS>(s, P, F, A, dir, ch); break;
+    case R_MDCD_HYBRID: tlm_faces_r<R_MDCD_HYBRID, FVS>(s, P, F, A, dir, ch); break;
+    case R_UD7: tlm_faces_r<R_UD7, FVS>(s, P, F, A, dir, ch); break;
+    default: tlm_faces_r<R_UD7L, FVS>(s, P, F, A, dir, ch); break;
     }
 }
 
@@ -727,14 +945,15 @@ static void tlm_visc(hipStream_t s, const DevParams &P, const Fields &F, const A
     hipLaunchKernelGGL(kt_vdiv<VIS>, agrid(P), kABlock, 0, s, P, A);
 }
 
-// Z <- R'(U) X for the state in F (TLM); uses Y, T, V as scratch
-hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A)
+// Z <- R'(U) X for the state in F (TLM); uses Y, T, V as scratch;
+// char_recon: the characteristic inviscid faces
+hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int char_recon)
 {
     if (!ad_supported(P)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kt_c2p, agrid(P), kABlock, 0, s, P, F, A);
     for (int d = 0; d < 3; ++d) {
-        if (P.fvs == FVS_SW) tlm_faces_f<FVS_SW>(s, P, F, A, d);
-        else tlm_faces_f<FVS_LF>(s, P, F, A, d);
+        if (P.fvs == FVS_SW) tlm_faces_f<FVS_SW>(s, P, F, A, d, char_recon);
+        else tlm_faces_f<FVS_LF>(s, P, F, A, d, char_recon);
         if (d == 0) hipLaunchKernelGGL(kt_diff<0>, agrid(P), kABlock, 0, s, P, A, 1);
         else if (d == 1) hipLaunchKernelGGL(kt_diff<1>, agrid(P), kABlock, 0, s, P, A, 0);
         else hipLaunchKernelGGL(kt_diff<2>, agrid(P), kABlock, 0, s, P, A, 0);
@@ -843,10 +1062,56 @@ static void adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const A
     hipLaunchKernelGGL(ka_grad<VIS>, agrid(P), kABlock, 0, s, P, F, A, cb);
 }
 
-// Y += W-adjoint of X . R_inv(U) along `dir` (Y = 0 before the first call)
-hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg)
+template <int RECON, int FVS>
+static void adj_char_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
+{
+    const int store = dir == 0 ? 1 : 0;
+    if (dir == 0) {
+        hipLaunchKernelGGL((ka_char_face<0, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_char_gather<0, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+    } else if (dir == 1) {
+        hipLaunchKernelGGL((ka_char_face<1, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_char_gather<1, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+    } else {
+        hipLaunchKernelGGL((ka_char_face<2, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_char_gather<2, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+    }
+}
+
+template <int FVS>
+static void adj_char_f(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
+{
+    switch (P.recon) {
+    case R_WENO5: adj_char_r<R_WENO5, FVS>(s, P, F, A, B, dir); break;
+    case R_LINEAR: adj_char_r<R_LINEAR, FVS>(s, P, F, A, B, dir); break;
+    case R_MDCD: adj_char_r<R_MDCD, FVS>(s, P, F, A, B, dir); break;
+    case R_MDCD_HYBRID: adj_char_r<R_MDCD_HYBRID, FVS>(s, P, F, A, B, dir); break;
+    case R_UD7: adj_char_r<R_UD7, FVS>(s, P, F, A, B, dir); break;
+    default: adj_char_r<R_UD7L, FVS>(s, P, F, A, B, dir); break;
+    }
+}
+
+// double arrays of the characteristic adjoint's per-face buffer: the F+ and
+// F- adjoints of every stencil slot and the Roe-state terms of two cells
+int char_buf_arrays(const DevParams &P)
+{
+    const int S = P.recon == R_UD7 || P.recon == R_UD7L ? 8 : P.recon == R_LINEAR ? 2 : 6;
+    return 10 * S + 10;
+}
+
+// Y += W-adjoint of X . R_inv(U) along `dir` (Y = 0 before the first call,
+// except with whole-row x sweeps or the characteristic form, whose x sweep
+// stores Y); cb: the characteristic form, through its per-face buffer
+hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg,
+                            const CharBuf *cb)
 {
     if (!ad_supported(P)) return hipErrorInvalidValue;
+    if (cb) {
+        if (!cb->base || cb->n != (long long)P.sx * P.sy * P.sz) return hipErrorInvalidValue;
+        if (P.fvs == FVS_SW) adj_char_f<FVS_SW>(s, P, F, A, *cb, dir);
+        else adj_char_f<FVS_LF>(s, P, F, A, *cb, dir);
+        return hipGetLastError();
+    }
     if (P.fvs == FVS_SW) adj_sweep_f<FVS_SW>(s, P, F, A, dir, seg);
     else adj_sweep_f<FVS_LF>(s, P, F, A, dir, seg);
     return hipGetLastError();

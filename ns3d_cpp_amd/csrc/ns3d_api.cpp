@@ -84,6 +84,8 @@ struct ns3d_ctx {
     // differentiation state, allocated on first use
     double *adj_slab = nullptr;     // the AdjArrays cell arrays
     AdjArrays adj{};
+    CharBuf adj_cb{};               // per-face buffer of the characteristic adjoint
+    long long adj_cb_bytes = 0;
     double *adj_part = nullptr;     // per-block partials of the adjoint kernels
     double *adj_sums = nullptr;     // one reduced sum per (step, stage)
     long long adj_nsums = 0;
@@ -494,7 +496,7 @@ int alloc_mixed(ns3d_ctx *c)
 
 int adj_check(ns3d_ctx *c)
 {
-    if (c->P.char_recon || !ad_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
+    if (!ad_supported(c->dp)) return NS3D_ERR_UNSUPPORTED;
     if (c->dp.nx < 8 || c->dp.ny < 8 || c->dp.nz < 8) return NS3D_ERR_GRID_TOO_SMALL;
     return NS3D_OK;
 }
@@ -516,6 +518,11 @@ int ensure_adj(ns3d_ctx *c)
     // the TLM's viscous-flux tangents and the adjoint's gradient adjoints are
     // never live at the same time
     for (int m = 0; m < 12; ++m) { A.V[m] = q; A.G[m] = q; q += c->ncell; }
+    if (c->P.char_recon) {
+        c->adj_cb_bytes = (long long)char_buf_arrays(c->dp) * c->ncell * (long long)sizeof(double);
+        HIPCALL(c, hipMalloc((void **)&c->adj_cb.base, (size_t)c->adj_cb_bytes));
+        c->adj_cb.n = c->ncell;
+    }
     return NS3D_OK;
 }
 
@@ -568,11 +575,13 @@ double compensated_sum(const std::vector<double> &h)
 int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot, double a, double bx, double bl,
                     int init, int add_acc)
 {
-    if (!adj_xrow(c->dp))
+    // the characteristic form and whole-row x sweeps store Y in the x sweep
+    const CharBuf *cb = c->P.char_recon ? &c->adj_cb : nullptr;
+    if (!cb && !adj_xrow(c->dp))
         HIPCALL(c, hipMemsetAsync(c->adj.Y[0], 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
     static const char *sname[3] = {"adj_sweep_x", "adj_sweep_y", "adj_sweep_z"};
     for (int d = 0; d < 3; ++d)
-        TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d)));
+        TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d), cb));
     TIMED(c, "adj_vflux", launch_adj_vflux(c->stream, c->dp, F, c->adj, w_obj, c->adj_part));
     TIMED(c, "adj_grad", launch_adj_grad(c->stream, c->dp, F, c->adj, a, bx, bl, init, add_acc));
     TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), c->adj_sums + slot));
@@ -900,6 +909,7 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_ptrs) (void)hipFree(c->d_ptrs);
     if (c->adj_slab) (void)hipFree(c->adj_slab);
+    if (c->adj_cb.base) (void)hipFree(c->adj_cb.base);
     if (c->adj_part) (void)hipFree(c->adj_part);
     if (c->adj_sums) (void)hipFree(c->adj_sums);
     if (c->ck_slab) (void)hipFree(c->ck_slab);
@@ -1435,7 +1445,7 @@ int ns3d_tlm_rhs(ns3d_ctx *c, const double *const dU[5], double *const out[5])
     Fields F;
     if (!st) st = fields64(c, F, 0);
     if (st) return st;
-    TIMED(c, "tlm_rhs", launch_tlm_rhs(c->stream, c->dp, F, c->adj));
+    TIMED(c, "tlm_rhs", launch_tlm_rhs(c->stream, c->dp, F, c->adj, c->P.char_recon));
     st = download_set(c, out, c->adj.Z);
     if (st) return st;
     return stream_sync(c);
@@ -1607,7 +1617,7 @@ int ns3d_adj_memory(const ns3d_ctx *c, long long *bytes)
 {
     if (!c || !bytes) return NS3D_ERR_INVALID_ARG;
     long long b = 0;
-    if (c->adj_slab) b += c->ncell * kAdjArrays * (long long)sizeof(double);
+    if (c->adj_slab) b += c->ncell * kAdjArrays * (long long)sizeof(double) + c->adj_cb_bytes;
     b += c->ck_states * ck_width(c) * c->ncell * (long long)sizeof(double);
     b += c->adj_nsums * (long long)sizeof(double);
     *bytes = b;

@@ -43,6 +43,13 @@ struct AdjArrays {
     double *G[12];
 };
 
+// Per-face buffer of the characteristic inviscid adjoint: char_buf_arrays(P)
+// double arrays of n cells each, array a at base + a * n.
+struct CharBuf {
+    double *base;
+    long long n;
+};
+
 __device__ __forceinline__ int wrapi_m(int q, int ng, int n)
 {
     return q < ng ? q + n : (q >= ng + n ? q - n : q);
@@ -113,9 +120,10 @@ hipError_t launch_ke(hipStream_t s, const DevParams &P, const Fields &F, double 
 bool ad_supported(const DevParams &P);
 int ad_blocks(const DevParams &P);
 int ad_seg(const DevParams &P, int dir);
-hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A);
+hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int char_recon);
 hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir,
-                            int seg);
+                            int seg, const CharBuf *cb);
+int char_buf_arrays(const DevParams &P);
 hipError_t launch_adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
                             double *partial);
 hipError_t launch_adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,

@@ -11,6 +11,7 @@ Checks that pin them independently of the code under test:
     latter with eps computed in numpy from the oracle's gradients;
   * d/d(1/Re) against finite differences in Re;
   * checkpoint spacing changes nothing (the recomputation is bitwise);
+  * the characteristic reconstruction (char_recon): the same checks;
   * in mixed-precision contexts (mixed steps forward, the FP64 scheme's
     derivatives at each stage state's FP64 primitives): the RHS derivatives
     against the FP64 context's, the gradient against the FP64 gradient and
@@ -466,12 +467,101 @@ def test_mixed_adjoint_memory_and_precision_switch(gpu):
     sm.adj_backward()
 
 
+# ---- characteristic reconstruction (char_recon): the TLM carries dual
+# numbers through the Roe state, its eigenvector matrices, the projections
+# and the reconstructions; the adjoint reverses the projections and the
+# reconstructions by hand and the Roe state's matrices by forward mode, so
+# the dot-product test still compares independent derivative code
+
+CHAR_CASES = [
+    dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", char_recon=1),
+    dict(recon="WENO5", fvs_type="LaxFriedrichs", vis_scheme="C6th", char_recon=1),
+    dict(recon="UPWIND_7TH", fvs_type="StagerWarming", vis_scheme="C4th", sw_eps=0.2, char_recon=1),
+    dict(recon="LINEAR", fvs_type="LaxFriedrichs", vis_scheme="C8th", mu_law="constant", char_recon=1),
+    dict(recon="MDCD_HYBRID", fvs_type="StagerWarming", vis_scheme="C8th", char_recon=1),
+]
+
+
+@pytest.mark.parametrize("cfg", CHAR_CASES, ids=["weno5-sw", "weno5-lf", "ud7-swe", "linear-lf", "mdcd-hybrid"])
+def test_char_dot_product_identity(gpu, cfg):
+    n = 16
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, **cfg)
+    rng = np.random.default_rng(23)
+    x = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    y = [s._full(rng.normal(size=(n, n, n))) for _ in range(5)]
+    Rx = s.tlm_rhs(x)
+    RTy, _ = s.adj_rhs(y, 0.0)
+    lhs, rhs = interior_dot(s, y, Rx), interior_dot(s, RTy, x)
+    scale = np.sqrt(interior_dot(s, y, y) * interior_dot(s, Rx, Rx))
+    print(cfg, lhs, rhs, scale)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_char_tlm_against_finite_differences(gpu, prec):
+    """The characteristic TLM against central differences of the FP64
+    characteristic RHS (whose faces the oracle pins); a mixed context gives
+    the FP64 scheme's TLM at C2P(U)."""
+    n = 16
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3, char_recon=1)
+    prims, Ma = state(n)
+    s = solver(n, prims, Ma, precision=prec, **cfg)
+    dU = [s._full(a) for a in smooth_dir(n, 5, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    tl = s.tlm_rhs(dU)
+    fd = _fd_rhs(n, prims, Ma, cfg, dU, 1e-5)
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(tl, fd)))
+    den = np.sqrt(sum(np.sum(s.interior(b) ** 2) for b in fd))
+    print(prec, "char TLM vs FD rel", num / den)
+    assert num / den < 1e-7
+
+
+def test_char_gradient_against_finite_differences(gpu):
+    """The gradient of J through the characteristic scheme: against central
+    differences of J (GPU forward) along a smooth direction and in 1/Re; the
+    mixed context's gradient against the FP64 one."""
+    n, nsteps, dt = 12, 4, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3, char_recon=1)
+    prims, Ma = state(n, seed=12)
+    s = solver(n, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    s.upload_conserved(*U)
+    J = s.adj_forward(nsteps, dt, 2)
+    lam0, dth = s.adj_backward()
+    dU = [s._full(a) for a in smooth_dir(n, 33, [0.05, 0.05, 0.05, 0.05, 1.0])]
+    h = 1e-5
+    fd = (_J_gpu(n, prims, Ma, cfg, U, dU, +h, nsteps, dt) - _J_gpu(n, prims, Ma, cfg, U, dU, -h, nsteps, dt)) / (2 * h)
+    ad = interior_dot(s, lam0, dU)
+    print("char J", J, "fd", fd, "adjoint", ad)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    theta = 1.0 / BASE["Re"]
+    ht = 1e-6 * theta
+    Jt = []
+    for sg in (+1, -1):
+        st_ = solver(n, prims, Ma, **dict(cfg, Re=1.0 / (theta + sg * ht)))
+        st_.upload_conserved(*U)
+        Jt.append(st_.adj_forward(nsteps, dt, nsteps))
+    fdt = (Jt[0] - Jt[1]) / (2 * ht)
+    print("char dJ/dtheta fd", fdt, "adjoint", dth)
+    assert abs(fdt - dth) <= 1e-6 * abs(fdt)
+    sm = solver(n, prims, Ma, precision="mixed", **cfg)
+    sm.upload_conserved(*U)
+    Jm = sm.adj_forward(nsteps, dt, 2)
+    lamm, thm = sm.adj_backward()
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(lamm, lam0)))
+    den = np.sqrt(sum(np.sum(s.interior(a) ** 2) for a in lam0))
+    print("char mixed: J", Jm, "gradient rel L2 vs fp64", num / den, "theta", thm)
+    assert abs(Jm - J) <= 1e-7 * abs(J)
+    assert num <= 1e-6 * den
+    assert abs(thm - dth) <= 1e-6 * abs(dth)
+
+
 def test_unsupported_settings(gpu):
     n = 16
     prims, Ma = state(n)
     walls = ["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"]
-    for cfg in (dict(char_recon=1), dict(bc=walls), dict(precision="mixed", char_recon=1),
-                dict(precision="mixed", bc=walls)):
+    for cfg in (dict(bc=walls), dict(precision="mixed", bc=walls), dict(char_recon=1, bc=walls)):
         s = solver(n, prims, Ma, **cfg)
         with pytest.raises(ns.NS3DError):
             s.tlm_rhs([np.zeros((n, n, n))] * 5)
