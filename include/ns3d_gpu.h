@@ -269,10 +269,10 @@ int ns3d_cfl_dt_noalloc(const ns3d_params *p, const ns3d_grid *g, const double *
                         const double *v, const double *w, const double *pr, double *dt_dev, void *stream);
 
 /* ---- Differentiation (no counterpart in the reference, which has no
- * derivative code).  A periodic box with a WENO5 / linear / MDCD /
- * MDCD_HYBRID / UD7 / UD7L reconstruction, component-wise or characteristic
- * (char_recon), and every grid extent >= 8; other settings return
- * NS3D_ERR_UNSUPPORTED.  Host arrays are full sx*sy*sz
+ * derivative code).  Every reconstruction a context accepts (WENO5 / linear /
+ * MDCD / MDCD_HYBRID / UD7 / UD7L, component-wise or characteristic), with
+ * periodic or wall / symmetry / outflow / inflow sides; every grid extent
+ * must be >= 8 (else NS3D_ERR_GRID_TOO_SMALL).  Host arrays are full sx*sy*sz
  * layouts; only interior cells are read or written.  |x| in the plain SW/LF
  * splits is differentiated as sign(x); sw_eps > 0 makes the split smooth.
  * Mixed-precision contexts: the forward and recomputed steps are the mixed

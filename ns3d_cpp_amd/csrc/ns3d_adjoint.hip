@@ -1,6 +1,6 @@
 // ns3d_adjoint.hip -- tangent-linear (TLM) and discrete adjoint of the FP64
-// RHS on a periodic box (component-wise or characteristic reconstruction),
-// and the dissipation objective.
+// RHS (component-wise or characteristic reconstruction; periodic or
+// wall / symmetry / outflow / inflow sides), and the dissipation objective.
 //
 // Notation: W = (rho, u, v, w, p) with T = p / (rho R); U = conserved.
 // R(U) = R_inv + R_vis as in range_kutta.cpp:11-99.  With periodic ghosts
@@ -8,7 +8,11 @@
 // of cell f, left face of cell f+1), so the adjoint of the flux difference is
 //     lambda_F(f) = (L(f+1) - L(f)) / h
 // and the central differences satisfy D^T = -D.  All adjoint arrays hold
-// interior cells; periodic images are reached by index wrap.
+// interior cells; periodic images are reached by index wrap.  A non-periodic
+// axis adds the first cell's left face (lambda = 0 outside the box), the
+// boundary fill's ghosts (the transpose of the fill maps their adjoints back
+// to the cells they copy) and the closures of choose_scheme, whose
+// transpose is applied row by row (dT_closure).
 #include "ns3d_ad.hpp"
 #include "ns3d_internal.hpp"
 
@@ -35,6 +39,52 @@ __device__ __forceinline__ double dwrap(const DevParams &P, const double *a, int
         win[o + R] = a[(kk * P.sy + jj) * P.sx + ii];
     }
     return diff_window<VIS>(win + R, h);
+}
+
+// derivative along `axis` at (i,j,k): the periodic central difference
+// (index wrap), or on a non-periodic axis the closures of choose_scheme
+// (ViscousFlux.cpp:43-118), which read interior values only
+template <int VIS>
+__device__ __forceinline__ double dgen(const DevParams &P, const double *a, int i, int j, int k, int axis, double h)
+{
+    if (P.periodic[axis]) return dwrap<VIS>(P, a, i, j, k, axis, h);
+    const int q = axis == 0 ? i : axis == 1 ? j : k;
+    const int n = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const long st = axis == 0 ? 1 : axis == 1 ? P.sx : (long)P.sx * P.sy;
+    return diff_central_bc<VIS>(a + (k * P.sy + j) * P.sx + i, st, h, q - P.ng, P.ng + n - 1 - q);
+}
+
+// (D^T a)(i,j,k) along a non-periodic `axis`: over the interior cells r
+// within reach, the weight of this cell in r's closure row times a(r).  The
+// weight is r's own row formula applied to a unit vector (exact: each row
+// is linear in its window).
+template <int VIS>
+__device__ __forceinline__ double dT_closure(const DevParams &P, const double *a, int i, int j, int k, int axis,
+                                             double h)
+{
+    constexpr int R = VisReach<VIS>::R;
+    const int n = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+    const int q = (axis == 0 ? i : axis == 1 ? j : k) - P.ng;
+    const long st = axis == 0 ? 1 : axis == 1 ? P.sx : (long)P.sx * P.sy;
+    const double *c = a + (k * P.sy + j) * P.sx + i;
+    double s = 0.0;
+#pragma unroll
+    for (int o = -R; o <= R; ++o) {
+        const int r = q - o;   // the row's cell; this cell sits at its offset o
+        if (r < 0 || r >= n) continue;
+        double e[2 * R + 1];
+#pragma unroll
+        for (int t = 0; t < 2 * R + 1; ++t) e[t] = t == R + o ? 1.0 : 0.0;
+        s += diff_window_bc<VIS>(e + R, h, r, n - 1 - r) * c[-o * st];
+    }
+    return s;
+}
+
+// index of the cell (i,j,k) with its coordinate along DIR replaced by q
+template <int DIR>
+__device__ __forceinline__ int along(const DevParams &P, int i, int j, int k, int q)
+{
+    return DIR == 0 ? (k * P.sy + j) * P.sx + q : DIR == 1 ? (k * P.sy + q) * P.sx + i : (q * P.sy + j) * P.sx + i;
 }
 
 __device__ __forceinline__ double dmu_dT(const DevParams &P, double T, double mu)
@@ -104,6 +154,27 @@ __global__ void kt_c2p(DevParams P, Fields F, AdjArrays A)
     A.Y[4][id] = (P.gamma - 1.0) * (d4 - (u * d1 + v * d2 + w * d3) + 0.5 * (u * u + v * v + w * w) * d0);
 }
 
+// linearised ghost fill (boundary.cpp:58-207) of the non-periodic side
+// `face` for the tangent dW in Y: wall and symmetry mirror it with the normal
+// velocity negated, outflow repeats the last interior cell, inflow writes a
+// constant state (tangent 0); the fill's density clamp has derivative 1 at
+// the positive densities of a valid state
+__global__ void kt_fill_bc(DevParams P, AdjArrays A, int face, int kind)
+{
+    const int axis = face >> 1;
+    const int na = axis == 0 ? P.sy : P.sx;
+    const int nb = axis == 2 ? P.sy : P.sz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.ng * na * nb) return;
+    int dst, src, normal;
+    bc_cell(P, face, kind, t, dst, src, normal);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double d = A.Y[m][src];
+        A.Y[m][dst] = kind == BCK_INFLOW ? 0.0 : (m == normal && kind != BCK_OUTFLOW ? -d : d);
+    }
+}
+
 // ------------------------------------------------------------------ TLM
 
 // tangent of the face flux of the right face of every interior cell along
@@ -123,56 +194,62 @@ __global__ __launch_bounds__(256) void kt_faces(DevParams P, Fields F, AdjArrays
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int c = DIR == 0 ? i : DIR == 1 ? j : k;
     const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const bool per = P.periodic[DIR];
     using D1 = Dual<1>;
-    D1 fp[S][5], fm[S][5], wl[5], wr[5];
+    // the cell's right face; on a non-periodic axis the first cell also
+    // forms its left face, from the ghost-filled W and dW
+    const int nf = !per && c == P.ng ? 2 : 1;
+    for (int t = 0; t < nf; ++t) {
+        const int fc = c - t;   // the face's left cell
+        D1 fp[S][5], fm[S][5], wl[5], wr[5];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int q = awrap(c - MID + s, P.ng, n);
-        const int id = DIR == 0 ? (k * P.sy + j) * P.sx + q : DIR == 1 ? (k * P.sy + q) * P.sx + i
-                                                                        : (q * P.sy + j) * P.sx + i;
-        D1 x[5];
+        for (int s = 0; s < S; ++s) {
+            const int id = along<DIR>(P, i, j, k, per ? awrap(fc - MID + s, P.ng, n) : fc - MID + s);
+            D1 x[5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) { x[m].v = F.W[m][id]; x[m].d[0] = A.Y[m][id]; }
-        split_t<FVS, DIR>(P, x[0], x[1], x[2], x[3], x[4], fp[s], fm[s]);
-        if (CHAR && s == MID)
+            for (int m = 0; m < 5; ++m) { x[m].v = F.W[m][id]; x[m].d[0] = A.Y[m][id]; }
+            split_t<FVS, DIR>(P, x[0], x[1], x[2], x[3], x[4], fp[s], fm[s]);
+            if (CHAR && s == MID)
 #pragma unroll
-            for (int m = 0; m < 5; ++m) wl[m] = x[m];
-        if (CHAR && s == MID + 1)
+                for (int m = 0; m < 5; ++m) wl[m] = x[m];
+            if (CHAR && s == MID + 1)
 #pragma unroll
-            for (int m = 0; m < 5; ++m) wr[m] = x[m];
-    }
-    const int id = (k * P.sy + j) * P.sx + i;
-    if constexpr (CHAR) {
-        D1 Lm[5][5], Rm[5][5], wc[5];
-        char_matrices_t(P.gamma, wl, wr, DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0, Lm, Rm);
+                for (int m = 0; m < 5; ++m) wr[m] = x[m];
+        }
+        const int id = along<DIR>(P, i, j, k, fc);
+        if constexpr (CHAR) {
+            D1 Lm[5][5], Rm[5][5], wc[5];
+            char_matrices_t(P.gamma, wl, wr, DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0, Lm,
+                            Rm);
 #pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            D1 sp[S], sm[S];
+            for (int e = 0; e < 5; ++e) {
+                D1 sp[S], sm[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                D1 ap = dconst<1>(0.0), am = dconst<1>(0.0);
+                for (int s = 0; s < S; ++s) {
+                    D1 ap = dconst<1>(0.0), am = dconst<1>(0.0);
 #pragma unroll
-                for (int r = 0; r < 5; ++r) { ap = ap + Lm[e][r] * fp[s][r]; am = am + Lm[e][r] * fm[s][r]; }
-                sp[s] = ap;
-                sm[S - 1 - s] = am;
+                    for (int r = 0; r < 5; ++r) { ap = ap + Lm[e][r] * fp[s][r]; am = am + Lm[e][r] * fm[s][r]; }
+                    sp[s] = ap;
+                    sm[S - 1 - s] = am;
+                }
+                wc[e] = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
             }
-            wc[e] = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
-        }
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            D1 r = dconst<1>(0.0);
+            for (int m = 0; m < 5; ++m) {
+                D1 r = dconst<1>(0.0);
 #pragma unroll
-            for (int e = 0; e < 5; ++e) r = r + Rm[m][e] * wc[e];
-            A.T[m][id] = r.d[0];
-        }
-    } else {
+                for (int e = 0; e < 5; ++e) r = r + Rm[m][e] * wc[e];
+                A.T[m][id] = r.d[0];
+            }
+        } else {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            D1 sp[S], sm[S];
+            for (int m = 0; m < 5; ++m) {
+                D1 sp[S], sm[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[S - 1 - s] = fm[s][m]; }
-            D1 r = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
-            A.T[m][id] = r.d[0];
+                for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[S - 1 - s] = fm[s][m]; }
+                D1 r = recon_t<RECON>(P, sp) + recon_t<RECON>(P, sm);
+                A.T[m][id] = r.d[0];
+            }
         }
     }
 }
@@ -186,10 +263,11 @@ __global__ void kt_diff(DevParams P, AdjArrays A, int first)
     const int k = P.ng + blockIdx.z;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int id = (k * P.sy + j) * P.sx + i;
-    int il = id;
-    if (DIR == 0) il = (k * P.sy + j) * P.sx + awrap(i - 1, P.ng, P.nx);
-    else if (DIR == 1) il = (k * P.sy + awrap(j - 1, P.ng, P.ny)) * P.sx + i;
-    else il = (awrap(k - 1, P.ng, P.nz) * P.sy + j) * P.sx + i;
+    // the left face: wrapped on a periodic axis, else the first cell's left
+    // face kept at its ghost neighbour
+    const int c = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const int il = along<DIR>(P, i, j, k, P.periodic[DIR] ? awrap(c - 1, P.ng, n) : c - 1);
     const double h = DIR == 0 ? P.dx : DIR == 1 ? P.dy : P.dz;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
@@ -217,8 +295,8 @@ __global__ __launch_bounds__(256) void kt_vflux(DevParams P, Fields F, AdjArrays
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            g[3 * q + d].v = dwrap<VIS>(P, vals[q], i, j, k, d, h[d]);
-            g[3 * q + d].d[0] = dwrap<VIS>(P, tans[q], i, j, k, d, h[d]);
+            g[3 * q + d].v = dgen<VIS>(P, vals[q], i, j, k, d, h[d]);
+            g[3 * q + d].d[0] = dgen<VIS>(P, tans[q], i, j, k, d, h[d]);
         }
     D1 T;
     T.v = F.W[5][id];
@@ -272,7 +350,7 @@ __global__ void kt_vdiv(DevParams P, AdjArrays A)
     for (int m = 0; m < 4; ++m) {
         double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s += dwrap<VIS>(P, A.V[4 * d + m], i, j, k, d, h[d]);
+        for (int d = 0; d < 3; ++d) s += dgen<VIS>(P, A.V[4 * d + m], i, j, k, d, h[d]);
         A.Z[m + 1][id] += s;
     }
 }
@@ -494,162 +572,198 @@ __global__ __launch_bounds__(1024) void ka_sweep_line(DevParams P, Fields F, Adj
     }
 }
 
-// index of the cell (i,j,k) with its coordinate along DIR replaced by q
-template <int DIR>
-__device__ __forceinline__ int along(const DevParams &P, int i, int j, int k, int q)
-{
-    return DIR == 0 ? (k * P.sy + j) * P.sx + q : DIR == 1 ? (k * P.sy + q) * P.sx + i : (q * P.sy + j) * P.sx + i;
-}
 
-// Characteristic inviscid adjoint along DIR (du_inviscous_character,
-// InviscousFlux_new.cpp:165-437), per face: the right face of every
-// interior cell.  The face is recomputed (Roe state, eigenvector matrices,
-// the split fluxes projected on the left eigenvectors, the reconstruction of
-// each characteristic field), then reversed: lambda_F = (X(c+1) - X(c)) / h,
-// lambda_wc = Rm^T lambda_F, the reconstructions' reverse rules field by
-// field, and the projections' reverse.  Written per face: for every stencil
-// slot s the adjoints of F+ and F- at that slot's cell (arrays 5 s + m and
-// 5 (S + s) + m of B), and the adjoint through the Roe state of the two
-// cells' primitives (arrays 10 S + t for the left cell, 10 S + 5 + t for the
-// right), by forward mode over those ten inputs.  ka_char_gather sums them
-// per cell in a fixed order.
-template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(256) void ka_char_face(DevParams P, Fields F, AdjArrays A, CharBuf B)
+// Inviscid adjoint along DIR by faces, for the characteristic form
+// (du_inviscous_character, InviscousFlux_new.cpp:165-437; CHAR = 1) and for
+// non-periodic axes (either form).  ka_face reverses one face: the right
+// face of every interior cell, and on a non-periodic axis also the first
+// cell's left face.  lambda_F(f) = (lambda(f+1) - lambda(f)) / h, where a
+// non-periodic axis has lambda = 0 outside the box.  The component-wise form
+// reverses each component's two reconstructions; the characteristic form
+// recomputes the face (Roe state, eigenvector matrices, the split fluxes
+// projected on the left eigenvectors, the reconstruction of each field) and
+// reverses lambda_wc = Rm^T lambda_F, each field's reconstructions and the
+// projections.  Written per face, at its left cell's index: for every
+// stencil slot s the adjoints of F+ and F- at that slot's cell (arrays 5 s + m
+// and 5 (S + s) + m of B); CHAR also writes the adjoint through the Roe state
+// of the two cells' primitives (arrays 10 S + t for the left cell, 10 S + 5 +
+// t for the right), by forward mode over those ten inputs.  ka_gather sums
+// them per cell in a fixed order.
+template <int DIR, int RECON, int FVS, int CHAR>
+__device__ __forceinline__ void face_adjoint(const DevParams &P, const Fields &F, const AdjArrays &A,
+                                             const CharBuf &B, int i, int j, int k, int fc)
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const bool per = P.periodic[DIR];
+    int ids[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) ids[s] = along<DIR>(P, i, j, k, per ? awrap(fc - MID + s, P.ng, n) : fc - MID + s);
+    const int id = ids[MID];
+    const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
+    // lambda of the face's two cells (0 outside a non-periodic box)
+    const bool in_l = per || fc >= P.ng, in_r = per || fc + 1 < P.ng + n;
+    const int idr = along<DIR>(P, i, j, k, per ? awrap(fc + 1, P.ng, n) : fc + 1);
+    double lf[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) lf[m] = ((in_r ? A.X[m][idr] : 0.0) - (in_l ? A.X[m][id] : 0.0)) * inv_d;
+    const size_t N = (size_t)B.n;
+    if constexpr (!CHAR) {
+        double fp[S][5], fm[S][5];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]],
+                              fp[s], fm[s]);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double sv[S], g[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) sv[s] = fp[s][m];
+            recon_vjp<RECON>(P, sv, lf[m], g);
+#pragma unroll
+            for (int s = 0; s < S; ++s) B.base[(size_t)(5 * s + m) * N + id] = g[s];
+#pragma unroll
+            for (int s = 0; s < S; ++s) sv[S - 1 - s] = fm[s][m];
+            recon_vjp<RECON>(P, sv, lf[m], g);
+#pragma unroll
+            for (int s = 0; s < S; ++s) B.base[(size_t)(5 * (S + s) + m) * N + id] = g[S - 1 - s];
+        }
+    } else {
+        const double nv[3] = {DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0};
+        double wl[5], wr[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { wl[m] = F.W[m][id]; wr[m] = F.W[m][ids[MID + 1]]; }
+        double Lm[5][5], Rm[5][5];
+        char_matrices_t(P.gamma, wl, wr, nv[0], nv[1], nv[2], Lm, Rm);
+        // the projected split fluxes: pp[e] in stencil order, pm[e] reversed
+        double pp[5][S], pm[5][S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            double fp[5], fm[5];
+            split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp,
+                              fm);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                double ap = 0.0, am = 0.0;
+#pragma unroll
+                for (int r = 0; r < 5; ++r) { ap += Lm[e][r] * fp[r]; am += Lm[e][r] * fm[r]; }
+                pp[e][s] = ap;
+                pm[e][S - 1 - s] = am;
+            }
+        }
+        double wc[5], lwc[5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) wc[e] = recon_t<RECON>(P, pp[e]) + recon_t<RECON>(P, pm[e]);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            double a = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) a += Rm[m][e] * lf[m];
+            lwc[e] = a;
+        }
+        // reverse rules of the reconstructions, in place: pp / pm now hold
+        // the adjoints of the projected stencil values
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            double g[S];
+            recon_vjp<RECON>(P, pp[e], lwc[e], g);
+#pragma unroll
+            for (int s = 0; s < S; ++s) pp[e][s] = g[s];
+            recon_vjp<RECON>(P, pm[e], lwc[e], g);
+#pragma unroll
+            for (int s = 0; s < S; ++s) pm[e][s] = g[s];
+        }
+        // adjoint of Lm, and of F+ / F- at every slot
+        double lL[5][5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+#pragma unroll
+            for (int r = 0; r < 5; ++r) lL[e][r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            double fp[5], fm[5];
+            split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp,
+                              fm);
+#pragma unroll
+            for (int e = 0; e < 5; ++e)
+#pragma unroll
+                for (int r = 0; r < 5; ++r) lL[e][r] += pp[e][s] * fp[r] + pm[e][S - 1 - s] * fm[r];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                double bp = 0.0, bm = 0.0;
+#pragma unroll
+                for (int e = 0; e < 5; ++e) { bp += Lm[e][m] * pp[e][s]; bm += Lm[e][m] * pm[e][S - 1 - s]; }
+                B.base[(size_t)(5 * s + m) * N + id] = bp;
+                B.base[(size_t)(5 * (S + s) + m) * N + id] = bm;
+            }
+        }
+        // the Roe state's eigenvector matrices, forward mode over the ten inputs
+        for (int q = 0; q < 10; ++q) {
+            Dual<1> dl[5], dr[5], dL[5][5], dR[5][5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                dl[m] = dconst<1>(wl[m]);
+                dr[m] = dconst<1>(wr[m]);
+                if (q == m) dl[m].d[0] = 1.0;
+                if (q == m + 5) dr[m].d[0] = 1.0;
+            }
+            char_matrices_t(P.gamma, dl, dr, nv[0], nv[1], nv[2], dL, dR);
+            double a = 0.0;
+#pragma unroll
+            for (int e = 0; e < 5; ++e)
+#pragma unroll
+                for (int r = 0; r < 5; ++r) a += lL[e][r] * dL[e][r].d[0] + lf[e] * wc[r] * dR[e][r].d[0];
+            B.base[(size_t)(10 * S + q) * N + id] = a;
+        }
+    }
+}
+
+template <int DIR, int RECON, int FVS, int CHAR>
+__global__ __launch_bounds__(256) void ka_face(DevParams P, Fields F, AdjArrays A, CharBuf B)
+{
     const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
     const int k = P.ng + blockIdx.z;
     if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
     const int c = DIR == 0 ? i : DIR == 1 ? j : k;
-    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
-    const double nv[3] = {DIR == 0 ? 1.0 : 0.0, DIR == 1 ? 1.0 : 0.0, DIR == 2 ? 1.0 : 0.0};
-    int ids[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) ids[s] = along<DIR>(P, i, j, k, awrap(c - MID + s, P.ng, n));
-    const int id = ids[MID];
-    double wl[5], wr[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) { wl[m] = F.W[m][id]; wr[m] = F.W[m][ids[MID + 1]]; }
-    double Lm[5][5], Rm[5][5];
-    char_matrices_t(P.gamma, wl, wr, nv[0], nv[1], nv[2], Lm, Rm);
-    // the projected split fluxes: pp[e] in stencil order, pm[e] reversed
-    double pp[5][S], pm[5][S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        double fp[5], fm[5];
-        split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp, fm);
-#pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            double ap = 0.0, am = 0.0;
-#pragma unroll
-            for (int r = 0; r < 5; ++r) { ap += Lm[e][r] * fp[r]; am += Lm[e][r] * fm[r]; }
-            pp[e][s] = ap;
-            pm[e][S - 1 - s] = am;
-        }
-    }
-    double wc[5];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) wc[e] = recon_t<RECON>(P, pp[e]) + recon_t<RECON>(P, pm[e]);
-    const int idn = along<DIR>(P, i, j, k, awrap(c + 1, P.ng, n));
-    const double inv_d = DIR == 0 ? P.idx : DIR == 1 ? P.idy : P.idz;
-    double lf[5], lwc[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) lf[m] = (A.X[m][idn] - A.X[m][id]) * inv_d;
-#pragma unroll
-    for (int e = 0; e < 5; ++e) {
-        double a = 0.0;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a += Rm[m][e] * lf[m];
-        lwc[e] = a;
-    }
-    // reverse rules of the reconstructions, in place: pp / pm now hold the
-    // adjoints of the projected stencil values
-#pragma unroll
-    for (int e = 0; e < 5; ++e) {
-        double g[S];
-        recon_vjp<RECON>(P, pp[e], lwc[e], g);
-#pragma unroll
-        for (int s = 0; s < S; ++s) pp[e][s] = g[s];
-        recon_vjp<RECON>(P, pm[e], lwc[e], g);
-#pragma unroll
-        for (int s = 0; s < S; ++s) pm[e][s] = g[s];
-    }
-    // adjoint of Lm, and of F+ / F- at every slot
-    double lL[5][5];
-#pragma unroll
-    for (int e = 0; e < 5; ++e)
-#pragma unroll
-        for (int r = 0; r < 5; ++r) lL[e][r] = 0.0;
-    const size_t N = (size_t)B.n;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        double fp[5], fm[5];
-        split_t<FVS, DIR>(P, F.W[0][ids[s]], F.W[1][ids[s]], F.W[2][ids[s]], F.W[3][ids[s]], F.W[4][ids[s]], fp, fm);
-#pragma unroll
-        for (int e = 0; e < 5; ++e)
-#pragma unroll
-            for (int r = 0; r < 5; ++r) lL[e][r] += pp[e][s] * fp[r] + pm[e][S - 1 - s] * fm[r];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            double bp = 0.0, bm = 0.0;
-#pragma unroll
-            for (int e = 0; e < 5; ++e) { bp += Lm[e][m] * pp[e][s]; bm += Lm[e][m] * pm[e][S - 1 - s]; }
-            B.base[(size_t)(5 * s + m) * N + id] = bp;
-            B.base[(size_t)(5 * (S + s) + m) * N + id] = bm;
-        }
-    }
-    // the Roe state's eigenvector matrices, forward mode over the ten inputs
-    for (int q = 0; q < 10; ++q) {
-        Dual<1> dl[5], dr[5], dL[5][5], dR[5][5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            dl[m] = dconst<1>(wl[m]);
-            dr[m] = dconst<1>(wr[m]);
-            if (q == m) dl[m].d[0] = 1.0;
-            if (q == m + 5) dr[m].d[0] = 1.0;
-        }
-        char_matrices_t(P.gamma, dl, dr, nv[0], nv[1], nv[2], dL, dR);
-        double a = 0.0;
-#pragma unroll
-        for (int e = 0; e < 5; ++e)
-#pragma unroll
-            for (int r = 0; r < 5; ++r) a += lL[e][r] * dL[e][r].d[0] + lf[e] * wc[r] * dR[e][r].d[0];
-        B.base[(size_t)(10 * S + q) * N + id] = a;
-    }
+    face_adjoint<DIR, RECON, FVS, CHAR>(P, F, A, B, i, j, k, c);
+    if (!P.periodic[DIR] && c == P.ng) face_adjoint<DIR, RECON, FVS, CHAR>(P, F, A, B, i, j, k, c - 1);
 }
 
-// Per cell, after ka_char_face: the slot adjoints of the faces whose
-// stencils hold the cell (face f = q + MID - s at slot s, periodic), the
-// reverse rule of the split at the cell, and the Roe-state terms of its two
-// faces (left cell of face q, right cell of face q - 1); into Y (store: Y =
-// instead of Y +=).
-template <int DIR, int RECON, int FVS>
-__global__ __launch_bounds__(256) void ka_char_gather(DevParams P, Fields F, AdjArrays A, CharBuf B, int store)
+// The adjoint of the primitives of cell x along DIR (an interior or, on a
+// non-periodic axis, a ghost cell) from the face buffer: its slots of the
+// faces whose stencils hold it (face f = x + MID - s at slot s), the reverse
+// rule of the split at the cell, and the Roe-state terms of its two faces
+// (left cell of face x, right cell of face x - 1).
+template <int DIR, int RECON, int FVS, int CHAR>
+__device__ __forceinline__ void cell_adjoint(const DevParams &P, const Fields &F, const CharBuf &B, int i, int j,
+                                             int k, int x, double g[5])
 {
     constexpr int S = Stencil<RECON>::S;
     constexpr int MID = (S - 1) / 2;
-    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = P.ng + blockIdx.z;
-    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
-    const int q = DIR == 0 ? i : DIR == 1 ? j : k;
     const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
-    const int id = (k * P.sy + j) * P.sx + i;
+    const bool per = P.periodic[DIR];
     const size_t N = (size_t)B.n;
+    // faces of the box: every wrapped face, or ng - 1 .. ng + n - 1
+    auto has_face = [&](int f) { return per || (f >= P.ng - 1 && f <= P.ng + n - 1); };
+    auto face_id = [&](int f) { return along<DIR>(P, i, j, k, per ? awrap(f, P.ng, n) : f); };
     double lp[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, lm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const int fid = along<DIR>(P, i, j, k, awrap(q + MID - s, P.ng, n));
+        const int f = x + MID - s;
+        if (!has_face(f)) continue;
+        const int fid = face_id(f);
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             lp[m] += B.base[(size_t)(5 * s + m) * N + fid];
             lm[m] += B.base[(size_t)(5 * (S + s) + m) * N + fid];
         }
     }
+    const int id = along<DIR>(P, i, j, k, x);
     const double w0 = F.W[0][id], w1 = F.W[1][id], w2 = F.W[2][id], w3 = F.W[3][id], w4 = F.W[4][id];
-    double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 5; ++t) g[t] = 0.0;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         double gv[5];
@@ -657,9 +771,60 @@ __global__ __launch_bounds__(256) void ka_char_gather(DevParams P, Fields F, Adj
 #pragma unroll
         for (int t = 0; t < 5; ++t) g[t] += gv[t];
     }
-    const int fl = along<DIR>(P, i, j, k, awrap(q - 1, P.ng, n));
+    if constexpr (CHAR) {
+        if (has_face(x)) {
+            const int fid = face_id(x);
 #pragma unroll
-    for (int t = 0; t < 5; ++t) g[t] += B.base[(size_t)(10 * S + t) * N + id] + B.base[(size_t)(10 * S + 5 + t) * N + fl];
+            for (int t = 0; t < 5; ++t) g[t] += B.base[(size_t)(10 * S + t) * N + fid];
+        }
+        if (has_face(x - 1)) {
+            const int fid = face_id(x - 1);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) g[t] += B.base[(size_t)(10 * S + 5 + t) * N + fid];
+        }
+    }
+}
+
+// Per interior cell, after ka_face: the cell's own adjoint and, on a
+// non-periodic axis, the adjoints of the ghost cells the boundary fill
+// (boundary.cpp:58-207) copies from it: wall and symmetry mirror one ghost
+// per cell of the first ng with the normal velocity negated, outflow
+// repeats the end cell into all ng ghosts, inflow ghosts are constant.  Into
+// Y (store: Y = instead of Y +=).
+template <int DIR, int RECON, int FVS, int CHAR>
+__global__ __launch_bounds__(256) void ka_gather(DevParams P, Fields F, AdjArrays A, CharBuf B, int store)
+{
+    const int i = P.ng + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = P.ng + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = P.ng + blockIdx.z;
+    if (i >= P.ng + P.nx || j >= P.ng + P.ny) return;
+    const int q = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int n = DIR == 0 ? P.nx : DIR == 1 ? P.ny : P.nz;
+    const int ng = P.ng;
+    double g[5];
+    cell_adjoint<DIR, RECON, FVS, CHAR>(P, F, B, i, j, k, q, g);
+    if (!P.periodic[DIR]) {
+        for (int side = 0; side < 2; ++side) {
+            const int kind = P.bc[2 * DIR + side];
+            int g0 = 0, g1 = 0;   // ghosts [g0, g1) that copy this cell
+            if (kind == BCK_WALL || kind == BCK_SYMMETRY) {
+                const int gq = side == 0 ? 2 * ng - 1 - q : 2 * (ng + n) - 1 - q;
+                const bool ok = side == 0 ? gq >= 0 && gq < ng : gq >= ng + n && gq < 2 * ng + n;
+                if (ok) { g0 = gq; g1 = gq + 1; }
+            } else if (kind == BCK_OUTFLOW) {
+                if (side == 0 && q == ng) { g0 = 0; g1 = ng; }
+                if (side == 1 && q == ng + n - 1) { g0 = ng + n; g1 = 2 * ng + n; }
+            }
+            for (int x = g0; x < g1; ++x) {
+                double gg[5];
+                cell_adjoint<DIR, RECON, FVS, CHAR>(P, F, B, i, j, k, x, gg);
+                if (kind != BCK_OUTFLOW) gg[1 + DIR] = -gg[1 + DIR];
+#pragma unroll
+                for (int t = 0; t < 5; ++t) g[t] += gg[t];
+            }
+        }
+    }
+    const int id = (k * P.sy + j) * P.sx + i;
     if (store) {
 #pragma unroll
         for (int t = 0; t < 5; ++t) A.Y[t][id] = g[t];
@@ -675,7 +840,9 @@ __global__ __launch_bounds__(256) void ka_char_gather(DevParams P, Fields F, Adj
 // adjoint LG and LW, adds w_obj * d(Phi)/d(.) of the dissipation objective,
 // and writes the block sum of (lambda_VF . VF + w_obj Phi) / mu_ref, the
 // derivative with respect to 1/Re of lambda . R_vis + w_obj * sum Phi.
-template <int VIS>
+// BC: on non-periodic axes the gradients take the closures of
+// choose_scheme and lambda_VF = D^T lambda the transposed closure rows.
+template <int VIS, int BC>
 __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays A, double w_obj, double *partial)
 {
     __shared__ double red[4];
@@ -693,9 +860,15 @@ __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double *c = src[q] + id;
-            g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
-            g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
-            g[3 * q + 2] = diff_central<VIS>(c, sxy, P.dz);
+            if constexpr (BC) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    g[3 * q + d] = dgen<VIS>(P, src[q], i, j, k, d, d == 0 ? P.dx : d == 1 ? P.dy : P.dz);
+            } else {
+                g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
+                g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
+                g[3 * q + 2] = diff_central<VIS>(c, sxy, P.dz);
+            }
         }
         const double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], T = F.W[5][id];
         const double mu = get_mu(P, T);
@@ -714,7 +887,9 @@ __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) lv[4 * d + m] = -diff_central<VIS>(A.X[m + 1] + id, strd[d], h[d]);
+            for (int m = 0; m < 4; ++m)
+                lv[4 * d + m] = BC && !P.periodic[d] ? dT_closure<VIS>(P, A.X[m + 1], i, j, k, d, h[d])
+                                                     : -diff_central<VIS>(A.X[m + 1] + id, strd[d], h[d]);
         const double Ex = lv[3], Ey = lv[7], Ez = lv[11];
         const double btxx = lv[0] + u * Ex;
         const double btxy = lv[1] + lv[4] + v * Ex + u * Ey;
@@ -784,9 +959,10 @@ struct Combine {
     int init, add_acc;
 };
 
-// Gradient adjoint LW += D^T LG (= -D LG on periodic arrays) for u, v, w, T,
-// which completes LW at the cell, then C2P'^T and the RK3 combination.
-template <int VIS>
+// Gradient adjoint LW += D^T LG (= -D LG on periodic arrays, the transposed
+// closure rows on non-periodic axes with BC) for u, v, w, T, which completes
+// LW at the cell, then C2P'^T and the RK3 combination.
+template <int VIS, int BC>
 __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays A, Combine cb)
 {
     int bx, by, bz;
@@ -803,7 +979,9 @@ __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays 
     for (int q = 0; q < 4; ++q) {
         double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s -= diff_central<VIS>(A.G[3 * q + d] + id, strd[d], h[d]);
+        for (int d = 0; d < 3; ++d)
+            s += BC && !P.periodic[d] ? dT_closure<VIS>(P, A.G[3 * q + d], i, j, k, d, h[d])
+                                      : -diff_central<VIS>(A.G[3 * q + d] + id, strd[d], h[d]);
         lq[q] = s;
     }
     const double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], T = F.W[5][id];
@@ -842,8 +1020,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int
     if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// volume-mean dissipation: block sums of Phi = tau_ij d_j u_i
-template <int VIS>
+// volume-mean dissipation: block sums of Phi = tau_ij d_j u_i (BC: the
+// closures on non-periodic axes, as the viscous flux takes them)
+template <int VIS, int BC>
 __global__ __launch_bounds__(256) void k_dissip(DevParams P, Fields F, double *partial)
 {
     __shared__ double red[4];
@@ -861,9 +1040,15 @@ __global__ __launch_bounds__(256) void k_dissip(DevParams P, Fields F, double *p
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double *c = src[q] + id;
-            g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
-            g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
-            g[3 * q + 2] = diff_central<VIS>(c, sxy, P.dz);
+            if constexpr (BC) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    g[3 * q + d] = dgen<VIS>(P, src[q], i, j, k, d, d == 0 ? P.dx : d == 1 ? P.dy : P.dz);
+            } else {
+                g[3 * q + 0] = diff_central<VIS>(c, 1, P.dx);
+                g[3 * q + 1] = diff_central<VIS>(c, P.sx, P.dy);
+                g[3 * q + 2] = diff_central<VIS>(c, sxy, P.dz);
+            }
         }
         const double mu = get_mu(P, F.W[5][id]);
         const double divU = g[0] + g[4] + g[8];
@@ -904,10 +1089,12 @@ int ad_blocks(const DevParams &P) { return ((P.nx + 63) / 64) * ((P.ny + 3) / 4)
 
 bool ad_supported(const DevParams &P)
 {
-    return (P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ||
-            P.recon == R_UD7 || P.recon == R_UD7L) &&
-           P.periodic[0] && P.periodic[1] && P.periodic[2];
+    return P.recon == R_WENO5 || P.recon == R_LINEAR || P.recon == R_MDCD || P.recon == R_MDCD_HYBRID ||
+           P.recon == R_UD7 || P.recon == R_UD7L;
 }
+
+// some axis is not periodic: the BC forms of the viscous kernels
+static bool any_bc(const DevParams &P) { return !(P.periodic[0] && P.periodic[1] && P.periodic[2]); }
 
 template <int RECON, int FVS, int CHAR>
 static void tlm_faces_c(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir)
@@ -937,6 +1124,17 @@ static void tlm_faces_f(hipStream_t s, const DevParams &P, const Fields &F, cons
     }
 }
 
+// the tangent's ghosts on the non-periodic sides, in apply_boundary's order
+static void tlm_fill_bc(hipStream_t s, const DevParams &P, const AdjArrays &A)
+{
+    for (int face = 0; face < 6; ++face) {
+        const int axis = face >> 1;
+        if (P.periodic[axis]) continue;
+        const int n = P.ng * (axis == 0 ? P.sy : P.sx) * (axis == 2 ? P.sy : P.sz);
+        hipLaunchKernelGGL(kt_fill_bc, dim3((n + 255) / 256), dim3(256), 0, s, P, A, face, P.bc[face]);
+    }
+}
+
 template <int VIS>
 static void tlm_visc(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A)
 {
@@ -951,6 +1149,7 @@ hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, co
 {
     if (!ad_supported(P)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kt_c2p, agrid(P), kABlock, 0, s, P, F, A);
+    tlm_fill_bc(s, P, A);
     for (int d = 0; d < 3; ++d) {
         if (P.fvs == FVS_SW) tlm_faces_f<FVS_SW>(s, P, F, A, d, char_recon);
         else tlm_faces_f<FVS_LF>(s, P, F, A, d, char_recon);
@@ -1049,7 +1248,8 @@ static void adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const 
 {
     ArraySet xs{{A.X[1], A.X[2], A.X[3], A.X[4]}, 4};
     fill_set(s, P, xs);
-    hipLaunchKernelGGL(ka_vflux<VIS>, agrid(P), kABlock, 0, s, P, F, A, w_obj, partial);
+    if (any_bc(P)) hipLaunchKernelGGL((ka_vflux<VIS, 1>), agrid(P), kABlock, 0, s, P, F, A, w_obj, partial);
+    else hipLaunchKernelGGL((ka_vflux<VIS, 0>), agrid(P), kABlock, 0, s, P, F, A, w_obj, partial);
 }
 
 template <int VIS>
@@ -1059,57 +1259,70 @@ static void adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const A
     for (int m = 0; m < 12; ++m) gs.p[m] = A.G[m];
     gs.n = 12;
     fill_set(s, P, gs);
-    hipLaunchKernelGGL(ka_grad<VIS>, agrid(P), kABlock, 0, s, P, F, A, cb);
+    if (any_bc(P)) hipLaunchKernelGGL((ka_grad<VIS, 1>), agrid(P), kABlock, 0, s, P, F, A, cb);
+    else hipLaunchKernelGGL((ka_grad<VIS, 0>), agrid(P), kABlock, 0, s, P, F, A, cb);
 }
 
-template <int RECON, int FVS>
-static void adj_char_r(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
+template <int RECON, int FVS, int CHAR>
+static void adj_face_c(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
 {
     const int store = dir == 0 ? 1 : 0;
     if (dir == 0) {
-        hipLaunchKernelGGL((ka_char_face<0, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
-        hipLaunchKernelGGL((ka_char_gather<0, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+        hipLaunchKernelGGL((ka_face<0, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_gather<0, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B, store);
     } else if (dir == 1) {
-        hipLaunchKernelGGL((ka_char_face<1, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
-        hipLaunchKernelGGL((ka_char_gather<1, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+        hipLaunchKernelGGL((ka_face<1, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_gather<1, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B, store);
     } else {
-        hipLaunchKernelGGL((ka_char_face<2, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B);
-        hipLaunchKernelGGL((ka_char_gather<2, RECON, FVS>), agrid(P), kABlock, 0, s, P, F, A, B, store);
+        hipLaunchKernelGGL((ka_face<2, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B);
+        hipLaunchKernelGGL((ka_gather<2, RECON, FVS, CHAR>), agrid(P), kABlock, 0, s, P, F, A, B, store);
     }
 }
 
-template <int FVS>
-static void adj_char_f(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
+template <int FVS, int CHAR>
+static void adj_face_f(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, const CharBuf &B, int dir)
 {
     switch (P.recon) {
-    case R_WENO5: adj_char_r<R_WENO5, FVS>(s, P, F, A, B, dir); break;
-    case R_LINEAR: adj_char_r<R_LINEAR, FVS>(s, P, F, A, B, dir); break;
-    case R_MDCD: adj_char_r<R_MDCD, FVS>(s, P, F, A, B, dir); break;
-    case R_MDCD_HYBRID: adj_char_r<R_MDCD_HYBRID, FVS>(s, P, F, A, B, dir); break;
-    case R_UD7: adj_char_r<R_UD7, FVS>(s, P, F, A, B, dir); break;
-    default: adj_char_r<R_UD7L, FVS>(s, P, F, A, B, dir); break;
+    case R_WENO5: adj_face_c<R_WENO5, FVS, CHAR>(s, P, F, A, B, dir); break;
+    case R_LINEAR: adj_face_c<R_LINEAR, FVS, CHAR>(s, P, F, A, B, dir); break;
+    case R_MDCD: adj_face_c<R_MDCD, FVS, CHAR>(s, P, F, A, B, dir); break;
+    case R_MDCD_HYBRID: adj_face_c<R_MDCD_HYBRID, FVS, CHAR>(s, P, F, A, B, dir); break;
+    case R_UD7: adj_face_c<R_UD7, FVS, CHAR>(s, P, F, A, B, dir); break;
+    default: adj_face_c<R_UD7L, FVS, CHAR>(s, P, F, A, B, dir); break;
     }
 }
 
-// double arrays of the characteristic adjoint's per-face buffer: the F+ and
-// F- adjoints of every stencil slot and the Roe-state terms of two cells
+// double arrays of the per-face buffer of the face-wise inviscid adjoint:
+// the F+ and F- adjoints of every stencil slot and the Roe-state terms of
+// two cells
 int char_buf_arrays(const DevParams &P)
 {
     const int S = P.recon == R_UD7 || P.recon == R_UD7L ? 8 : P.recon == R_LINEAR ? 2 : 6;
     return 10 * S + 10;
 }
 
-// Y += W-adjoint of X . R_inv(U) along `dir` (Y = 0 before the first call,
-// except with whole-row x sweeps or the characteristic form, whose x sweep
-// stores Y); cb: the characteristic form, through its per-face buffer
+// the inviscid adjoint along `dir` runs face-wise (per-face buffer): always
+// for the characteristic form, and on non-periodic axes
+bool adj_by_faces(const DevParams &P, int dir, int char_recon) { return char_recon || !P.periodic[dir]; }
+
+// the x adjoint sweep stores Y (else Y = 0 is needed before it)
+bool adj_x_stores(const DevParams &P, int char_recon) { return adj_by_faces(P, 0, char_recon) || adj_xrow(P); }
+
+// Y += W-adjoint of X . R_inv(U) along `dir` (Y = 0 before the first call
+// unless adj_x_stores); the face-wise form goes through cb
 hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir, int seg,
-                            const CharBuf *cb)
+                            const CharBuf *cb, int char_recon)
 {
     if (!ad_supported(P)) return hipErrorInvalidValue;
-    if (cb) {
-        if (!cb->base || cb->n != (long long)P.sx * P.sy * P.sz) return hipErrorInvalidValue;
-        if (P.fvs == FVS_SW) adj_char_f<FVS_SW>(s, P, F, A, *cb, dir);
-        else adj_char_f<FVS_LF>(s, P, F, A, *cb, dir);
+    if (adj_by_faces(P, dir, char_recon)) {
+        if (!cb || !cb->base || cb->n != (long long)P.sx * P.sy * P.sz) return hipErrorInvalidValue;
+        if (char_recon) {
+            if (P.fvs == FVS_SW) adj_face_f<FVS_SW, 1>(s, P, F, A, *cb, dir);
+            else adj_face_f<FVS_LF, 1>(s, P, F, A, *cb, dir);
+        } else {
+            if (P.fvs == FVS_SW) adj_face_f<FVS_SW, 0>(s, P, F, A, *cb, dir);
+            else adj_face_f<FVS_LF, 0>(s, P, F, A, *cb, dir);
+        }
         return hipGetLastError();
     }
     if (P.fvs == FVS_SW) adj_sweep_f<FVS_SW>(s, P, F, A, dir, seg);
@@ -1151,12 +1364,19 @@ hipError_t launch_sum_partials(hipStream_t s, const double *partial, int n, doub
     return hipGetLastError();
 }
 
+template <int VIS>
+static void dissip_v(hipStream_t s, const DevParams &P, const Fields &F, double *partial)
+{
+    if (any_bc(P)) hipLaunchKernelGGL((k_dissip<VIS, 1>), agrid(P), kABlock, 0, s, P, F, partial);
+    else hipLaunchKernelGGL((k_dissip<VIS, 0>), agrid(P), kABlock, 0, s, P, F, partial);
+}
+
 hipError_t launch_dissip(hipStream_t s, const DevParams &P, const Fields &F, double *partial)
 {
     switch (P.vis) {
-    case VIS_C8: hipLaunchKernelGGL(k_dissip<VIS_C8>, agrid(P), kABlock, 0, s, P, F, partial); break;
-    case VIS_C6: hipLaunchKernelGGL(k_dissip<VIS_C6>, agrid(P), kABlock, 0, s, P, F, partial); break;
-    default: hipLaunchKernelGGL(k_dissip<VIS_C4>, agrid(P), kABlock, 0, s, P, F, partial); break;
+    case VIS_C8: dissip_v<VIS_C8>(s, P, F, partial); break;
+    case VIS_C6: dissip_v<VIS_C6>(s, P, F, partial); break;
+    default: dissip_v<VIS_C4>(s, P, F, partial); break;
     }
     return hipGetLastError();
 }

@@ -84,7 +84,7 @@ struct ns3d_ctx {
     // differentiation state, allocated on first use
     double *adj_slab = nullptr;     // the AdjArrays cell arrays
     AdjArrays adj{};
-    CharBuf adj_cb{};               // per-face buffer of the characteristic adjoint
+    CharBuf adj_cb{};               // per-face buffer of the face-wise inviscid adjoint
     long long adj_cb_bytes = 0;
     double *adj_part = nullptr;     // per-block partials of the adjoint kernels
     double *adj_sums = nullptr;     // one reduced sum per (step, stage)
@@ -518,7 +518,8 @@ int ensure_adj(ns3d_ctx *c)
     // the TLM's viscous-flux tangents and the adjoint's gradient adjoints are
     // never live at the same time
     for (int m = 0; m < 12; ++m) { A.V[m] = q; A.G[m] = q; q += c->ncell; }
-    if (c->P.char_recon) {
+    if (adj_by_faces(c->dp, 0, c->P.char_recon) || adj_by_faces(c->dp, 1, c->P.char_recon) ||
+        adj_by_faces(c->dp, 2, c->P.char_recon)) {
         c->adj_cb_bytes = (long long)char_buf_arrays(c->dp) * c->ncell * (long long)sizeof(double);
         HIPCALL(c, hipMalloc((void **)&c->adj_cb.base, (size_t)c->adj_cb_bytes));
         c->adj_cb.n = c->ncell;
@@ -575,13 +576,12 @@ double compensated_sum(const std::vector<double> &h)
 int adj_rhs_enqueue(ns3d_ctx *c, const Fields &F, double w_obj, long long slot, double a, double bx, double bl,
                     int init, int add_acc)
 {
-    // the characteristic form and whole-row x sweeps store Y in the x sweep
-    const CharBuf *cb = c->P.char_recon ? &c->adj_cb : nullptr;
-    if (!cb && !adj_xrow(c->dp))
+    const CharBuf *cb = c->adj_cb.base ? &c->adj_cb : nullptr;
+    if (!adj_x_stores(c->dp, c->P.char_recon))
         HIPCALL(c, hipMemsetAsync(c->adj.Y[0], 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
     static const char *sname[3] = {"adj_sweep_x", "adj_sweep_y", "adj_sweep_z"};
     for (int d = 0; d < 3; ++d)
-        TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d), cb));
+        TIMED(c, sname[d], launch_adj_sweep(c->stream, c->dp, F, c->adj, d, ad_seg(c->dp, d), cb, c->P.char_recon));
     TIMED(c, "adj_vflux", launch_adj_vflux(c->stream, c->dp, F, c->adj, w_obj, c->adj_part));
     TIMED(c, "adj_grad", launch_adj_grad(c->stream, c->dp, F, c->adj, a, bx, bl, init, add_acc));
     TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), c->adj_sums + slot));

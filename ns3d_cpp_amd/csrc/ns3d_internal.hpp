@@ -43,7 +43,7 @@ struct AdjArrays {
     double *G[12];
 };
 
-// Per-face buffer of the characteristic inviscid adjoint: char_buf_arrays(P)
+// Per-face buffer of the face-wise inviscid adjoint: char_buf_arrays(P)
 // double arrays of n cells each, array a at base + a * n.
 struct CharBuf {
     double *base;
@@ -122,8 +122,10 @@ int ad_blocks(const DevParams &P);
 int ad_seg(const DevParams &P, int dir);
 hipError_t launch_tlm_rhs(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int char_recon);
 hipError_t launch_adj_sweep(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir,
-                            int seg, const CharBuf *cb);
+                            int seg, const CharBuf *cb, int char_recon);
 int char_buf_arrays(const DevParams &P);
+bool adj_by_faces(const DevParams &P, int dir, int char_recon);
+bool adj_x_stores(const DevParams &P, int char_recon);
 hipError_t launch_adj_vflux(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double w_obj,
                             double *partial);
 hipError_t launch_adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, double a,

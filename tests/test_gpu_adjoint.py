@@ -11,7 +11,8 @@ Checks that pin them independently of the code under test:
     latter with eps computed in numpy from the oracle's gradients;
   * d/d(1/Re) against finite differences in Re;
   * checkpoint spacing changes nothing (the recomputation is bitwise);
-  * the characteristic reconstruction (char_recon): the same checks;
+  * the characteristic reconstruction (char_recon) and non-periodic sides
+    (wall, symmetry, outflow, inflow): the same checks;
   * in mixed-precision contexts (mixed steps forward, the FP64 scheme's
     derivatives at each stage state's FP64 primitives): the RHS derivatives
     against the FP64 context's, the gradient against the FP64 gradient and
@@ -557,14 +558,170 @@ def test_char_gradient_against_finite_differences(gpu):
     assert abs(thm - dth) <= 1e-6 * abs(dth)
 
 
+# ---- non-periodic sides: the linearised ghost fill (and its transpose in
+# the adjoint), the first cell's left face, lambda = 0 outside the box, and
+# the transposed closure rows of the viscous derivatives
+
+BOX = (16, 12, 10)
+LAYOUTS = {
+    "channel": ["Periodic", "Periodic", "Wall", "Wall", "Outflow", "Outflow"],
+    "slab": ["Outflow", "Outflow", "Periodic", "Periodic", "Symmetry", "Symmetry"],
+    "box": ["Wall", "Wall", "Symmetry", "Symmetry", "Wall", "Outflow"],
+    "inflow": ["Inflow", "Outflow", "Periodic", "Periodic", "Wall", "Wall"],
+}
+
+
+def box_state(seed, Ma=0.3):
+    """Smooth state on BOX (z, y, x arrays): rho ~ 1, |u| ~ 0.3, p ~ 1/(gamma Ma^2)."""
+    nx, ny, nz = BOX
+    rng = np.random.default_rng(seed)
+    X, Y, Z = (2 * np.pi * np.arange(n) / n for n in (nx, ny, nz))
+    Z, Y, X = np.meshgrid(Z, Y, X, indexing="ij")
+    p0 = 1.0 / (1.4 * Ma * Ma)
+    out = []
+    for base, amp in ((1.0, 0.05), (0.1, 0.3), (0.0, 0.3), (0.0, 0.3), (p0, 0.05 * p0)):
+        f = np.zeros_like(X)
+        for _ in range(3):
+            k = rng.integers(-2, 3, 3)
+            f += np.sin(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 6.3))
+        out.append(base + amp * f / 3.0)
+    return out, Ma
+
+
+def bsolver(bcs, prims, Ma, **cfg):
+    kw = dict(BASE)
+    kw.update(cfg)
+    p, g = ns.make_params(BOX, Ma=Ma, bc=bcs, **kw)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*prims)
+    return s
+
+
+def box_dir(s, seed, scale):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = BOX
+    X, Y, Z = (2 * np.pi * np.arange(n) / n for n in (nx, ny, nz))
+    Z, Y, X = np.meshgrid(Z, Y, X, indexing="ij")
+    out = []
+    for m in range(5):
+        a = np.zeros_like(X)
+        for _ in range(3):
+            k = rng.integers(-2, 3, 3)
+            a += rng.normal() * np.cos(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 6.3))
+        out.append(s._full(scale[m] * a))
+    return out
+
+
+BC_SCHEMES = {
+    "weno5-sw-c8": dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th"),
+    "ud7-lf-c4": dict(recon="UPWIND_7TH", fvs_type="LaxFriedrichs", vis_scheme="C4th", mu_law="power",
+                      mu_power=0.76),
+    "char-weno5-c6": dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C6th", sw_eps=0.2, char_recon=1),
+}
+
+
+@pytest.mark.parametrize("scheme", list(BC_SCHEMES))
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_bc_dot_product_identity(gpu, layout, scheme):
+    prims, Ma = box_state(3)
+    s = bsolver(LAYOUTS[layout], prims, Ma, **BC_SCHEMES[scheme])
+    rng = np.random.default_rng(29)
+    x = [s._full(rng.normal(size=BOX[::-1])) for _ in range(5)]
+    y = [s._full(rng.normal(size=BOX[::-1])) for _ in range(5)]
+    Rx = s.tlm_rhs(x)
+    RTy, _ = s.adj_rhs(y, 0.0)
+    lhs, rhs = interior_dot(s, y, Rx), interior_dot(s, RTy, x)
+    scale = np.sqrt(interior_dot(s, y, y) * interior_dot(s, Rx, Rx))
+    print(layout, scheme, lhs, rhs, scale)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_bc_tlm_against_finite_differences(gpu, layout):
+    """The TLM against 4th-order central differences of the GPU RHS, whose
+    boundary handling the oracle pins.  A wall mirrors the normal velocity,
+    so a state that does not vanish there has a jump across the wall, where
+    WENO5's weights vary fast: 2nd-order differences converge as h^2 from
+    1.9e-5 (h = 1e-4) to 1.8e-7 (1e-5), 4th-order ones as h^4 to the 9e-10
+    rounding floor."""
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3)
+    prims, Ma = box_state(5)
+    s = bsolver(LAYOUTS[layout], prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    dU = box_dir(s, 7, [0.05, 0.05, 0.05, 0.05, 1.0])
+    tl = s.tlm_rhs(dU)
+    h = 1e-5
+    R = {}
+    for q in (-2, -1, 1, 2):
+        s.upload_conserved(*[a + q * h * d for a, d in zip(U, dU)])
+        R[q] = s.compute_rhs()
+    fd = [(8.0 * (a - b) - (c - d)) / (12.0 * h) for a, b, c, d in zip(R[1], R[-1], R[2], R[-2])]
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(tl, fd)))
+    den = np.sqrt(sum(np.sum(s.interior(b) ** 2) for b in fd))
+    print(layout, "TLM vs FD rel", num / den)
+    assert num / den < 1e-8
+
+
+@pytest.mark.parametrize("layout,char", [("box", 0), ("channel", 0), ("inflow", 1)])
+def test_bc_gradient_against_finite_differences(gpu, layout, char):
+    """The gradient of J with non-periodic sides against central differences
+    of J (GPU forward) along a smooth direction and in 1/Re, the same for
+    any checkpoint spacing; a mixed context's gradient against the FP64 one."""
+    nsteps, dt = 4, 0.01
+    cfg = dict(recon="WENO5", fvs_type="StagerWarming", vis_scheme="C8th", sw_eps=0.3, char_recon=char)
+    bcs = LAYOUTS[layout]
+    prims, Ma = box_state(11)
+    s = bsolver(bcs, prims, Ma, **cfg)
+    U = [s._full(a) for a in conserved_of(prims)]
+    res = []
+    for ck in (2, 1):
+        s.upload_conserved(*U)
+        J = s.adj_forward(nsteps, dt, ck)
+        res.append((J,) + s.adj_backward())
+    J, lam0, dth = res[0]
+    assert res[1][0] == J and res[1][2] == dth
+    for a, b in zip(res[1][1], lam0):
+        assert np.array_equal(s.interior(a), s.interior(b))
+    dU = box_dir(s, 33, [0.05, 0.05, 0.05, 0.05, 1.0])
+
+    def J_of(h, **over):
+        t = bsolver(bcs, prims, Ma, **dict(cfg, **over))
+        t.upload_conserved(*[a + h * d for a, d in zip(U, dU)])
+        return t.adj_forward(nsteps, dt, nsteps)
+
+    h = 1e-5
+    fd = (8.0 * (J_of(h) - J_of(-h)) - (J_of(2 * h) - J_of(-2 * h))) / (12.0 * h)
+    ad = interior_dot(s, lam0, dU)
+    print(layout, "J", J, "fd", fd, "adjoint", ad)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    theta = 1.0 / BASE["Re"]
+    ht = 1e-6 * theta
+    fdt = (J_of(0.0, Re=1.0 / (theta + ht)) - J_of(0.0, Re=1.0 / (theta - ht))) / (2 * ht)
+    print(layout, "dJ/dtheta fd", fdt, "adjoint", dth)
+    assert abs(fdt - dth) <= 1e-6 * abs(fdt)
+    sm = bsolver(bcs, prims, Ma, precision="mixed", **cfg)
+    sm.upload_conserved(*U)
+    Jm = sm.adj_forward(nsteps, dt, 2)
+    lamm, thm = sm.adj_backward()
+    num = np.sqrt(sum(np.sum((s.interior(a) - s.interior(b)) ** 2) for a, b in zip(lamm, lam0)))
+    den = np.sqrt(sum(np.sum(s.interior(a) ** 2) for a in lam0))
+    print(layout, "mixed: J", Jm, "gradient rel L2 vs fp64", num / den)
+    assert abs(Jm - J) <= 1e-6 * abs(J)
+    assert num <= 1e-5 * den
+
+
 def test_unsupported_settings(gpu):
+    """Every scheme a context accepts is differentiated (WCNS is refused at
+    creation); an extent below 8 cells is not."""
+    kw = dict(BASE)
+    p, g = ns.make_params((16, 16, 6), Ma=0.3, **kw)
+    s = ns.Solver(p, g)
+    s.upload_primitive(*[np.full((6, 16, 16), v) for v in (1.0, 0.1, 0.0, 0.0, 8.0)])
+    with pytest.raises(ns.NS3DError):
+        s.tlm_rhs([np.zeros((6, 16, 16))] * 5)
     n = 16
     prims, Ma = state(n)
-    walls = ["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"]
-    for cfg in (dict(bc=walls), dict(precision="mixed", bc=walls), dict(char_recon=1, bc=walls)):
-        s = solver(n, prims, Ma, **cfg)
-        with pytest.raises(ns.NS3DError):
-            s.tlm_rhs([np.zeros((n, n, n))] * 5)
     s = solver(n, prims, Ma)
     with pytest.raises(ns.NS3DError):
         s.adj_backward()   # no forward pass recorded
