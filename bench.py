@@ -201,8 +201,9 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
     H.hipEventSynchronize(ev[1])
     fwd = C.c_float()
     H.hipEventElapsedTime(C.byref(fwd), ev[0], ev[1])
+    # timed without per-kernel profiling: the reverse pass then recomputes
+    # the next checkpoint segment on a second stream while it reverses one
     s.upload_conserved(*U0)
-    s.profile(True)
     s.sync()
     H.hipEventRecord(ev[0], s.stream())
     J = s.adj_forward(nsteps, dt, ckpt)
@@ -213,9 +214,50 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
     t_f, t_b = C.c_float(), C.c_float()
     H.hipEventElapsedTime(C.byref(t_f), ev[0], ev[1])
     H.hipEventElapsedTime(C.byref(t_b), ev[1], ev[2])
+    mem = s.adj_memory()
+    # the per-kernel breakdown, from a profiled pass (one stream, so each
+    # kernel's time is its own); its gradient is the same bit for bit
+    s.upload_conserved(*U0)
+    s.profile(True)
+    s.sync()
+    H.hipEventRecord(ev[0], s.stream())
+    Jp = s.adj_forward(nsteps, dt, ckpt)
+    lamp, thp = s.adj_backward()
+    H.hipEventRecord(ev[2], s.stream())
+    H.hipEventSynchronize(ev[2])
+    t_seq = C.c_float()
+    H.hipEventElapsedTime(C.byref(t_seq), ev[0], ev[2])
     rep = s.profile_report()
     s.profile(False)
-    mem = s.adj_memory()
+    seq_same = Jp == J and thp == dJ_dinvRe and all(
+        np.array_equal(s.interior(a), s.interior(b)) for a, b in zip(lamp, lam0))
+    # the same pass on one stream without profiling (a context made with
+    # NS3D_ADJ_OVERLAP=0): what the overlap itself gains
+    prev = os.environ.get("NS3D_ADJ_OVERLAP")
+    os.environ["NS3D_ADJ_OVERLAP"] = "0"
+    try:
+        s1 = ns.Solver(p, g)
+    finally:
+        if prev is None:
+            del os.environ["NS3D_ADJ_OVERLAP"]
+        else:
+            os.environ["NS3D_ADJ_OVERLAP"] = prev
+    s1.upload_conserved(*U0)
+    s1.adj_forward(2, dt, 1)
+    s1.adj_backward()
+    s1.upload_conserved(*U0)
+    s1.sync()
+    H.hipEventRecord(ev[0], s1.stream())
+    J1s = s1.adj_forward(nsteps, dt, ckpt)
+    lam1s, th1s = s1.adj_backward()
+    H.hipEventRecord(ev[2], s1.stream())
+    H.hipEventSynchronize(ev[2])
+    t_one = C.c_float()
+    H.hipEventElapsedTime(C.byref(t_one), ev[0], ev[2])
+    one_same = J1s == J and th1s == dJ_dinvRe and all(
+        np.array_equal(s1.interior(a), s1.interior(b)) for a, b in zip(lam1s, lam0))
+    one_mem = s1.adj_memory()
+    s1.close()
     base = ns.lib().ns3d_cells(s.ctx) * 40 * 8
     gnorm = float(np.sqrt(sum(np.sum(s.interior(a) ** 2) for a in lam0)))
     # the same gradient with every stage state of the run kept in HBM (one
@@ -280,6 +322,13 @@ def adjoint_cost(ns, H, n, nsteps, ckpt, seed):
             "forward_ms": round(fwd.value, 3), "adj_forward_ms": round(t_f.value, 3),
             "adj_backward_ms": round(t_b.value, 3), "adjoint_total_ms": round(total, 3),
             "cost_vs_forward": round(total / fwd.value, 3),
+            "one_stream": {"adjoint_total_ms": round(t_one.value, 3),
+                           "cost_vs_forward": round(t_one.value / fwd.value, 3),
+                           "device_GB": round((one_mem + base) / 1e9, 3),
+                           "gradient_bitwise_equal": bool(one_same)},
+            "one_stream_profiled": {"adjoint_total_ms": round(t_seq.value, 3),
+                                    "cost_vs_forward": round(t_seq.value / fwd.value, 3),
+                                    "gradient_bitwise_equal": bool(seq_same)},
             "device_bytes": int(mem + base), "device_GB": round((mem + base) / 1e9, 3),
             "store_all_stages": store_all, "mixed": mixed,
             "kernels_ms": {k: round(v[0], 2) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}}

@@ -91,6 +91,12 @@ struct ns3d_ctx {
     long long adj_nsums = 0;
     double *ck_slab = nullptr;      // checkpoints, stage snapshots, final state
     long long ck_states = 0;        // 6-array states held in ck_slab
+    int ck_bufs = 1;                // stage-snapshot buffers of one segment each
+    // the reverse pass recomputes the next segment on a second stream while
+    // it reverses the current one (NS3D_ADJ_OVERLAP, default on)
+    int adj_overlap = 1;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_start = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     int fw_steps = 0, fw_ckpt = 0;
     double fw_dt = 0.0;
     bool fw_valid = false;          // a forward pass matches the current state
@@ -527,6 +533,19 @@ int ensure_adj(ns3d_ctx *c)
     return NS3D_OK;
 }
 
+// the reverse pass's second stream (default priority) and its events
+int ensure_stream2(ns3d_ctx *c)
+{
+    if (c->stream2) return NS3D_OK;
+    HIPCALL(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCALL(c, hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
+    for (int b = 0; b < 2; ++b) {
+        HIPCALL(c, hipEventCreateWithFlags(&c->ev_ready[b], hipEventDisableTiming));
+        HIPCALL(c, hipEventCreateWithFlags(&c->ev_free[b], hipEventDisableTiming));
+    }
+    return NS3D_OK;
+}
+
 int ensure_sums(ns3d_ctx *c, long long n)
 {
     if (c->adj_nsums >= n) return NS3D_OK;
@@ -877,6 +896,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
         if (v > 0) for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = v;
     }
     if (const char *env = std::getenv("NS3D_GRAPHS")) c->use_graphs = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_ADJ_OVERLAP")) c->adj_overlap = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_VD_LDS")) c->vd_lds = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_VD_IMG")) c->vd_img = std::atoi(env) != 0;
@@ -922,6 +942,9 @@ int ns3d_destroy(ns3d_ctx *c)
     for (auto &g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    for (hipEvent_t e : {c->ev_start, c->ev_ready[0], c->ev_ready[1], c->ev_free[0], c->ev_free[1]})
+        if (e) (void)hipEventDestroy(e);
     delete c;
     return NS3D_OK;
 }
@@ -1497,8 +1520,10 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     c->fw_valid = false;
     if (ckpt_every > nsteps) ckpt_every = nsteps;
     const long long ncp = (nsteps + ckpt_every - 1) / ckpt_every;
-    // checkpoints, 3 stage inputs per step of one segment, the final state
-    const long long states = ncp + 3LL * ckpt_every + 1;
+    // checkpoints, 3 stage inputs per step of one segment (of two segments
+    // when the reverse pass overlaps recompute and reversal), the final state
+    const int nbuf = c->adj_overlap && ncp > 1 ? 2 : 1;
+    const long long states = ncp + 3LL * ckpt_every * nbuf + 1;
     if (states > c->ck_states) {
         if (c->ck_slab) { HIPCALL(c, hipFree(c->ck_slab)); c->ck_slab = nullptr; c->ck_states = 0; }
         HIPCALL(c, hipMalloc((void **)&c->ck_slab, (size_t)states * ck_width(c) * c->ncell * sizeof(double)));
@@ -1546,6 +1571,7 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     *J = dt * compensated_sum(h) / ((double)c->dp.nx * c->dp.ny * c->dp.nz);
     c->fw_steps = nsteps;
     c->fw_ckpt = ckpt_every;
+    c->ck_bufs = nbuf;
     c->fw_dt = dt;
     c->fw_valid = true;
     c->fw_last_valid = true;
@@ -1566,35 +1592,82 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
     st = upload_set(c, c->adj.X, lamN);
     if (st) return st;
     const long long snap0 = (N + K - 1) / K;   // first stage-snapshot slot
-    for (int seg = (N - 1) / K; seg >= 0; --seg) {
+    const int last = (N - 1) / K;
+    // stage input `stage` of step n of a segment starting at n0, buffer b
+    auto snap = [&](int b, int n, int n0, int stage) {
+        return ck_state(c, snap0 + 3LL * K * b + 3LL * (n - n0) + stage - 1);
+    };
+    // the segment recomputed from its checkpoint on c->stream, keeping every
+    // stage input in buffer b
+    auto recompute = [&](int seg, int b) -> int {
         const int n0 = seg * K, n1 = n0 + K < N ? n0 + K : N;
-        int n1r;
-        // recompute the segment from its checkpoint, keeping every stage input
-        // (the last segment's inputs are still there from the forward pass)
-        st = load_state(c, ck_state(c, seg));
-        if (st) return st;
-        if (seg == (N - 1) / K && c->fw_last_valid) n1r = n0;
-        else n1r = n1;
-        c->fw_last_valid = false;
-        for (int n = n0; n < n1r; ++n)
-            for (int stage = 1; stage <= 3; ++stage) {
-                st = snapshot_stage(c, ck_state(c, snap0 + 3LL * (n - n0) + stage - 1), stage);
-                if (!st) st = stage_enqueue(c, stage, dt);
-                if (st) return st;
+        int r = load_state(c, ck_state(c, seg));
+        for (int n = n0; n < n1 && !r; ++n)
+            for (int stage = 1; stage <= 3 && !r; ++stage) {
+                r = snapshot_stage(c, snap(b, n, n0, stage), stage);
+                if (!r) r = stage_enqueue(c, stage, dt);
             }
+        return r;
+    };
+    // the segment's steps reversed from the stage inputs in buffer b
+    auto reverse = [&](int seg, int b) -> int {
+        const int n0 = seg * K, n1 = n0 + K < N ? n0 + K : N;
         for (int n = n1 - 1; n >= n0; --n) {
             for (int stage = 3; stage >= 1; --stage) {
                 Fields F = c->f;
-                double *W = ck_state(c, snap0 + 3LL * (n - n0) + stage - 1);
+                double *W = snap(b, n, n0, stage);
                 for (int m = 0; m < 6; ++m) F.W[m] = W + m * c->ncell;
                 const long long slot = 3LL * n + stage - 1;
-                if (stage == 3) st = adj_rhs_enqueue(c, F, 0.0, slot, a3, b3, b3 * dt, 1, 0);
-                else if (stage == 2) st = adj_rhs_enqueue(c, F, 0.0, slot, a2, b2, b2 * dt, 0, 0);
-                else st = adj_rhs_enqueue(c, F, w_obj, slot, 0.0, 1.0, dt, 0, 1);
-                if (st) return st;
+                int r;
+                if (stage == 3) r = adj_rhs_enqueue(c, F, 0.0, slot, a3, b3, b3 * dt, 1, 0);
+                else if (stage == 2) r = adj_rhs_enqueue(c, F, 0.0, slot, a2, b2, b2 * dt, 0, 0);
+                else r = adj_rhs_enqueue(c, F, w_obj, slot, 0.0, 1.0, dt, 0, 1);
+                if (r) return r;
                 for (int m = 0; m < 5; ++m) std::swap(c->adj.X[m], c->adj.Z[m]);
             }
         }
+        return NS3D_OK;
+    };
+    // the last segment's inputs are still there from the forward pass
+    if (!c->fw_last_valid) {
+        st = recompute(last, 0);
+        if (st) return st;
+    }
+    c->fw_last_valid = false;
+    // Overlap (two snapshot buffers, not while per-kernel profiling): the
+    // recompute of segment seg - 1 runs on the second stream, into the
+    // buffer the reverse of seg + 1 has released, while the main stream
+    // reverses seg.  The recompute touches only the forward arrays, the
+    // reversal only the snapshots it reads and the adjoint arrays.
+    const bool overlap = c->ck_bufs == 2 && !c->prof && last > 0;
+    if (overlap) {
+        st = ensure_stream2(c);
+        if (st) return st;
+        HIPCALL(c, hipEventRecord(c->ev_start, c->stream));
+        HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_start, 0));
+    }
+    for (int seg = last; seg >= 0; --seg) {
+        const int b = overlap ? (last - seg) & 1 : 0;
+        if (overlap && seg > 0) {
+            const int bn = b ^ 1;
+            if (seg < last) HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_free[bn], 0));
+            const hipStream_t main_stream = c->stream;
+            c->stream = c->stream2;
+            st = recompute(seg - 1, bn);
+            c->stream = main_stream;
+            if (st) return st;
+            HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream2));
+        }
+        if (seg < last) {
+            if (overlap) HIPCALL(c, hipStreamWaitEvent(c->stream, c->ev_ready[b], 0));
+            else {
+                st = recompute(seg, 0);
+                if (st) return st;
+            }
+        }
+        st = reverse(seg, b);
+        if (st) return st;
+        if (overlap) HIPCALL(c, hipEventRecord(c->ev_free[b], c->stream));
     }
     // leave the context at the final state of the forward pass
     st = load_state(c, ck_state(c, c->ck_states - 1));
