@@ -53,6 +53,10 @@ ALG_BYTES = {
         # read 9 FP32 fluxes, A32, U64 (+ dU32 in stages 2, 3); write dU32,
         # shadow(6) (+ U64 in stage 3): 140, 160, 200 B, mean over a step
         "vdiv_stage": 500 / 3,
+        # the stage kernel with the z sweep folded in: the stage's bytes and
+        # the sweep's shadow read (5); the sweep's A read and write are gone:
+        # 160, 180, 220 B
+        "vdiv_zs_stage": 500 / 3 + 20,
         "cfl": 20,            # read the FP32 shadow
     },
 }
@@ -137,8 +141,8 @@ def time_path(ns, H, n, st, Ma, Re, recon, prec, steps, warmup):
         # the fraction is of HBM, but what bounds these kernels is VALU issue
         roof["note"] = ("VALU-bound: the reference's correctly rounded FP64 divisions "
                         "(94 per cell and direction) dominate the sweep")
-    cfl_per_step = rep["cfl"][1] / steps if "cfl" in rep else 0.0   # CFL passes the timed steps ran
-    step_bytes = round(3 * sum(alg[k] for k in alg if k != "cfl") + alg["cfl"] * cfl_per_step, 1)
+    # the kernels the timed steps ran, each with its launches per step
+    step_bytes = round(sum(alg[k] * rep[k][1] / steps for k in rep if k in alg), 1)
     return {"value": value, "ms_per_step": ms_step, "wall_s_timed": round(wall, 4), "kernels": kernels,
             "roofline": roof,
             "kernel_bytes_per_cell_update": step_bytes,

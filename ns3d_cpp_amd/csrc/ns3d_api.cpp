@@ -29,6 +29,8 @@ struct StepSig {
     MFields m;
     FaceArrays faces;
     int mixed, char_recon, seg_len[3], seg_z, gradflux_minb, sweep_minw, m_gf_block[2], m_vd_block[2];
+    int zfuse;
+    Shadow sh_other;
     unsigned long long *d_flag;
     double dt;
     int dev_dt;
@@ -50,11 +52,20 @@ struct ns3d_ctx {
     int gf_rows = 3;
     int vd_lds = 1;    // LDS-staged form of the mixed divergence/stage pass on periodic boxes (NS3D_VD_LDS)
     int vd_img = 1;    // its stages write the shadow's periodic ghost images, no fill pass (NS3D_VD_IMG)
+    // the stages fold the z sweep into the stage kernel where it can
+    // (m_vdiv_zs_form), the new shadow into the other buffer (NS3D_ZFUSE)
+    int zfuse = 1;
+    Shadow sh_other{};            // the shadow buffer the current one is not
+    float *sh_base0 = nullptr;    // S[0] of the first buffer
+    long long sh_swaps = 0;       // buffer exchanges so far
     // captured RK3 steps: [0] fixed dt, [1] device dt (ns3d_advance*), used
     // with NS3D_GRAPHS=1.  Off by default: at 64^3-256^3 replays measured no
     // faster than eager launches (the kernels outlast the launch cost)
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    StepSig graph_sig[2];
+    // step graphs per (device-dt mode, which shadow buffer the step starts
+    // from: the stages that fold in the z sweep alternate the two)
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    StepSig graph_sig[2][2];
+    int graph_swap[2][2] = {{0, 0}, {0, 0}};   // the step exchanges the shadow buffers
     bool use_graphs = false;
     bool cfl_fuse = true;   // mixed third stages reduce the next CFL minimum (NS3D_CFL_FUSE=0: off)
     double *slab = nullptr;      // one allocation for all cell arrays
@@ -268,9 +279,25 @@ int inviscid(ns3d_ctx *c)
 }
 
 // one RHS evaluation of the mixed-precision path
+// the new shadow of a stage that folded in the z sweep is in the other
+// buffer: exchange the two
+void shadow_swap(ns3d_ctx *c)
+{
+    for (int q = 0; q < 6; ++q) std::swap(c->m.S[q], c->sh_other.S[q]);
+    c->sh_swaps++;
+}
+
+// a stage (mode 1..3) folds the z sweep into its stage kernel
+bool zs_stage(const ns3d_ctx *c, int mode)
+{
+    return mode >= 1 && c->zfuse && !c->P.char_recon &&
+           m_vdiv_zs_form(c->dp, c->m_vd_block[0], c->m_vd_block[1], c->vd_lds, c->vd_img);
+}
+
 int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
 {
     static const char *sname[3] = {"sweep_x", "sweep_y", "sweep_z"};
+    const bool zs = zs_stage(c, mode);
     if (c->P.char_recon) {
         // characteristic-wise faces in FP64 from the widened shadow (in the
         // gradient arrays G[6..11], unused in mixed mode)
@@ -285,10 +312,17 @@ int rhs_pass_mixed(ns3d_ctx *c, int mode, double dt, int dev_dt)
         for (int d = 0; d < 3; ++d) TIMED(c, fname[d], launch_faces(c->stream, c->dp, fw, c->faces, d, 1));
         TIMED(c, "face_diff", launch_m_face_diff(c->stream, c->dp, c->m, c->faces));
     } else {
-        for (int d = 0; d < 3; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
+        const int nd = zs ? 2 : 3;   // the z sweep folded into the stage kernel
+        for (int d = 0; d < nd; ++d) TIMED(c, sname[d], launch_m_sweep(c->stream, c->dp, c->m, d, c->seg_len[d]));
     }
     TIMED(c, "gradflux",
           launch_m_gradflux(c->stream, c->dp, c->m, c->seg_z, c->m_gf_block[0], c->m_gf_block[1], c->gf_rows));
+    if (zs) {
+        TIMED(c, "vdiv_zs_stage",
+              launch_m_vdiv_zs(c->stream, c->dp, c->m, c->sh_other, mode, dt, c->seg_z, c->d_flag, dev_dt));
+        shadow_swap(c);
+        return NS3D_OK;
+    }
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
           launch_m_vdiv(c->stream, c->dp, c->m, mode, dt, c->seg_z, c->d_flag, dev_dt, c->m_vd_block[0],
                         c->m_vd_block[1], c->vd_lds, c->vd_img));
@@ -384,6 +418,8 @@ void step_sig(const ns3d_ctx *c, double dt, int dev_dt, StepSig &g)
     g.sweep_minw = c->sweep_minw;
     for (int d = 0; d < 2; ++d) { g.m_gf_block[d] = c->m_gf_block[d]; g.m_vd_block[d] = c->m_vd_block[d]; }
     g.d_flag = c->d_flag;
+    g.zfuse = c->zfuse;
+    g.sh_other = c->sh_other;
     g.dt = dev_dt ? 0.0 : dt;
     g.dev_dt = dev_dt;
 }
@@ -395,36 +431,45 @@ void step_sig(const ns3d_ctx *c, double dt, int dev_dt, StepSig &g)
 int step_graph(ns3d_ctx *c, double dt, int dev_dt)
 {
     const int k = dev_dt ? 1 : 0;
+    const int par = c->mixed && c->m.S[0] != c->sh_base0 ? 1 : 0;
+    hipGraphExec_t &gx = c->graph[k][par];
+    StepSig &gsig = c->graph_sig[k][par];
     StepSig want;
     step_sig(c, dt, dev_dt, want);
-    if (!c->graph[k] || std::memcmp(&want, &c->graph_sig[k], sizeof(StepSig)) != 0) {
-        if (c->graph[k]) {
+    if (!gx || std::memcmp(&want, &gsig, sizeof(StepSig)) != 0) {
+        if (gx) {
             // the old graph's last launch may still be running
             const int ss = stream_sync(c);
             if (ss) return ss;
-            HIPCALL(c, hipGraphExecDestroy(c->graph[k]));
-            c->graph[k] = nullptr;
+            HIPCALL(c, hipGraphExecDestroy(gx));
+            gx = nullptr;
         }
         hipGraph_t g = nullptr;
         HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const long long steps = c->steps_done;
+        const long long steps = c->steps_done, swaps = c->sh_swaps;
         const int st = rk3_enqueue(c, dt, dev_dt, false);
         c->steps_done = steps;
+        // the shadow buffers as before the step: the launch below exchanges
+        // them as the step does
+        c->graph_swap[k][par] = (int)((c->sh_swaps - swaps) & 1);
+        if (c->graph_swap[k][par]) shadow_swap(c);
+        c->sh_swaps = swaps;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st) {
             if (e == hipSuccess && g) (void)hipGraphDestroy(g);
             return st;
         }
         if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-        const hipError_t ei = hipGraphInstantiate(&c->graph[k], g, nullptr, nullptr, 0);
+        const hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
-            c->graph[k] = nullptr;
+            gx = nullptr;
             return hip_fail(c, ei, "hipGraphInstantiate");
         }
-        std::memcpy(&c->graph_sig[k], &want, sizeof(StepSig));
+        std::memcpy(&gsig, &want, sizeof(StepSig));
     }
-    HIPCALL(c, hipGraphLaunch(c->graph[k], c->stream));
+    HIPCALL(c, hipGraphLaunch(gx, c->stream));
+    if (c->graph_swap[k][par]) shadow_swap(c);
     c->steps_done++;
     return NS3D_OK;
 }
@@ -474,11 +519,13 @@ int alloc_mixed(ns3d_ctx *c)
 {
     if (c->fslab) return NS3D_OK;
     const long long ncell = c->ncell;
-    const int nf = 6 + 5 + 9;
+    const int nf = 6 + 6 + 5 + 9;
     HIPCALL(c, hipMalloc((void **)&c->fslab, (size_t)ncell * nf * sizeof(float)));
     HIPCALL(c, hipMemsetAsync(c->fslab, 0, (size_t)ncell * nf * sizeof(float), c->stream));
     float *fq = c->fslab;
     for (int m = 0; m < 6; ++m) { c->m.S[m] = fq; fq += ncell; }
+    for (int m = 0; m < 6; ++m) { c->sh_other.S[m] = fq; fq += ncell; }
+    c->sh_base0 = c->m.S[0];
     for (int m = 0; m < 5; ++m) { c->m.A[m] = fq; fq += ncell; }
     // 9 distinct viscous-flux arrays: the y and z fluxes' txy, txz, tyz are
     // the x and y fluxes' (symmetric stress)
@@ -900,6 +947,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     if (const char *env = std::getenv("NS3D_CFL_FUSE")) c->cfl_fuse = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_VD_LDS")) c->vd_lds = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_VD_IMG")) c->vd_img = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_ZFUSE")) c->zfuse = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GF_ROWS")) {
         const int v = std::atoi(env);
         c->gf_rows = v >= 1 && v <= 3 ? v : 2;
@@ -939,8 +987,9 @@ int ns3d_destroy(ns3d_ctx *c)
     if (c->mon_part) (void)hipFree(c->mon_part);
     for (auto &q : c->pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-    for (auto &g : c->graph)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto &row : c->graph)
+        for (auto &g : row)
+            if (g) (void)hipGraphExecDestroy(g);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     for (hipEvent_t e : {c->ev_start, c->ev_ready[0], c->ev_ready[1], c->ev_free[0], c->ev_free[1]})

@@ -29,6 +29,13 @@ struct MFields {
     float *VF[12];   // viscous fluxes
 };
 
+// the second shadow buffer: the output of the stage kernel that folds in the
+// z sweep (km_vdiv_zs), which reads the input shadow across its segments'
+// ends and so cannot overwrite it
+struct Shadow {
+    float *S[6];
+};
+
 // Arrays of the tangent-linear and adjoint RHS (FP64, interior cells used).
 //   TLM:     X = dU (in), Y = dW, T = face-flux tangents / dT, V = dVF, Z = dR (out)
 //   adjoint: X = lambda (in), Y = lambda_W, G = lambda of the gradients,
@@ -111,6 +118,9 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
 hipError_t launch_m_vdiv(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                          unsigned long long *flag, int dev_dt, int bx = 64, int by = 4, int lds = 0, int img = 0);
 bool m_vdiv_lds_form(const DevParams &P, int bx, int by, int lds);
+bool m_vdiv_zs_form(const DevParams &P, int bx, int by, int lds, int img);
+hipError_t launch_m_vdiv_zs(hipStream_t s, const DevParams &P, const MFields &M, const Shadow &So, int mode,
+                            double dt, int seg, unsigned long long *flag, int dev_dt);
 // the workgroup shape tiles the interior with whole waves: km_vdiv_z can
 // reduce the next step's CFL minimum (dev_dt 2)
 bool m_vdiv_cfl_next(const DevParams &P, int bx, int by);

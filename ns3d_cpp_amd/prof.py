@@ -48,6 +48,8 @@ def kind_of(name):
         return ["sweep_x", "sweep_y", "sweep_z"][int(m.group(1))]
     if "gradflux" in n:
         return "gradflux"
+    if re.search(r"km_vdiv_zs<", n):
+        return "vdiv_zs_stage"
     m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, (?:\d|true|false))?>", n)
     if m:
         return "vdiv_rhs" if m.group(1) == "0" else "vdiv_stage"
@@ -64,7 +66,7 @@ def kind_of(name):
 
 def stage_of(name):
     """RK3 stage (1..3) of a stage-kernel instantiation, else None."""
-    m = re.search(r"km?_vdiv(?:_z|_lds)?<\d+, (\d)(?:, (?:\d|true|false))?>", name.replace("ns3d::", ""))
+    m = re.search(r"km?_vdiv(?:_z|_lds|_zs)?<\d+, (\d)(?:, (?:\d|true|false))*>", name.replace("ns3d::", ""))
     return int(m.group(1)) if m and m.group(1) != "0" else None
 
 
@@ -120,7 +122,8 @@ def counter_rows(counters, n3):
         row["valu_activity"] = (g("SQ_ACTIVE_INST_VALU") / g("SQ_BUSY_CYCLES")
                                 if g("SQ_ACTIVE_INST_VALU") and g("SQ_BUSY_CYCLES") else None)
         rows.append(row)
-    order = ["copy", "copy_nt", "sweep_x", "sweep_y", "sweep_z", "gradflux", "vdiv_stage", "fill_ghosts", "cfl"]
+    order = ["copy", "copy_nt", "sweep_x", "sweep_y", "sweep_z", "gradflux", "vdiv_stage", "vdiv_zs_stage",
+             "fill_ghosts", "cfl"]
     rows.sort(key=lambda r: (order.index(r["kind"]) if r["kind"] in order else 99, r["kernel"]))
     return rows
 
@@ -153,6 +156,8 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--alg", default="", help="kind=bytes_per_cell,... algorithmic bytes")
     ap.add_argument("--stage-bytes", default="", help="algorithmic bytes of the stage kernel's stages 1,2,3")
+    ap.add_argument("--zs-stage-bytes", default="",
+                    help="the same for the stage kernel with the z sweep folded in (vdiv_zs_stage)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
     a = ap.parse_args()
@@ -163,6 +168,7 @@ def main():
         k, v = kv.split("=")
         alg[k] = float(v)
     stage_bytes = [float(x) for x in a.stage_bytes.split(",")] if a.stage_bytes else None
+    zs_bytes = [float(x) for x in a.zs_stage_bytes.split(",")] if a.zs_stage_bytes else None
     lines = [f"# {a.title}", ""]
     if a.command:
         lines += ["Commands (run from the repository root on the GPU box):", "", "```", a.command, "```", ""]
@@ -197,8 +203,9 @@ def main():
             tr = (f[k] * fs + w.get(k, 0.0) * ws) * 1024.0
             al = alg.get(kind)
             st = stage_of(k)
-            if st is not None and stage_bytes:
-                al = stage_bytes[st - 1]
+            sb = zs_bytes if kind == "vdiv_zs_stage" else stage_bytes
+            if st is not None and sb:
+                al = sb[st - 1]
                 per_kind[f"{kind}_{st}"].append(tr)
             per_kind[kind].append(tr)
             lines.append(f"| `{short(k)}` | {kind}{'' if st is None else f' (stage {st})'} | {f[k]:.0f} | "

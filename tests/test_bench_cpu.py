@@ -1,7 +1,8 @@
 """bench.py's contract parts that need no GPU: more than one GPU is refused
 with a message, and the per-kernel byte model adds up to what its comments
-state (the mixed stage kernel averages its three stages' 140 / 160 / 200 B;
-the mixed step has no CFL pass of its own)."""
+state (the mixed stage kernel averages its three stages' 140 / 160 / 200 B,
+160 / 180 / 220 B with the z sweep folded in; the mixed step has no CFL pass
+of its own)."""
 import importlib.util
 import os
 import subprocess
@@ -37,7 +38,13 @@ def test_byte_model():
     # not at all in mixed mode (its third stage reduces the next minimum)
     step = lambda a, ncfl: 3 * sum(v for k, v in a.items() if k != "cfl") + a["cfl"] * ncfl
     assert step(fp, 1) == 3 * (80 + 120 + 120 + 104 + 240) + 40
-    assert abs(step(mx, 0) - (3 * (40 + 60 + 60 + 52) + 500)) < 1e-9
+    sep = {k: v for k, v in mx.items() if k != "vdiv_zs_stage"}
+    assert abs(step(sep, 0) - (3 * (40 + 60 + 60 + 52) + 500)) < 1e-9
+    # the z sweep folded into the stage kernel: that kernel also reads the
+    # shadow (20 B), and the sweep's accumulator read and write (40 B) are gone
+    assert abs(mx["vdiv_zs_stage"] - (sum(stages) / 3 + 20)) < 1e-12
+    fused = {k: v for k, v in mx.items() if k not in ("vdiv_stage", "sweep_z")}
+    assert abs(step(fused, 0) - (3 * (40 + 60 + 52) + 500 + 60)) < 1e-9
     # the metric's model keeps the CFL read in both (2408 + 40, 1544 + 20)
     assert b.STEP_MODEL_BYTES == {"fp64": 2448, "mixed": 1564}
 
@@ -58,4 +65,8 @@ def test_prof_kinds_and_stages():
     assert prof.stage_of("void ns3d::km_vdiv_z<0, 2, 0>(ns3d::DevParams)") == 2
     assert prof.stage_of("void ns3d::km_vdiv_z<0, 0, 0>(ns3d::DevParams)") is None
     assert prof.kind_of("ns3d::k_copy_nt(double2 const*, double2*, long)") == "copy_nt"
+    assert prof.kind_of("void ns3d::km_vdiv_zs<0, 3, 0, 0>(ns3d::DevParams, ns3d::MFields)") == "vdiv_zs_stage"
+    assert prof.stage_of("void ns3d::km_vdiv_zs<0, 3, 0, 0>(ns3d::DevParams, ns3d::MFields)") == 3
+    assert prof.kind_of("void ns3d::km_vdiv_lds<0, 1, true>(ns3d::DevParams)") == "vdiv_stage"
+    assert prof.stage_of("void ns3d::km_vdiv_lds<0, 1, true>(ns3d::DevParams)") == 1
     assert prof.kind_of("ns3d::k_copy(double2 const*, double2*, long)") == "copy"

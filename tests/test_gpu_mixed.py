@@ -168,42 +168,70 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
     with each plane's x / y neighbourhood staged in LDS.  The divergence and
     stage pass has cached stencil loads or an LDS-staged plane, the latter
     writing the new shadow's periodic ghost images itself (no fill pass) or
-    not.  All of them evaluate the same arithmetic on the same values, so the
-    RHS, two RK3 steps at a fixed dt, two CFL steps with the device time step
-    (the third stage reducing the next CFL minimum) and the downloaded
-    primitives, ghost layers included, must agree bit for bit.  The
-    box is 64 x 32 x 24 (a different spacing per axis) in 2 z segments of
-    unequal length: tile shapes and segment ends that the 256^3 workload
-    does not exercise."""
+    not.  The stage pass can also fold in the z sweep, writing the new shadow
+    into a second buffer (the two alternate, each with its own step graph).
+    All of them evaluate the same arithmetic on the same values, so the RHS,
+    two RK3 steps at a fixed dt, three graph-replayed steps, four CFL steps
+    with the device time step (the third stage reducing the next CFL
+    minimum) and the downloaded primitives, ghost layers included, must
+    agree bit for bit.  The box is 64 x 32 x 24 (a different spacing per
+    axis) in 2 z segments of unequal length: tile shapes and segment ends
+    that the 256^3 workload does not exercise."""
+    out = _mixed_forms(vis, CFG, monkeypatch, [("1", "0", "0", "0"), ("2", "0", "0", "0"), ("3", "0", "0", "0"),
+                                               ("1", "1", "0", "0"), ("3", "1", "0", "0"), ("3", "1", "1", "0"),
+                                               ("3", "1", "1", "1")])
+    _assert_forms_equal(out)
+
+
+def _mixed_forms(vis, cfg, monkeypatch, forms):
     nx, ny, nz = 64, 32, 24
     rng = np.random.default_rng(7)
     st, Ma = fields.hit(nz, k0=2, Mt=0.3, seed=11)
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    forms = [("1", "0", "0"), ("2", "0", "0"), ("3", "0", "0"), ("1", "1", "0"), ("3", "1", "0"), ("3", "1", "1")]
-    for rows, lds, img in forms:
+    for rows, lds, img, zf in forms:
         monkeypatch.setenv("NS3D_GF_ROWS", rows)
         monkeypatch.setenv("NS3D_VD_LDS", lds)
         monkeypatch.setenv("NS3D_VD_IMG", img)
+        monkeypatch.setenv("NS3D_ZFUSE", zf)
         monkeypatch.setenv("NS3D_SEG_Z", "16")
-        p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(CFG, vis_scheme=vis))
+        p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(cfg, vis_scheme=vis))
         s = ns.Solver(p, g)
         s.upload_primitive(*x)
         r = [s.interior(a).copy() for a in s.compute_rhs()]
         s.rk3_step(1e-3)
         s.rk3_step(1e-3)
         u = [s.interior(a).copy() for a in s.conserved()]
-        t = s.advance_cfl(2)
+        s.advance(3, 1e-3)
+        u1 = [s.interior(a).copy() for a in s.conserved()]
+        t = s.advance_cfl(4)
         u2 = [s.interior(a).copy() for a in s.conserved()]
         w = list(s.primitive().values())
         s.close()
-        out[(rows, lds, img)] = (r + u + u2 + w, t)
+        out[(rows, lds, img, zf)] = (r + u + u1 + u2 + w, t)
+    return out
+
+
+def _assert_forms_equal(out):
+    forms = list(out)
     ref = out[forms[0]]
     for f in forms[1:]:
         assert out[f][1] == ref[1], f
         for a, b in zip(out[f][0], ref[0]):
             assert np.array_equal(a, b), f
+
+
+@pytest.mark.parametrize("recon,fvs,eps", [("WENO5", "LaxFriedrichs", 0.0), ("WENO5", "StagerWarming", 0.2),
+                                           ("UPWIND_7TH", "StagerWarming", 0.0),
+                                           ("UPWIND_7TH", "LaxFriedrichs", 0.0)])
+def test_z_sweep_folded_into_stage_bitwise(gpu, recon, fvs, eps, monkeypatch):
+    """The stage kernel with the z sweep folded in against the separate z
+    sweep and stage kernels, bit for bit, for the splits and
+    reconstructions it is built for."""
+    cfg = dict(CFG, recon=recon, fvs_type=fvs, sw_eps=eps)
+    out = _mixed_forms("C8th", cfg, monkeypatch, [("3", "1", "1", "0"), ("3", "1", "1", "1")])
+    _assert_forms_equal(out)
 
 
 @pytest.mark.parametrize("recon,fvs,eps", [("WENO5", "StagerWarming", 0.0), ("WENO5", "LaxFriedrichs", 0.0),
