@@ -53,9 +53,10 @@ def test_traffic_record_is_keyed_by_size_and_scheme():
     """The roofline's measured traffic comes from the profile of the same
     size and scheme, and is null where none was recorded."""
     b = load_bench()
-    assert isinstance(b.traffic_from_profiles("mixed", "vdiv_stage", 256), int)
-    assert b.traffic_from_profiles("mixed", "vdiv_stage", 256, "UPWIND_7TH") is None
-    assert b.traffic_from_profiles("mixed", "vdiv_stage", 96) is None
+    assert isinstance(b.traffic_from_profiles("mixed", "vdiv_zs_stage", 256), int)
+    assert isinstance(b.traffic_from_profiles("fp64", "vdiv_stage", 256), int)
+    assert b.traffic_from_profiles("mixed", "vdiv_zs_stage", 256, "UPWIND_7TH") is None
+    assert b.traffic_from_profiles("mixed", "vdiv_zs_stage", 96) is None
 
 
 def test_prof_kinds_and_stages():
