@@ -1259,27 +1259,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// km_vdiv_z on a periodic box with each plane's x / y stencil fluxes staged
-// in LDS: a 64 x 4 workgroup writes its plane's x fluxes with an R-cell x
-// halo and its y fluxes with an R-row y halo (periodic images of interior
-// cells: the flux arrays hold no ghosts) into one of two buffers, one
-// barrier per plane, and takes both derivatives from there.  Everything the
-// next plane needs (own fluxes, halos, the entering z-flux plane, A, U, dU)
-// is loaded before this plane's arithmetic, so the loads of a plane overlap
-// the arithmetic of the previous one: about 35 loads per cell instead of 91.
-// The same arithmetic in the same order as km_vdiv_z<VIS, MODE, 0>: equal bit
-// for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before a
-// barrier).
+// The LDS-staged stage pass on a periodic box, one z segment of a 64 x 4
+// tile per workgroup (km_vdiv_lds; with ZS, km_vdiv_zs).
+//
+// km_vdiv_z's arithmetic with each plane's x / y stencil fluxes staged in
+// LDS: the workgroup writes its plane's x fluxes with an R-cell x halo and
+// its y fluxes with an R-row y halo (periodic images of interior cells: the
+// flux arrays hold no ghosts) into one of two buffers, one barrier per
+// plane, and takes both derivatives from there.  Everything the next plane
+// needs (own fluxes, halos, the entering z-flux plane, A, U, dU) is loaded
+// before this plane's arithmetic, so the loads of a plane overlap the
+// arithmetic of the previous one: about 35 loads per cell instead of 91.
+// The same arithmetic in the same order as km_vdiv_z<VIS, MODE, 0>: equal
+// bit for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before
+// a barrier).
 // IMG (MODE 1..3): the stage also writes the new shadow's periodic ghost
-// images (the face layers of the fill pass), so that pass can be skipped
-template <int VIS, int MODE, bool IMG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
-    DevParams P, MFields M, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
+// images (the face layers of the fill pass), so that pass can be skipped.
+// ZS (MODE 1..3, with IMG): the same stage's z sweep folded in.  Each thread
+// also marches its column's z sweep (the state and arithmetic of
+// km_sweep<2>, its minus-side ring 256 lanes wide), which at plane k yields
+// that cell's z flux difference, added to the accumulator of the x and y
+// sweeps as km_sweep<2> adds it.  The accumulator's z update is then
+// neither stored nor read back (40 B per cell), and the sweep's arithmetic
+// runs beside the stage's loads.  The z sweep of a segment reads the input
+// shadow M.S across the segment's ends, which other workgroups' stages
+// overwrite: the new shadow goes to So (a second buffer, the next stage's
+// input).  Equal bit for bit to km_sweep<2> followed by the stage.
+template <int VIS, int MODE, bool IMG, bool ZS, int RECON, int FVS>
+__device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields &M, const Shadow &So, double dt,
+                                              int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
     constexpr int R = VisReach<VIS>::R;
     constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
     __shared__ float fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
     __shared__ float fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
+    // the z sweep (km_sweep<2>): per-lane ring of the minus side's window
+    constexpr int S = Stencil<RECON>::S;
+    constexpr int MID = (S - 1) / 2;
+    constexpr int T0 = S == 2 ? 0 : 1;
+    constexpr int L = S - T0;
+    constexpr int R0 = 5;
+    constexpr bool CARRY = ZS && RECON == R_WENO5 && FVS != FVS_SWE;
+    constexpr int NL = ZS ? 256 : 1;
+    __shared__ float ring[ZS ? L : 1][5][NL];
+    __shared__ float uring[CARRY ? 2 : 1][5][NL];
     const int ng = P.ng;
     const int t = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
@@ -1291,6 +1314,131 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
     const int sx = P.sx, sxy = P.sx * P.sy;
     const int col = j * sx + i;
+    MFields Mo = M;   // the stage's outputs; with ZS the shadow into So
+    if constexpr (ZS) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Mo.S[q] = So.S[q];
+    }
+
+    // ---- z sweep state (km_sweep<2> with base = col, st = sxy)
+    const int lane = ZS ? tid : 0;
+    const int st = sxy;
+    float e[S];
+    float wp[5][S];
+    int n0 = L - 1;
+    double prev[10];
+    float nx_[5];
+    float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float up0[5], up1[5];
+    const auto slot = [&n0](int tt) { const int q = tt - T0 + n0; return q >= L ? q - L : q; };
+    const auto get = [&](int m, int tt) -> float {
+        if (m >= R0) return ring[slot(tt)][m - R0][lane];
+        return wp[m][tt];
+    };
+    const auto push = [&](const double *Fp, const double *Fm) {
+#pragma unroll
+        for (int tt = T0; tt < S - 1; ++tt)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) wp[m][tt] = wp[m][tt + 1];
+        const int q = slot(S - 1);
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
+            const float d = (float)(Fv - prev[m]);
+            if (m >= R0) ring[q][m - R0][lane] = d;
+            else wp[m][S - 1] = d;
+            prev[m] = Fv;
+        }
+    };
+    // one step of the z sweep: face f; for f >= k0 the cell's flux difference
+    // (times 1/dz as the pair of km_sweep) added to a[]
+    const auto zstep = [&](int f, float *a) {
+        n0 = n0 + 1 == L ? 0 : n0 + 1;
+        {
+            float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
+            if (f + 1 < k1) {
+                const int id = col + (f + 1 - MID + S - 1) * st;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
+            }
+            double Fp[5], Fm[5];
+            split_fast<FVS, 2>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
+                               P.p_ref + (double)cur[4], Fp, Fm);
+            push(Fp, Fm);
+        }
+        float rel[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            float D[S], Dm[S];
+#pragma unroll
+            for (int tt = 1; tt < S; ++tt) {
+                D[tt] = get(m, tt);
+                Dm[tt] = -get(5 + m, S - tt);
+            }
+            if constexpr (CARRY) {
+                const float up2 = weno5_u(D[4] - D[3]);
+                const float um0 = weno5_u(Dm[2] - Dm[1]);
+                const float um1 = uring[(f + 1) & 1][m][lane];
+                const float um2 = uring[f & 1][m][lane];
+                uring[f & 1][m][lane] = um0;
+                rel[m] = rel32_weno5_u(D, up0[m], up1[m], up2) + rel32_weno5_u(Dm, um0, um1, um2);
+                up0[m] = up1[m];
+                up1[m] = up2;
+            } else {
+                rel[m] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
+            }
+        }
+        if (f >= k0) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const float dF = get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m]);
+                a[m] = a[m] + mul2(dF, P.id2[2]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
+    };
+    if constexpr (ZS) {
+        diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);
+        {
+            double rho, u, v, w, p, Fp[5], Fm[5];
+            shadow_prims(P, M, col + (k0 - 2 - MID + T0) * st, rho, u, v, w, p);
+            split_fast<FVS, 2>(P, rho, u, v, w, p, Fp, Fm);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) { prev[m] = Fp[m]; prev[5 + m] = Fm[m]; }
+        }
+#pragma unroll
+        for (int tt = T0 + 1; tt < S; ++tt) {
+            double rho, u, v, w, p, Fp[5], Fm[5];
+            shadow_prims(P, M, col + (k0 - 2 - MID + tt) * st, rho, u, v, w, p);
+            split_fast<FVS, 2>(P, rho, u, v, w, p, Fp, Fm);
+            const int q = tt - T0 - 1;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) wp[m][tt] = (float)(Fp[m] - prev[m]);
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+                const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
+                if (m >= R0) ring[q][m - R0][lane] = (float)(Fv - prev[m]);
+                prev[m] = Fv;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][col + (k0 - 1 - MID + S - 1) * st];
+        if constexpr (CARRY) {
+            const int f0 = k0 - 1;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                up0[m] = weno5_u(wp[m][3] - wp[m][2]);
+                up1[m] = weno5_u(wp[m][4] - wp[m][3]);
+                uring[(f0 + 1) & 1][m][lane] = weno5_u(get(5 + m, 5) - get(5 + m, 4));   // s1 of face f0
+                uring[f0 & 1][m][lane] = weno5_u(get(5 + m, 4) - get(5 + m, 3));         // s2 of face f0
+            }
+        }
+        float dummy[5];
+        zstep(k0 - 1, dummy);
+    }
+
+    // ---- the stage pass
     // the x-halo cell of this thread (tid < 32 R): flux tid / 8R, tile row
     // (tid / 2R) % 4, halo column tid % 2R
     const bool xh_on = tid < 32 * R;
@@ -1377,6 +1525,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         __syncthreads();
         if (k + 1 < k1) prefetch(k + 1);
+        if constexpr (ZS) zstep(k, an);
         double r[5];
         r[0] = -(double)an[0];
 #pragma unroll
@@ -1393,7 +1542,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         if constexpr (IMG && MODE != 0) {
             float sh[6];
-            vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz, sh);
+            vdiv_finish<MODE, true>(P, Mo, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz, sh);
             // the cell's images in the ghost layers of each axis: i < 2 ng is
             // the right ghost i + n, i >= n the left ghost i - n
             const int im[3] = {i, j, k};
@@ -1403,15 +1552,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int a = 0; a < 3; ++a) {
                 if (im[a] < 2 * ng) {
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) M.S[q][id + nn[a] * str[a]] = sh[q];
+                    for (int q = 0; q < 6; ++q) Mo.S[q][id + nn[a] * str[a]] = sh[q];
                 }
                 if (im[a] >= nn[a]) {
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) M.S[q][id - nn[a] * str[a]] = sh[q];
+                    for (int q = 0; q < 6; ++q) Mo.S[q][id - nn[a] * str[a]] = sh[q];
                 }
             }
         } else {
-            vdiv_finish<MODE, true>(P, M, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
+            vdiv_finish<MODE, true>(P, Mo, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz);
         }
     }
     if (cfl_next) {
@@ -1420,284 +1569,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// km_vdiv_lds (MODE 1..3, ghost images written) with the same stage's z
-// sweep folded in: each thread also marches its column's z sweep (the state
-// and arithmetic of km_sweep<2>), which at plane k yields that cell's z
-// flux difference, added to the accumulator of the x and y sweeps as
-// km_sweep<2> adds it.  The accumulator's z update is then neither stored
-// nor read back (40 B per cell), and the sweep's arithmetic runs beside the
-// stage's loads.  The z sweep of a segment reads the input shadow M.S across
-// the segment's ends, which other workgroups' stages overwrite: the new
-// shadow goes to So (a second buffer, the next stage's input).  Equal bit
-// for bit to km_sweep<2> followed by km_vdiv_lds.
+// Occupancy 2 for both: at the default target of 4 waves the compiler waits
+// for the ~35 loads of a plane in small groups
+template <int VIS, int MODE, bool IMG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
+    DevParams P, MFields M, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
+{
+    vdiv_lds_pass<VIS, MODE, IMG, false, R_WENO5, FVS_SW>(P, M, Shadow{}, dt, seg_len, gx, gy, flag, dev_dt);
+}
+
 template <int VIS, int MODE, int RECON, int FVS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_zs(
     DevParams P, MFields M, Shadow So, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
-    constexpr int R = VisReach<VIS>::R;
-    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
-    __shared__ float fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
-    __shared__ float fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
-    // the z sweep (km_sweep<2>): per-lane ring of the minus side's window
-    constexpr int S = Stencil<RECON>::S;
-    constexpr int MID = (S - 1) / 2;
-    constexpr int T0 = S == 2 ? 0 : 1;
-    constexpr int L = S - T0;
-    constexpr int R0 = 5;
-    constexpr bool CARRY = RECON == R_WENO5 && FVS != FVS_SWE;
-    __shared__ float ring[L][5][256];
-    __shared__ float uring[CARRY ? 2 : 1][5][256];
-    const int ng = P.ng;
-    const int t = xcd_logical(blockIdx.x, gridDim.x);
-    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int i0 = ng + bx * 64, j0 = ng + by * 4;
-    const int i = i0 + tx, j = j0 + ty;
-    const int k0 = ng + bz * seg_len;
-    const int k1 = min(k0 + seg_len, ng + P.nz);
-    if (dev_dt) dt = cfl_dt(P, flag);
-    const int sx = P.sx, sxy = P.sx * P.sy;
-    const int col = j * sx + i;
-    MFields Mo = M;   // the stage's outputs, the shadow into So
-#pragma unroll
-    for (int q = 0; q < 6; ++q) Mo.S[q] = So.S[q];
-
-    // ---- z sweep state (km_sweep<2> with base = col, st = sxy)
-    const int lane = tid;
-    const int st = sxy;
-    float e[S];
-    diff_coeffs<RECON == R_WENO5 ? R_MDCD : RECON>(P, e);
-    float wp[5][S];
-    int n0 = L - 1;
-    const auto slot = [&n0](int tt) { const int q = tt - T0 + n0; return q >= L ? q - L : q; };
-    const auto get = [&](int m, int tt) -> float {
-        if (m >= R0) return ring[slot(tt)][m - R0][lane];
-        return wp[m][tt];
-    };
-    double prev[10];
-    {
-        double rho, u, v, w, p, Fp[5], Fm[5];
-        shadow_prims(P, M, col + (k0 - 2 - MID + T0) * st, rho, u, v, w, p);
-        split_fast<FVS, 2>(P, rho, u, v, w, p, Fp, Fm);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) { prev[m] = Fp[m]; prev[5 + m] = Fm[m]; }
-    }
-    const auto push = [&](const double *Fp, const double *Fm) {
-#pragma unroll
-        for (int tt = T0; tt < S - 1; ++tt)
-#pragma unroll
-            for (int m = 0; m < 5; ++m) wp[m][tt] = wp[m][tt + 1];
-        const int q = slot(S - 1);
-#pragma unroll
-        for (int m = 0; m < 10; ++m) {
-            const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
-            const float d = (float)(Fv - prev[m]);
-            if (m >= R0) ring[q][m - R0][lane] = d;
-            else wp[m][S - 1] = d;
-            prev[m] = Fv;
-        }
-    };
-#pragma unroll
-    for (int tt = T0 + 1; tt < S; ++tt) {
-        double rho, u, v, w, p, Fp[5], Fm[5];
-        shadow_prims(P, M, col + (k0 - 2 - MID + tt) * st, rho, u, v, w, p);
-        split_fast<FVS, 2>(P, rho, u, v, w, p, Fp, Fm);
-        const int q = tt - T0 - 1;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) wp[m][tt] = (float)(Fp[m] - prev[m]);
-#pragma unroll
-        for (int m = 0; m < 10; ++m) {
-            const double Fv = m < 5 ? Fp[m] : Fm[m - 5];
-            if (m >= R0) ring[q][m - R0][lane] = (float)(Fv - prev[m]);
-            prev[m] = Fv;
-        }
-    }
-    float nx_[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][col + (k0 - 1 - MID + S - 1) * st];
-    float relprev[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float up0[5], up1[5];
-    if constexpr (CARRY) {
-        const int f0 = k0 - 1;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            up0[m] = weno5_u(wp[m][3] - wp[m][2]);
-            up1[m] = weno5_u(wp[m][4] - wp[m][3]);
-            uring[(f0 + 1) & 1][m][lane] = weno5_u(get(5 + m, 5) - get(5 + m, 4));
-            uring[f0 & 1][m][lane] = weno5_u(get(5 + m, 4) - get(5 + m, 3));
-        }
-    }
-    // one step of the z sweep: face f; for f >= k0 the cell's flux difference
-    // (times 1/dz as the pair of km_sweep) added to a[]
-    const auto zstep = [&](int f, float *a) {
-        n0 = n0 + 1 == L ? 0 : n0 + 1;
-        {
-            float cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
-            if (f + 1 < k1) {
-                const int id = col + (f + 1 - MID + S - 1) * st;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) nx_[q] = M.S[q][id];
-            }
-            double Fp[5], Fm[5];
-            split_fast<FVS, 2>(P, P.rho_ref + (double)cur[0], (double)cur[1], (double)cur[2], (double)cur[3],
-                               P.p_ref + (double)cur[4], Fp, Fm);
-            push(Fp, Fm);
-        }
-        float rel[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            float D[S], Dm[S];
-#pragma unroll
-            for (int tt = 1; tt < S; ++tt) {
-                D[tt] = get(m, tt);
-                Dm[tt] = -get(5 + m, S - tt);
-            }
-            if constexpr (CARRY) {
-                const float up2 = weno5_u(D[4] - D[3]);
-                const float um0 = weno5_u(Dm[2] - Dm[1]);
-                const float um1 = uring[(f + 1) & 1][m][lane];
-                const float um2 = uring[f & 1][m][lane];
-                uring[f & 1][m][lane] = um0;
-                rel[m] = rel32_weno5_u(D, up0[m], up1[m], up2) + rel32_weno5_u(Dm, um0, um1, um2);
-                up0[m] = up1[m];
-                up1[m] = up2;
-            } else {
-                rel[m] = rel_scale<RECON>(rel32<RECON>(e, D) + rel32<RECON>(e, Dm));
-            }
-        }
-        if (f >= k0) {
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const float dF = get(m, MID) + get(5 + m, MID + 1) + (rel[m] - relprev[m]);
-                a[m] = a[m] + mul2(dF, P.id2[2]);
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) relprev[m] = rel[m];
-    };
-    {
-        float dummy[5];
-        zstep(k0 - 1, dummy);
-    }
-
-    // ---- stage kernel state (km_vdiv_lds)
-    const bool xh_on = tid < 32 * R;
-    const int xh_q = xh_on ? tid / (8 * R) : 0;
-    const int xh_r = (tid / (2 * R)) % 4;
-    const int xh_c = tid % (2 * R);
-    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
-    const int xh_off = (j0 + xh_r) * sx + wrapi_m(i0 - R + xh_tc, ng, P.nx);
-    constexpr int NYH = (2 * R + 3) / 4;
-    int yh_row[NYH], yh_off[NYH];
-#pragma unroll
-    for (int n = 0; n < NYH; ++n) {
-        const int h = ty + 4 * n;
-        const int tr = h < R ? h : h + 4;
-        yh_row[n] = h < 2 * R ? tr : -1;
-        yh_off[n] = wrapi_m(j0 - R + tr, ng, P.ny) * sx + i;
-    }
-    float pfx[4], pfy[3], pxh = 0.0f, pyh[NYH][4], pz[4], pa[5], pdu[5];
-    double pu[5];
-    const auto prefetch = [&](int kp) {
-        const int pl = kp * sxy;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pfx[q] = M.VF[q][pl + col];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) pfy[q] = M.VF[5 + q][pl + col];
-        if (xh_on) pxh = M.VF[xh_q][pl + xh_off];
-#pragma unroll
-        for (int n = 0; n < NYH; ++n)
-            if (yh_row[n] >= 0) {
-                pyh[n][0] = M.VF[1][pl + yh_off[n]];
-#pragma unroll
-                for (int q = 1; q < 4; ++q) pyh[n][q] = M.VF[4 + q][pl + yh_off[n]];
-            }
-        const int zn = wrapi_m(kp + R, ng, P.nz) * sxy + col;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) pz[m] = M.VF[8 + m][zn];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            pa[m] = M.A[m][pl + col];
-            pu[m] = M.U[m][pl + col];
-            if constexpr (MODE >= 2) pdu[m] = M.dU[m][pl + col];
-        }
-    };
-    float win[4][2 * R + 1];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int o = 1; o < 2 * R + 1; ++o)
-            win[m][o] = M.VF[8 + m][wrapi_m(k0 - R - 1 + o, ng, P.nz) * sxy + col];
-    prefetch(k0);
-    const bool cfl_next = MODE == 3 && dev_dt == 2;
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int k = k0; k < k1; ++k) {
-        const int id = k * sxy + col;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-#pragma unroll
-            for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
-            win[m][2 * R] = pz[m];
-        }
-        float (*FX)[4][TC] = fxt[k & 1];
-        float (*FY)[TYR][64] = fyt[k & 1];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) FX[q][ty][R + tx] = pfx[q];
-        if (xh_on) FX[xh_q][xh_r][xh_tc] = pxh;
-        FY[0][R + ty][tx] = pfx[1];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) FY[q][R + ty][tx] = pfy[q - 1];
-#pragma unroll
-        for (int n = 0; n < NYH; ++n)
-            if (yh_row[n] >= 0)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) FY[q][yh_row[n]][tx] = pyh[n][q];
-        float an[5], du0[5];
-        double u0[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            an[m] = pa[m];
-            u0[m] = pu[m];
-            du0[m] = MODE >= 2 ? pdu[m] : 0.0f;
-        }
-        __syncthreads();
-        if (k + 1 < k1) prefetch(k + 1);
-        zstep(k, an);
-        double r[5];
-        r[0] = -(double)an[0];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            float wx[2 * R + 1], wy[2 * R + 1];
-#pragma unroll
-            for (int o = -R; o <= R; ++o) {
-                wx[o + R] = FX[m][ty][R + tx + o];
-                wy[o + R] = FY[m][R + ty + o][tx];
-            }
-            const float vd = diff32<VIS>(wx + R, P.dsc[0]) + diff32<VIS>(wy + R, P.dsc[1]) +
-                             diff32<VIS>(&win[m][R], P.dsc[2]);
-            r[m + 1] = (double)vd - (double)an[m + 1];
-        }
-        float sh[6];
-        vdiv_finish<MODE, true>(P, Mo, id, r, u0, du0, dt, flag, cfl_next, cx, cy, cz, sh);
-        const int im[3] = {i, j, k};
-        const int nn[3] = {P.nx, P.ny, P.nz};
-        const int str[3] = {1, sx, sxy};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (im[a] < 2 * ng) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) So.S[q][id + nn[a] * str[a]] = sh[q];
-            }
-            if (im[a] >= nn[a]) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) So.S[q][id - nn[a] * str[a]] = sh[q];
-            }
-        }
-    }
-    if (cfl_next) {
-        const double cm = wave_min(cfl_bound(P, cx, cy, cz));
-        if (__lane_id() == 0) atomicMin(flag + FLAG_CFL_NEXT, (unsigned long long)__double_as_longlong(cm));
-    }
+    static_assert(MODE != 0, "the folded z sweep belongs to a stage");
+    vdiv_lds_pass<VIS, MODE, true, true, RECON, FVS>(P, M, So, dt, seg_len, gx, gy, flag, dev_dt);
 }
 
 // ------------------------------------------------------ state conversion
