@@ -847,7 +847,7 @@ __global__ __launch_bounds__(256) void ka_vflux(DevParams P, Fields F, AdjArrays
 {
     __shared__ double red[4];
     int bx, by, bz;
-    xcd_block3(bx, by, bz);
+    xcd_block3_band(bx, by, bz);
     const int i = P.ng + bx * blockDim.x + threadIdx.x;
     const int j = P.ng + by * blockDim.y + threadIdx.y;
     const int k = P.ng + bz;
@@ -966,7 +966,7 @@ template <int VIS, int BC>
 __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays A, Combine cb)
 {
     int bx, by, bz;
-    xcd_block3(bx, by, bz);
+    xcd_block3_band(bx, by, bz);
     const int i = P.ng + bx * blockDim.x + threadIdx.x;
     const int j = P.ng + by * blockDim.y + threadIdx.y;
     const int k = P.ng + bz;
@@ -1027,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_dissip(DevParams P, Fields F, double *p
 {
     __shared__ double red[4];
     int bx, by, bz;
-    xcd_block3(bx, by, bz);
+    xcd_block3_band(bx, by, bz);
     const int i = P.ng + bx * blockDim.x + threadIdx.x;
     const int j = P.ng + by * blockDim.y + threadIdx.y;
     const int k = P.ng + bz;

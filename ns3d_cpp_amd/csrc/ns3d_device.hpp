@@ -204,6 +204,30 @@ __device__ __forceinline__ void xcd_block3(int &bx, int &by, int &bz)
     bz = l / (gx * gy);
 }
 
+// For kernels whose blocks each take one tile of one plane and read R planes
+// above and below it (the non-marching 3D stencils): logical blocks go band
+// by band (about 16 tiles of a plane: band = 16 / gx tile rows), within a
+// band plane by plane.  The blocks an XCD runs at once are then a few
+// consecutive planes of one band, whose z stencil (~3 MB for 8 FP64 arrays
+// and R = 4) stays in its L2; plane by plane over whole planes, as
+// xcd_block3 orders them, the 2R + 1 planes of a 128^2 plane exceed it.
+// Falls back to xcd_block3's order where the band does not divide gy.
+__device__ __forceinline__ void xcd_block3_band(int &bx, int &by, int &bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const int nwg = gx * gy * gz;
+    const int wg = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int l = xcd_logical(wg, nwg);
+    int band = 16 / gx;
+    if (band < 1) band = 1;
+    if (gy % band) band = gy;
+    const int per_band = gx * band * gz;
+    const int b = l / per_band, r = l % per_band;
+    bx = r % gx;
+    by = b * band + (r / gx) % band;
+    bz = r / (gx * band);
+}
+
 // 1/x: the hardware estimate refined by two Newton-Raphson steps
 // r <- r + r (1 - x r) (quadratic convergence; x > 0 and normal where used).
 // For the mixed-precision and derivative kernels, which need the value and

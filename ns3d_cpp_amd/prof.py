@@ -158,6 +158,8 @@ def main():
     ap.add_argument("--stage-bytes", default="", help="algorithmic bytes of the stage kernel's stages 1,2,3")
     ap.add_argument("--zs-stage-bytes", default="",
                     help="the same for the stage kernel with the z sweep folded in (vdiv_zs_stage)")
+    ap.add_argument("--scale", default="",
+                    help="FETCH,WRITE scale factors for runs without the copy kernel (from a run with it)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
     a = ap.parse_args()
@@ -187,12 +189,17 @@ def main():
         w = load_counter(a.write, "WRITE_SIZE")
         copy = [k for k in f if kind_of(k) == "copy"]
         copy_kib = 12 * ncell * 8 / 1024.0
-        fs = copy_kib / f[copy[0]] if copy else 1.0
-        ws = copy_kib / w[copy[0]] if copy else 1.0
-        lines += ["", f"HBM traffic per dispatch from FETCH_SIZE and WRITE_SIZE (separate runs). Calibration: the copy "
-                      f"kernel `k_copy` moves {12 * ncell * 8 / 1e9:.3f} GB each way per launch; FETCH_SIZE reads "
-                      f"{1 / fs:.3f} of that (scale x{fs:.3f}), WRITE_SIZE {1 / ws:.3f} (scale x{ws:.3f}). "
-                      "Each kernel is compared with its own algorithmic bytes (the stage kernel stage by stage).", "",
+        if copy:
+            fs, ws = copy_kib / f[copy[0]], copy_kib / w[copy[0]]
+            cal = (f"Calibration: the copy kernel `k_copy` moves {12 * ncell * 8 / 1e9:.3f} GB each way per launch; "
+                   f"FETCH_SIZE reads {1 / fs:.3f} of that (scale x{fs:.3f}), WRITE_SIZE {1 / ws:.3f} "
+                   f"(scale x{ws:.3f}).")
+        else:
+            fs, ws = [float(x) for x in a.scale.split(",")] if a.scale else (1.0, 1.0)
+            cal = (f"Calibration: this run has no copy kernel; FETCH_SIZE is scaled x{fs:.3f} and WRITE_SIZE "
+                   f"x{ws:.3f}, what the copy kernel of the same counters gave in the runs that have it.")
+        lines += ["", "HBM traffic per dispatch from FETCH_SIZE and WRITE_SIZE (separate runs). " + cal +
+                  " Each kernel is compared with its own algorithmic bytes (the stage kernel stage by stage).", "",
                   "| kernel | kind | FETCH KiB raw | WRITE KiB raw | traffic GB | algorithmic GB | ratio |",
                   "|---|---|---|---|---|---|---|"]
         per_kind = collections.defaultdict(list)
