@@ -959,6 +959,36 @@ struct Combine {
     int init, add_acc;
 };
 
+// The end of the gradient adjoint at one cell: LW = Y plus the gradient
+// terms lq of u, v, w, T, then C2P'^T and the RK3 combination into C and Z.
+// wv = rho, u, v, w, T; yv, xv, cv = Y, X, C at the cell (cv = 0 with init)
+__device__ __forceinline__ void grad_finish(const DevParams &P, const AdjArrays &A, const Combine &cb, int id,
+                                            const double lq[4], const double wv[5], const double yv[5],
+                                            const double xv[5], const double cv[5])
+{
+    const double rho = wv[0], u = wv[1], v = wv[2], w = wv[3], T = wv[4];
+    const double irho = 1.0 / rho;
+    const double br = yv[0] - lq[3] * T * irho;
+    const double bu = yv[1] + lq[0];
+    const double bv = yv[2] + lq[1];
+    const double bw = yv[3] + lq[2];
+    const double bp = yv[4] + lq[3] * irho / P.Rgas;
+    const double g1 = P.gamma - 1.0;
+    double lu[5];
+    lu[0] = br - (u * bu + v * bv + w * bw) * irho + g1 * 0.5 * (u * u + v * v + w * w) * bp;
+    lu[1] = bu * irho - g1 * u * bp;
+    lu[2] = bv * irho - g1 * v * bp;
+    lu[3] = bw * irho - g1 * w * bp;
+    lu[4] = g1 * bp;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double lo = xv[m];
+        const double acc = cv[m] + cb.a * lo;
+        A.C[m][id] = acc;
+        A.Z[m][id] = cb.bx * lo + cb.bl * lu[m] + (cb.add_acc ? acc : 0.0);
+    }
+}
+
 // Gradient adjoint LW += D^T LG (= -D LG on periodic arrays, the transposed
 // closure rows on non-periodic axes with BC) for u, v, w, T, which completes
 // LW at the cell, then C2P'^T and the RK3 combination.
@@ -984,26 +1014,139 @@ __global__ __launch_bounds__(256) void ka_grad(DevParams P, Fields F, AdjArrays 
                                       : -diff_central<VIS>(A.G[3 * q + d] + id, strd[d], h[d]);
         lq[q] = s;
     }
-    const double rho = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], T = F.W[5][id];
-    const double irho = 1.0 / rho;
-    const double br = A.Y[0][id] - lq[3] * T * irho;
-    const double bu = A.Y[1][id] + lq[0];
-    const double bv = A.Y[2][id] + lq[1];
-    const double bw = A.Y[3][id] + lq[2];
-    const double bp = A.Y[4][id] + lq[3] * irho / P.Rgas;
-    const double g1 = P.gamma - 1.0;
-    double lu[5];
-    lu[0] = br - (u * bu + v * bv + w * bw) * irho + g1 * 0.5 * (u * u + v * v + w * w) * bp;
-    lu[1] = bu * irho - g1 * u * bp;
-    lu[2] = bv * irho - g1 * v * bp;
-    lu[3] = bw * irho - g1 * w * bp;
-    lu[4] = g1 * bp;
+    double wv[5], yv[5], xv[5], cv[5];
+    const double *wsrc[5] = {F.W[0], F.W[1], F.W[2], F.W[3], F.W[5]};
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double lo = A.X[m][id];
-        const double acc = (cb.init ? 0.0 : A.C[m][id]) + cb.a * lo;
-        A.C[m][id] = acc;
-        A.Z[m][id] = cb.bx * lo + cb.bl * lu[m] + (cb.add_acc ? acc : 0.0);
+        wv[m] = wsrc[m][id];
+        yv[m] = A.Y[m][id];
+        xv[m] = A.X[m][id];
+        cv[m] = cb.init ? 0.0 : A.C[m][id];
+    }
+    grad_finish(P, A, cb, id, lq, wv, yv, xv, cv);
+}
+
+// Gradient adjoint as a z march on a periodic box that 64 x 4 blocks tile
+// exactly: each block marches a z segment of its 64 x 4 column; the four
+// x-differentiated arrays of each plane are staged in LDS with an R-cell x
+// halo, the four y-differentiated ones with an R-row y halo (both from the
+// ghost layers, double-buffered, one barrier per plane), and the four
+// z-differentiated ones slide through a register window.  Everything the
+// next plane needs is loaded before this plane's arithmetic, so the loads
+// overlap it: about 41 loads per cell instead of 108 (ka_grad is bound by
+// its load instructions).  The same values in the same formulas as
+// ka_grad<VIS, 0>: equal bit for bit.
+template <int VIS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void ka_grad_z(
+    DevParams P, Fields F, AdjArrays A, Combine cb, int seg_len, int gx, int gy)
+{
+    constexpr int R = VisReach<VIS>::R;
+    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
+    __shared__ double xt[2][4][4][TC];
+    __shared__ double yt[2][4][TYR][64];
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int ng = P.ng;
+    const int i0 = ng + bx * 64, j0 = ng + by * 4;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, ng + P.nz);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    // the x-halo cell of this thread (tid < 32 R): array tid / 8R, tile row
+    // (tid / 2R) % 4, halo column tid % 2R
+    const bool xh_on = tid < 32 * R;
+    const int xh_q = xh_on ? tid / (8 * R) : 0;
+    const int xh_r = (tid / (2 * R)) % 4;
+    const int xh_c = tid % (2 * R);
+    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
+    const int xh_off = (j0 + xh_r) * sx + i0 - R + xh_tc;
+    // the y-halo rows of this thread: h = ty, ty + 4, ... < 2R, tile row h
+    // (below the tile) or h + 4 (above it)
+    constexpr int NYH = (2 * R + 3) / 4;
+    int yh_row[NYH], yh_off[NYH];
+#pragma unroll
+    for (int n = 0; n < NYH; ++n) {
+        const int h = ty + 4 * n;
+        const int tr = h < R ? h : h + 4;
+        yh_row[n] = h < 2 * R ? tr : -1;
+        yh_off[n] = (j0 - R + tr) * sx + i;
+    }
+    const double *wsrc[5] = {F.W[0], F.W[1], F.W[2], F.W[3], F.W[5]};
+    double pxo[4], pyo[4], pxh = 0.0, pyh[NYH][4], pz[4], pw[5], py[5], px[5], pc[5];
+    const auto prefetch = [&](int kp) {
+        const int pl = kp * sxy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pxo[q] = A.G[3 * q][pl + col];
+            pyo[q] = A.G[3 * q + 1][pl + col];
+        }
+        if (xh_on) pxh = A.G[3 * xh_q][pl + xh_off];
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pyh[n][q] = A.G[3 * q + 1][pl + yh_off[n]];
+        const int zn = (kp + R) * sxy + col;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pz[q] = A.G[3 * q + 2][zn];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            pw[m] = wsrc[m][pl + col];
+            py[m] = A.Y[m][pl + col];
+            px[m] = A.X[m][pl + col];
+            pc[m] = cb.init ? 0.0 : A.C[m][pl + col];
+        }
+    };
+    double win[4][2 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = A.G[3 * q + 2][(k0 - R - 1 + o) * sxy + col];
+    prefetch(k0);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[q][o] = win[q][o + 1];
+            win[q][2 * R] = pz[q];
+        }
+        double (*XT)[4][TC] = xt[k & 1];
+        double (*YT)[TYR][64] = yt[k & 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            XT[q][ty][R + tx] = pxo[q];
+            YT[q][R + ty][tx] = pyo[q];
+        }
+        if (xh_on) XT[xh_q][xh_r][xh_tc] = pxh;
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) YT[q][yh_row[n]][tx] = pyh[n][q];
+        double wv[5], yv[5], xv[5], cv[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { wv[m] = pw[m]; yv[m] = py[m]; xv[m] = px[m]; cv[m] = pc[m]; }
+        __syncthreads();
+        if (k + 1 < k1) prefetch(k + 1);
+        double lq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double vx[2 * R + 1], vy[2 * R + 1];
+#pragma unroll
+            for (int o = -R; o <= R; ++o) {
+                vx[o + R] = XT[q][ty][R + tx + o];
+                vy[o + R] = YT[q][R + ty + o][tx];
+            }
+            double s = 0.0;
+            s += -diff_window<VIS>(vx + R, P.dx);
+            s += -diff_window<VIS>(vy + R, P.dy);
+            s += -diff_window<VIS>(&win[q][R], P.dz);
+            lq[q] = s;
+        }
+        grad_finish(P, A, cb, id, lq, wv, yv, xv, cv);
     }
 }
 
@@ -1095,6 +1238,15 @@ bool ad_supported(const DevParams &P)
 
 // some axis is not periodic: the BC forms of the viscous kernels
 static bool any_bc(const DevParams &P) { return !(P.periodic[0] && P.periodic[1] && P.periodic[2]); }
+
+// the z-marching forms of the viscous adjoint kernels: a periodic box that
+// 64 x 4 blocks tile exactly (every thread reaches the barriers); z
+// segments of kAdjSegZ planes
+constexpr int kAdjSegZ = 32;
+static bool adj_zmarch(const DevParams &P)
+{
+    return !any_bc(P) && P.adj_zmarch && P.nx % 64 == 0 && P.ny % 4 == 0;
+}
 
 template <int RECON, int FVS, int CHAR>
 static void tlm_faces_c(hipStream_t s, const DevParams &P, const Fields &F, const AdjArrays &A, int dir)
@@ -1259,8 +1411,15 @@ static void adj_grad(hipStream_t s, const DevParams &P, const Fields &F, const A
     for (int m = 0; m < 12; ++m) gs.p[m] = A.G[m];
     gs.n = 12;
     fill_set(s, P, gs);
-    if (any_bc(P)) hipLaunchKernelGGL((ka_grad<VIS, 1>), agrid(P), kABlock, 0, s, P, F, A, cb);
-    else hipLaunchKernelGGL((ka_grad<VIS, 0>), agrid(P), kABlock, 0, s, P, F, A, cb);
+    if (any_bc(P)) {
+        hipLaunchKernelGGL((ka_grad<VIS, 1>), agrid(P), kABlock, 0, s, P, F, A, cb);
+    } else if (adj_zmarch(P)) {
+        const int seg = P.nz < kAdjSegZ ? P.nz : kAdjSegZ;
+        const int gx = P.nx / 64, gy = P.ny / 4, n = gx * gy * ((P.nz + seg - 1) / seg);
+        hipLaunchKernelGGL((ka_grad_z<VIS>), dim3(n), kABlock, 0, s, P, F, A, cb, seg, gx, gy);
+    } else {
+        hipLaunchKernelGGL((ka_grad<VIS, 0>), agrid(P), kABlock, 0, s, P, F, A, cb);
+    }
 }
 
 template <int RECON, int FVS, int CHAR>

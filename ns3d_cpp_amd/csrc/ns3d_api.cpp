@@ -918,6 +918,8 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     // (they serve lines too long for LDS)
     d.adj_lines = 1;
     if (const char *env = std::getenv("NS3D_ADJ_MARCH")) d.adj_lines = std::atoi(env) ? 0 : 1;
+    d.adj_zmarch = 1;
+    if (const char *env = std::getenv("NS3D_ADJ_ZMARCH")) d.adj_zmarch = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
     if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
     if (const char *env = std::getenv("NS3D_M_GRADFLUX_BLOCK")) {   // "BXxBY"

@@ -150,6 +150,35 @@ def test_dot_product_marching_sweeps(gpu, cfg, monkeypatch):
     assert abs(th - th_line) <= 1e-12 * abs(th_line)
 
 
+@pytest.mark.parametrize("vis", ["C8th", "C6th", "C4th"])
+def test_z_marching_viscous_adjoint_is_bitwise(gpu, vis, monkeypatch):
+    """The viscous adjoint kernels as z marches with LDS-staged planes (boxes
+    the 64 x 4 blocks tile) against the cached-load forms: the adjoint RHS,
+    the 1/Re term and a two-step reverse pass bit for bit, on a 64 x 16 x 40
+    box (two z segments, the second shorter)."""
+    nx, ny, nz = 64, 16, 40
+    rng = np.random.default_rng(41)
+    st, Ma = fields.hit(16, k0=2, Mt=0.3, seed=9)
+    shape = (nz, ny, nx)
+    prims = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
+    y = [rng.normal(size=shape) for _ in range(5)]
+    out = []
+    for zm in ("1", "0"):
+        monkeypatch.setenv("NS3D_ADJ_ZMARCH", zm)
+        kw = dict(BASE, recon="WENO5", fvs_type="StagerWarming", vis_scheme=vis)
+        p, g = ns.make_params((nx, ny, nz), Ma=Ma, **kw)
+        s = ns.Solver(p, g)
+        s.upload_primitive(*prims)
+        r, th = s.adj_rhs([s._full(a) for a in y], 0.4)
+        J = s.adj_forward(2, 1e-3, 1)
+        lam0, dth = s.adj_backward()
+        out.append(([s.interior(a).copy() for a in r + lam0], (th, J, dth)))
+        s.close()
+    assert out[0][1] == out[1][1]
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+
+
 def _fd_rhs(n, prims, Ma, cfg, dU, h):
     s = solver(n, prims, Ma, **cfg)
     U = [s._full(a) for a in conserved_of(prims)]
