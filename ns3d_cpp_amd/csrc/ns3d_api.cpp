@@ -1697,28 +1697,38 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
         HIPCALL(c, hipEventRecord(c->ev_start, c->stream));
         HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_start, 0));
     }
-    for (int seg = last; seg >= 0; --seg) {
-        const int b = overlap ? (last - seg) & 1 : 0;
-        if (overlap && seg > 0) {
-            const int bn = b ^ 1;
-            if (seg < last) HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_free[bn], 0));
-            const hipStream_t main_stream = c->stream;
-            c->stream = c->stream2;
-            st = recompute(seg - 1, bn);
-            c->stream = main_stream;
-            if (st) return st;
-            HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream2));
-        }
-        if (seg < last) {
-            if (overlap) HIPCALL(c, hipStreamWaitEvent(c->stream, c->ev_ready[b], 0));
-            else {
-                st = recompute(seg, 0);
-                if (st) return st;
+    auto segments = [&]() -> int {
+        for (int seg = last; seg >= 0; --seg) {
+            const int b = overlap ? (last - seg) & 1 : 0;
+            if (overlap && seg > 0) {
+                const int bn = b ^ 1;
+                if (seg < last) HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_free[bn], 0));
+                const hipStream_t main_stream = c->stream;
+                c->stream = c->stream2;
+                const int r = recompute(seg - 1, bn);
+                c->stream = main_stream;
+                if (r) return r;
+                HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream2));
             }
+            if (seg < last) {
+                if (overlap) HIPCALL(c, hipStreamWaitEvent(c->stream, c->ev_ready[b], 0));
+                else {
+                    const int r = recompute(seg, 0);
+                    if (r) return r;
+                }
+            }
+            const int r = reverse(seg, b);
+            if (r) return r;
+            if (overlap) HIPCALL(c, hipEventRecord(c->ev_free[b], c->stream));
         }
-        st = reverse(seg, b);
-        if (st) return st;
-        if (overlap) HIPCALL(c, hipEventRecord(c->ev_free[b], c->stream));
+        return NS3D_OK;
+    };
+    st = segments();
+    if (st) {
+        // a failed pass may leave a recompute running on the second stream:
+        // finish it before the caller touches the forward arrays again
+        if (overlap) (void)hipStreamSynchronize(c->stream2);
+        return st;
     }
     // leave the context at the final state of the forward pass
     st = load_state(c, ck_state(c, c->ck_states - 1));
