@@ -397,6 +397,97 @@ __device__ __forceinline__ double recon_weno5(const double *s)
     return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
+// ------------------------------------ quotients with a refined reciprocal
+// hipcc takes an FP64 quotient a / b on gfx950 as
+//   d = div_scale(b, b, a), n = div_scale(a, b, a), r = rcp(d),
+//   r = fma(r, fma(-d, r, 1), r) twice, q = n r, e = fma(-d, q, n),
+//   a / b = div_fixup(div_fmas(e, r, q), b, a).
+// For finite operands away from the ends of the exponent range (a above
+// 2^-969, a / b normal and below 2^768 b, b below 2^1022) div_scale returns
+// its operand, div_fmas is fma(e, r, q) and div_fixup returns its first
+// operand, so quot_rcp(a, b, rcp_refined(b)) is the same bits as a / b, and
+// one refined reciprocal serves every quotient with that denominator.  (A
+// numerator of -0 gives +0 here: equal as a number, the sign bit differs.)
+
+__device__ __forceinline__ double rcp_refined(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    return r;
+}
+
+__device__ __forceinline__ double quot_rcp(double a, double b, double r)
+{
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+}
+
+// The refined reciprocal of 6 as the device computes it for x / 6.0 (the
+// empty asm keeps the compiler from folding rcp of a constant into a value
+// the hardware's rcp need not give).
+__device__ __forceinline__ double rcp_refined_six()
+{
+    double six = 6.0;
+    asm volatile("" : "+v"(six));
+    return rcp_refined(six);
+}
+
+// The range of split fluxes under which recon_weno5_rcp equals recon_weno5
+// bit for bit: zero or 2^-900 <= |f| <= 2^200.  Then each smoothness
+// indicator is below 2^406, (eps + beta)^2 lies in [2^-40, 2^812], each
+// alpha in [2^-816, 2^40], each weight in [2^-857, 1], and a nonzero
+// numerator of the 1/6 quotients, an integer combination of stencil values
+// that are multiples of 2^-952, is at least 2^-952 and below 2^205: every
+// quotient inside the range where the steps above equal a / b.  Non-finite
+// values fall outside.
+__device__ __forceinline__ bool weno_rcp_range_ok(double f)
+{
+    const double a = fabs(f);
+    return a <= 0x1p200 && (a >= 0x1p-900 || a == 0.0);
+}
+
+__device__ __forceinline__ bool weno_rcp_range_ok5(const double f[5])
+{
+    bool ok = true;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) ok = ok && weno_rcp_range_ok(f[m]);
+    return ok;
+}
+
+// recon_weno5 (reconstruction.cpp:93-124) with its nine quotients taken by
+// the steps above: three reciprocals for the alphas, one shared by the
+// three weights, and r6 (rcp_refined_six) for the 1/6 of the candidate
+// polynomials; 47 instead of 99 VALU operations.  The same expressions in
+// the same order otherwise.  Only for stencils that weno_rcp_range_ok
+// admits.
+__device__ __forceinline__ double recon_weno5_rcp(const double *s, double r6)
+{
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    double eps = 1e-6;
+    double beta0 = 13.0 / 12.0 * (f0 - 2 * f1 + f2) * (f0 - 2 * f1 + f2)
+                 + 0.25 * (f0 - 4 * f1 + 3 * f2) * (f0 - 4 * f1 + 3 * f2);
+    double beta1 = 13.0 / 12.0 * (f1 - 2 * f2 + f3) * (f1 - 2 * f2 + f3)
+                 + 0.25 * (f1 - f3) * (f1 - f3);
+    double beta2 = 13.0 / 12.0 * (f2 - 2 * f3 + f4) * (f2 - 2 * f3 + f4)
+                 + 0.25 * (3 * f2 - 4 * f3 + f4) * (3 * f2 - 4 * f3 + f4);
+    double d0 = (eps + beta0) * (eps + beta0);
+    double d1 = (eps + beta1) * (eps + beta1);
+    double d2 = (eps + beta2) * (eps + beta2);
+    double alpha0 = quot_rcp(0.1, d0, rcp_refined(d0));
+    double alpha1 = quot_rcp(0.6, d1, rcp_refined(d1));
+    double alpha2 = quot_rcp(0.3, d2, rcp_refined(d2));
+    double asum = alpha0 + alpha1 + alpha2;
+    const double ra = rcp_refined(asum);
+    double w0 = quot_rcp(alpha0, asum, ra);
+    double w1 = quot_rcp(alpha1, asum, ra);
+    double w2 = quot_rcp(alpha2, asum, ra);
+    double p0 = quot_rcp(2 * f0 - 7 * f1 + 11 * f2, 6.0, r6);
+    double p1 = quot_rcp(-f1 + 5 * f2 + 2 * f3, 6.0, r6);
+    double p2 = quot_rcp(2 * f2 + 5 * f3 - f4, 6.0, r6);
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
 __device__ __forceinline__ double recon_mdcd(const double *s, double disp, double diss)
 {   // reconstruction.cpp:126-135
     return (1.0 / 2.0 * disp + 1.0 / 2.0 * diss) * s[0] +

@@ -6,6 +6,8 @@
 //   U0[5]                 conserved state at the start of the RK3 step
 //   A[5]                  inviscid accumulator S = (ax + ay) + az, then RHS
 //   G[12], VF[12]         velocity/temperature gradients, viscous fluxes
+#include <type_traits>
+
 #include "ns3d_device.hpp"
 #include "ns3d_internal.hpp"
 
@@ -115,6 +117,24 @@ __global__ void k_fill_bc(DevParams P, Fields F, int face, int kind)
 }
 
 // --------------------------------------------------------- fused sweeps
+
+// One face's flux of one component: the plus and minus reconstructions
+// added (InviscousFlux_new.cpp:4-163 through reconstruct_select).  RCP:
+// WENO5 by recon_weno5_rcp, bit for bit recon_weno5 on the range of split
+// fluxes that weno_rcp_range_ok admits.
+template <int RECON, bool RCP>
+__device__ __forceinline__ double face_flux(const DevParams &P, const double *sp, const double *sm, double r6)
+{
+    if constexpr (RCP) {
+        double plus_face = recon_weno5_rcp(sp, r6);
+        double minus_face = recon_weno5_rcp(sm, r6);
+        return plus_face + minus_face;
+    } else {
+        double plus_face = recon<RECON>(P, sp);
+        double minus_face = recon<RECON>(P, sm);
+        return plus_face + minus_face;
+    }
+}
 //
 // One thread marches one column of cells along the sweep axis DIR over a
 // segment [c0, c1), keeping the split fluxes of the S cells of the current
@@ -152,11 +172,18 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
     const double *__restrict__ ww = F.W[3];
     const double *__restrict__ pp = F.W[4];
 
+    // WENO5 faces by recon_weno5_rcp while every split flux that has entered
+    // the windows of the wave's columns lies in its range (sticky: the
+    // literal form from the first flux outside it on)
+    constexpr bool WRCP = RECON == R_WENO5;
+    const double r6 = WRCP ? rcp_refined_six() : 0.0;
+    bool rng_ok = true;
     double fp[S][5], fm[S][5];
 #pragma unroll
     for (int s = 1; s < S; ++s) {
         int id = base + (c0 - 2 - MID + s) * st;
         split_state<FVS, DIR>(P, rho[id], uu[id], vv[id], ww[id], pp[id], fp[s], fm[s]);
+        if constexpr (WRCP) rng_ok = rng_ok && weno_rcp_range_ok5(fp[s]) && weno_rcp_range_ok5(fm[s]);
     }
     double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // the primitives of the cell entering the window are loaded one face
@@ -173,39 +200,40 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
 #pragma unroll
         for (int m = 0; m < 5; ++m) an[m] = F.A[m][base + c0 * st];
     }
-    for (int f = c0 - 1; f < c1; ++f) {
+    double acur[5];
+    // window and loads for face f: the cell entering the window is split
+    const auto advance = [&](int f) {
 #pragma unroll
         for (int s = 0; s < S - 1; ++s)
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fp[s][m] = fp[s + 1][m]; fm[s][m] = fm[s + 1][m]; }
-        double acur[5];
         if constexpr (ACC == 1) {
 #pragma unroll
             for (int m = 0; m < 5; ++m) acur[m] = an[m];
         }
-        {
-            double cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
-            if (f + 1 < c1) {
-                int id = base + (f + 1 - MID + S - 1) * st;
-                nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
-                if constexpr (ACC == 1) {
-                    if (f >= c0) {
+        double cur[5] = {nx_[0], nx_[1], nx_[2], nx_[3], nx_[4]};
+        if (f + 1 < c1) {
+            int id = base + (f + 1 - MID + S - 1) * st;
+            nx_[0] = rho[id]; nx_[1] = uu[id]; nx_[2] = vv[id]; nx_[3] = ww[id]; nx_[4] = pp[id];
+            if constexpr (ACC == 1) {
+                if (f >= c0) {
 #pragma unroll
-                        for (int m = 0; m < 5; ++m) an[m] = F.A[m][base + (f + 1) * st];
-                    }
+                    for (int m = 0; m < 5; ++m) an[m] = F.A[m][base + (f + 1) * st];
                 }
             }
-            split_state<FVS, DIR>(P, cur[0], cur[1], cur[2], cur[3], cur[4], fp[S - 1], fm[S - 1]);
         }
+        split_state<FVS, DIR>(P, cur[0], cur[1], cur[2], cur[3], cur[4], fp[S - 1], fm[S - 1]);
+        if constexpr (WRCP) rng_ok = rng_ok && weno_rcp_range_ok5(fp[S - 1]) && weno_rcp_range_ok5(fm[S - 1]);
+    };
+    // face f's fluxes, and for f >= c0 the cell's difference stored
+    const auto emit = [&](int f, auto rcp) {
         double ff[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double sp[S], sm[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) { sp[s] = fp[s][m]; sm[s] = fm[S - 1 - s][m]; }
-            double plus_face = recon<RECON>(P, sp);
-            double minus_face = recon<RECON>(P, sm);
-            ff[m] = plus_face + minus_face;
+            ff[m] = face_flux<RECON, decltype(rcp)::value>(P, sp, sm, r6);
         }
         if (f >= c0) {
             int idc = base + f * st;
@@ -218,6 +246,22 @@ __global__ __launch_bounds__(64, MINW) void k_sweep(DevParams P, Fields F, int s
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) fprev[m] = ff[m];
+    };
+    int f = c0 - 1;
+    if constexpr (WRCP) {
+        // the reciprocal form while the whole wave is in range (a uniform
+        // exit), then the literal form from the first face that is not: two
+        // loops, so the two forms never share a body's registers
+        for (; f < c1; ++f) {
+            advance(f);
+            if (!__all(rng_ok)) break;
+            emit(f, std::true_type{});
+        }
+        if (f < c1) emit(f++, std::false_type{});
+    }
+    for (; f < c1; ++f) {
+        advance(f);
+        emit(f, std::false_type{});
     }
 }
 
@@ -244,16 +288,25 @@ __global__ __launch_bounds__(1024) void k_sweep_xrow(DevParams P, Fields F)
     const int k = P.ng + blockIdx.y;
     const int row = (k * P.sy + j) * P.sx;
     const int i0 = P.ng - MID;              // cell of slot 0
+    // WENO5 faces by recon_weno5_rcp when every split flux of the row lies
+    // in its range, else the literal form for the whole row
+    constexpr bool WRCP = RECON == R_WENO5;
+    bool rng_ok = true;
     for (int q = t; q < NSLOT; q += blockDim.x) {
         const int id = row + i0 + q;
         double Fp[5], Fm[5];
         split_state<FVS, 0>(P, F.W[0][id], F.W[1][id], F.W[2][id], F.W[3][id], F.W[4][id], Fp, Fm);
+        if constexpr (WRCP) rng_ok = rng_ok && weno_rcp_range_ok5(Fp) && weno_rcp_range_ok5(Fm);
 #pragma unroll
         for (int m = 0; m < 5; ++m) { fpl[m * NSLOT + q] = Fp[m]; fml[m * NSLOT + q] = Fm[m]; }
     }
-    __syncthreads();
-    if (t < TX) {
+    bool row_rcp = false;
+    if constexpr (WRCP) row_rcp = __syncthreads_and(rng_ok) != 0;
+    else __syncthreads();
+    const auto faces = [&](auto rcp) {
         // face ng+t (right face of cell ng+t): stencil cells ng+t-MID .. +S-1
+        double r6 = 0.0;
+        if constexpr (decltype(rcp)::value) r6 = rcp_refined_six();
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double sp[S], sm[S];
@@ -262,9 +315,15 @@ __global__ __launch_bounds__(1024) void k_sweep_xrow(DevParams P, Fields F)
                 sp[s2] = fpl[m * NSLOT + t + s2];
                 sm[S - 1 - s2] = fml[m * NSLOT + t + s2];
             }
-            double plus_face = recon<RECON>(P, sp);
-            double minus_face = recon<RECON>(P, sm);
-            ffl[m * TX + t] = plus_face + minus_face;
+            ffl[m * TX + t] = face_flux<RECON, decltype(rcp)::value>(P, sp, sm, r6);
+        }
+    };
+    if (t < TX) {
+        if constexpr (WRCP) {
+            if (row_rcp) faces(std::true_type{});
+            else faces(std::false_type{});
+        } else {
+            faces(std::false_type{});
         }
     }
     __syncthreads();

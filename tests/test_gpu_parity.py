@@ -408,3 +408,70 @@ def test_set_precision_round_trip(gpu):
     for k in ("rho", "u", "v", "w", "p", "T"):
         assert np.array_equal(a.interior(pa[k]), b.interior(pb[k])), k
     a.rk3_step(2e-3)        # the FP64 path runs on the converted state
+
+
+def rough_box(n, seed):
+    """rough_field on an (nx, ny, nz) box (an int: the cube)."""
+    if isinstance(n, int):
+        return rough_field(n, seed)
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    Z, Y, X = np.meshgrid(*(2 * np.pi * np.arange(k) / k for k in (nz, ny, nx)), indexing="ij")
+    out = []
+    for m, base in enumerate((1.0, 0.0, 0.0, 0.0, 1.0 / (1.4 * 0.09))):
+        f = np.zeros_like(X)
+        for _ in range(3):
+            k = rng.integers(-2, 3, 3)
+            f += np.sin(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 6.3))
+        f += 0.3 * rng.standard_normal(X.shape)
+        out.append(base + (0.1 * base if m in (0, 4) else 0.5) * f)
+    return out
+
+
+def blob_field(n, seed, scale):
+    """rough_box with rho and p multiplied by `scale` inside a ball (same
+    temperature and sound speed): split fluxes there leave the range in
+    which the sweeps take WENO5's quotients by refined reciprocals, so the
+    columns and rows through the ball switch to the literal quotients."""
+    st = rough_box(n, seed)
+    nz, ny, nx = (n, n, n) if isinstance(n, int) else n[::-1]
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ball = (x - 0.3 * nx) ** 2 + (y - 0.6 * ny) ** 2 + (z - 0.5 * nz) ** 2 < (0.2 * min(nx, ny, nz)) ** 2
+    for m in (0, 4):
+        st[m] = np.where(ball, st[m] * scale, st[m])
+    return st
+
+
+RCP_CASES = [
+    ("in-range", 32, None, dict(fvs_type="StagerWarming")),
+    ("in-range", 32, None, dict(fvs_type="StagerWarming", sw_eps=0.1)),
+    ("in-range", 32, None, dict(fvs_type="LaxFriedrichs")),
+    ("in-range", (40, 24, 20), None, dict(fvs_type="StagerWarming")),
+    ("in-range-walls", 32, None, dict(fvs_type="StagerWarming",
+                                      bc=["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])),
+    ("large-ball", 32, 1e62, dict(fvs_type="StagerWarming")),
+    ("large-ball", (40, 24, 20), 1e62, dict(fvs_type="LaxFriedrichs")),
+    ("tiny-ball", 32, 1e-285, dict(fvs_type="StagerWarming")),
+    ("large-ball-walls", 32, 1e62, dict(fvs_type="StagerWarming",
+                                        bc=["Wall", "Wall", "Periodic", "Periodic", "Periodic", "Periodic"])),
+]
+
+
+@pytest.mark.parametrize("case,n,scale,cfg", RCP_CASES,
+                         ids=[f"{c[0]}-{i}" for i, c in enumerate(RCP_CASES)])
+def test_weno5_sweeps_rhs_bitwise(gpu, case, n, scale, cfg):
+    """The FP64 sweeps take WENO5's quotients (reconstruction.cpp:106-116)
+    with one refined reciprocal per denominator where every split flux of a
+    wave's stencils is zero or within [2^-900, 2^200], and by the literal
+    quotients elsewhere.  Both must give the oracle's right-hand side bit
+    for bit (constant mu, so no pow() in the comparison): in range, and
+    with a ball of 1e62 or 1e-285 times the density and pressure that sends
+    the columns and rows through it to the literal form part way along."""
+    st = rough_box(n, 11) if scale is None else blob_field(n, 11, scale)
+    s, o = pair(n, st, recon="WENO5", vis_scheme="C8th", mu_law="constant", **cfg)
+    gr = s.compute_rhs()
+    orr = o.compute_rhs()
+    for m in range(5):
+        a, b = s.interior(gr[m]), o.interior(orr[m])
+        assert np.all(np.isfinite(b)), m
+        assert np.array_equal(a, b), (m, relmax(a, b), int(np.sum(a != b)))
