@@ -88,7 +88,7 @@ hipError_t launch_fill_ghosts(hipStream_t s, const DevParams &P, const Fields &F
 hipError_t launch_sweep(hipStream_t s, const DevParams &P, const Fields &F, int dir, int seg_len, int minw);
 bool xrow_supported(const DevParams &P);
 hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F);
-hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb);
+hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb, int lds);
 hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
                          unsigned long long *flag, int dev_dt);
 hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &out,

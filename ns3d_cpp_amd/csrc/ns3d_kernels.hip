@@ -348,6 +348,38 @@ __device__ __forceinline__ void gradxy(const DevParams &P, const double *const s
     }
 }
 
+// The viscous fluxes of cell id from its gradients g (du/dx, du/dy, du/dz,
+// dv/dx, ..., dT/dz), velocity and temperature (ViscousFlux.cpp:282-339)
+__device__ __forceinline__ void vflux_store(const DevParams &P, const Fields &F, int id, const double g[12],
+                                            double u, double v, double w, double T)
+{
+    const double Pr = P.Pr, Cp = P.Cp;
+    double mu = get_mu(P, T);
+    double du_dx = g[0], du_dy = g[1], du_dz = g[2];
+    double dv_dx = g[3], dv_dy = g[4], dv_dz = g[5];
+    double dw_dx = g[6], dw_dy = g[7], dw_dz = g[8];
+    double dT_dx = g[9], dT_dy = g[10], dT_dz = g[11];
+    double divU = du_dx + dv_dy + dw_dz;
+    double tau_xx = 2.0 * mu * du_dx - (2.0 / 3.0) * mu * divU;
+    double tau_yy = 2.0 * mu * dv_dy - (2.0 / 3.0) * mu * divU;
+    double tau_zz = 2.0 * mu * dw_dz - (2.0 / 3.0) * mu * divU;
+    double tau_xy = mu * (du_dy + dv_dx);
+    double tau_xz = mu * (du_dz + dw_dx);
+    double tau_yz = mu * (dv_dz + dw_dy);
+    double qx = mu * Cp / Pr * dT_dx;
+    double qy = mu * Cp / Pr * dT_dy;
+    double qz = mu * Cp / Pr * dT_dz;
+    F.VF[0][id] = tau_xx;
+    F.VF[1][id] = tau_xy;
+    F.VF[2][id] = tau_xz;
+    F.VF[3][id] = u * tau_xx + v * tau_xy + w * tau_xz + qx;
+    F.VF[5][id] = tau_yy;
+    F.VF[6][id] = tau_yz;
+    F.VF[7][id] = u * tau_xy + v * tau_yy + w * tau_yz + qy;
+    F.VF[10][id] = tau_zz;
+    F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
+}
+
 // Gradients (ViscousFlux.cpp:240-279) fused with the viscous flux
 // (ViscousFlux.cpp:282-339): one thread per (i, j) column marching along z
 // with the R planes above and below held in registers, so the z stencil
@@ -386,7 +418,6 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
 #pragma unroll
     for (int s = 1; s <= R; ++s)
         if (k0 + s - 1 < k1) gradxy<VIS>(P, src, (k0 + s - 1) * sxy + col, dxl, dxh, dyl, dyh, gxy[s]);
-    const double Pr = P.Pr, Cp = P.Cp;
     for (int k = k0; k < k1; ++k) {
         const int id = k * sxy + col;
         if constexpr (BC) axis_dist(P.periodic[2], k, P.ng, P.nz, R, dzl, dzh);
@@ -411,31 +442,123 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
             g[3 * q + 1] = gxy[0][2 * q + 1];
             g[3 * q + 2] = diff_window_bc<VIS>(&win[q][R], P.dz, dzl, dzh);
         }
-        double mu = get_mu(P, win[3][R]);
-        double du_dx = g[0], du_dy = g[1], du_dz = g[2];
-        double dv_dx = g[3], dv_dy = g[4], dv_dz = g[5];
-        double dw_dx = g[6], dw_dy = g[7], dw_dz = g[8];
-        double dT_dx = g[9], dT_dy = g[10], dT_dz = g[11];
-        double divU = du_dx + dv_dy + dw_dz;
-        double tau_xx = 2.0 * mu * du_dx - (2.0 / 3.0) * mu * divU;
-        double tau_yy = 2.0 * mu * dv_dy - (2.0 / 3.0) * mu * divU;
-        double tau_zz = 2.0 * mu * dw_dz - (2.0 / 3.0) * mu * divU;
-        double tau_xy = mu * (du_dy + dv_dx);
-        double tau_xz = mu * (du_dz + dw_dx);
-        double tau_yz = mu * (dv_dz + dw_dy);
-        double qx = mu * Cp / Pr * dT_dx;
-        double qy = mu * Cp / Pr * dT_dy;
-        double qz = mu * Cp / Pr * dT_dz;
-        double u = win[0][R], v = win[1][R], w = win[2][R];
-        F.VF[0][id] = tau_xx;
-        F.VF[1][id] = tau_xy;
-        F.VF[2][id] = tau_xz;
-        F.VF[3][id] = u * tau_xx + v * tau_xy + w * tau_xz + qx;
-        F.VF[5][id] = tau_yy;
-        F.VF[6][id] = tau_yz;
-        F.VF[7][id] = u * tau_xy + v * tau_yy + w * tau_yz + qy;
-        F.VF[10][id] = tau_zz;
-        F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
+        vflux_store(P, F, id, g, win[0][R], win[1][R], win[2][R], win[3][R]);
+    }
+}
+
+// The gradient/flux pass on a box periodic in x and y with each entering
+// plane's x / y neighbourhood staged in LDS: a 64 x 4 workgroup writes its
+// plane tile with an R-cell halo in x (own rows) and y (own columns; the
+// halo cells are the primitive arrays' ghost layers) into one of two
+// buffers, one barrier per plane, and takes the x / y derivatives from
+// there by diff_central on the tile.  The own values are the z window's
+// entering plane; the halo costs up to 2 + 1 loads per thread and quantity
+// instead of the 4R stencil loads of k_gradflux's cached form.  The next
+// plane's values are loaded before this plane's derivative and flux
+// arithmetic.  The same arithmetic in the same order as k_gradflux<VIS, *,
+// 0>: equal bit for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread
+// leaves before a barrier).
+template <int VIS>
+__global__ __launch_bounds__(256) void k_gradflux_lds(DevParams P, Fields F, int seg_len, int gx, int gy)
+{
+    constexpr int R = VisReach<VIS>::R;
+    constexpr int TR = 4 + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
+    __shared__ double tile[2][4][TR][TC];
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int i0 = P.ng + bx * 64, j0 = P.ng + by * 4;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = P.ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, P.ng + P.nz);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    const double *const src[4] = {F.W[1], F.W[2], F.W[3], F.W[5]};
+    // halo cells this thread stages: y-halo rows h = ty, ty + 4, ... < 2R
+    // (tile row h, or h + 4 above the tile), and one x-halo cell (tid < 32 R:
+    // quantity tid / 8R, own row (tid / 2R) % 4, halo column tid % 2R)
+    constexpr int NYH = (2 * R + 3) / 4;
+    int yh_row[NYH], yh_off[NYH];
+#pragma unroll
+    for (int n = 0; n < NYH; ++n) {
+        const int h = ty + 4 * n;
+        const int tr = h < R ? h : h + 4;
+        yh_row[n] = h < 2 * R ? tr : -1;
+        yh_off[n] = (j0 - R + tr) * sx + i;
+    }
+    const bool xh_on = tid < 32 * R;
+    const int xh_q = xh_on ? tid / (8 * R) : 0;
+    const int xh_r = (tid / (2 * R)) % 4;
+    const int xh_c = tid % (2 * R);
+    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
+    const int xh_off = (j0 + xh_r) * sx + i0 - R + xh_tc;
+    double win[4][2 * R + 1];
+    double gxy[R + 1][8];     // slot s: x / y derivatives of plane k + s
+    double nyh[NYH][4], nxh = 0.0, nown[4];
+    const auto prefetch = [&](int p, bool halo) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nown[q] = src[q][p * sxy + col];
+        if (halo) {
+#pragma unroll
+            for (int n = 0; n < NYH; ++n)
+                if (yh_row[n] >= 0)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) nyh[n][q] = src[q][p * sxy + yh_off[n]];
+            if (xh_on) nxh = src[xh_q][p * sxy + xh_off];
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[q][o] = src[q][(k0 - R - 1 + o) * sxy + col];
+#pragma unroll
+    for (int s = 1; s <= R; ++s)
+        if (k0 + s - 1 < k1) gradxy<VIS>(P, src, (k0 + s - 1) * sxy + col, R, R, R, R, gxy[s]);
+    prefetch(k0 + R, k0 + R < k1);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+        const int p = k + R;   // the entering plane
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) gxy[s][q] = gxy[s + 1][q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[q][o] = win[q][o + 1];
+            win[q][2 * R] = nown[q];
+        }
+        const bool stage = p < k1;   // uniform over the workgroup
+        if (stage) {
+            double (*T)[TR][TC] = tile[p & 1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) T[q][R + ty][R + tx] = nown[q];
+#pragma unroll
+            for (int n = 0; n < NYH; ++n)
+                if (yh_row[n] >= 0)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) T[q][yh_row[n]][R + tx] = nyh[n][q];
+            if (xh_on) T[xh_q][R + xh_r][xh_tc] = nxh;
+            __syncthreads();
+        }
+        if (p + 1 < k1 + R) prefetch(p + 1, p + 1 < k1);
+        if (stage) {
+            double (*T)[TR][TC] = tile[p & 1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double *c = &T[q][R + ty][R + tx];
+                gxy[R][2 * q + 0] = diff_central<VIS>(c, 1, P.dx);
+                gxy[R][2 * q + 1] = diff_central<VIS>(c, TC, P.dy);
+            }
+        }
+        double g[12];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            g[3 * q + 0] = gxy[0][2 * q];
+            g[3 * q + 1] = gxy[0][2 * q + 1];
+            g[3 * q + 2] = diff_window<VIS>(&win[q][R], P.dz);
+        }
+        vflux_store(P, F, id, g, win[0][R], win[1][R], win[2][R], win[3][R]);
     }
 }
 
@@ -1063,9 +1186,22 @@ static hipError_t gradflux_vis(hipStream_t s, const DevParams &P, const Fields &
 }
 
 // the closure-aware variant only where an axis is not periodic
-hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb)
+// the LDS-staged form where it applies (lds != 0; periodic box, nx % 64 == 0,
+// ny % 4 == 0), else k_gradflux with its waves-per-EU floor minb
+hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb, int lds)
 {
     const bool bc = !(P.periodic[0] && P.periodic[1] && P.periodic[2]);
+    if (lds && !bc && P.nx % 64 == 0 && P.ny % 4 == 0) {
+        const int gx = P.nx / 64, gy = P.ny / 4, gz = (P.nz + seg_len - 1) / seg_len;
+        const dim3 b(64, 4);
+        switch (P.vis) {
+        case VIS_C8: hipLaunchKernelGGL((k_gradflux_lds<VIS_C8>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
+        case VIS_C6: hipLaunchKernelGGL((k_gradflux_lds<VIS_C6>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
+        case VIS_C4: hipLaunchKernelGGL((k_gradflux_lds<VIS_C4>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (minb == 2) return bc ? gradflux_vis<2, 1>(s, P, F, seg_len) : gradflux_vis<2, 0>(s, P, F, seg_len);
     return bc ? gradflux_vis<1, 1>(s, P, F, seg_len) : gradflux_vis<1, 0>(s, P, F, seg_len);
 }
