@@ -79,6 +79,7 @@ struct ns3d_ctx {
     int seg_z = 32;                          // z segments of the z-marching kernels
     int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
     int gf64_lds = 1;                        // FP64 gradient/flux pass LDS-staged where it applies
+    int vd64_lds = 1;                        // FP64 divergence/stage pass LDS-staged where it applies
     int sweep_minw = 0;                      // waves-per-EU floor of k_sweep (0: per scheme)
     int m_gf_block[2] = {64, 4};             // workgroup shape of km_gradflux
     int m_vd_block[2] = {64, 4};             // workgroup shape of km_vdiv_z
@@ -339,7 +340,7 @@ int rhs_pass(ns3d_ctx *c, int mode, double dt, int dev_dt = 0)
     if (st) return st;
     TIMED(c, "gradflux", launch_gradflux(c->stream, c->dp, c->f, c->seg_z, c->gradflux_minb, c->gf64_lds));
     TIMED(c, mode == 0 ? "vdiv_rhs" : "vdiv_stage",
-          launch_vdiv_z(c->stream, c->dp, c->f, mode, dt, c->seg_z, c->d_flag, dev_dt));
+          launch_vdiv_z(c->stream, c->dp, c->f, mode, dt, c->seg_z, c->d_flag, dev_dt, c->vd64_lds));
     return NS3D_OK;
 }
 
@@ -923,6 +924,7 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     if (const char *env = std::getenv("NS3D_ADJ_ZMARCH")) d.adj_zmarch = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
     if (const char *env = std::getenv("NS3D_GF64_LDS")) c->gf64_lds = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NS3D_VD64_LDS")) c->vd64_lds = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
     if (const char *env = std::getenv("NS3D_M_GRADFLUX_BLOCK")) {   // "BXxBY"
         int bx = 0, by = 0;
@@ -1241,7 +1243,7 @@ int ns3d_compute_vis_flux(ns3d_ctx *c)
     // rhs = 0, then the viscous divergence of the stored fluxes
     for (int m = 0; m < 5; ++m)   // one fill per array: each holds < 2^31 cells
         HIPCALL(c, launch_fill_u64(c->stream, (unsigned long long *)c->f.A[m], 0ull, (int)c->ncell));
-    HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0));
+    HIPCALL(c, launch_vdiv_z(c->stream, c->dp, c->f, 0, 0.0, c->seg_z, c->d_flag, 0, c->vd64_lds));
     return stream_sync(c);
 }
 

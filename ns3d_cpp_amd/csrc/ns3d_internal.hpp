@@ -90,7 +90,7 @@ bool xrow_supported(const DevParams &P);
 hipError_t launch_sweep_xrow(hipStream_t s, const DevParams &P, const Fields &F);
 hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb, int lds);
 hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                         unsigned long long *flag, int dev_dt);
+                         unsigned long long *flag, int dev_dt, int lds);
 hipError_t launch_faces(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &out,
                         int dir, int char_recon);
 hipError_t launch_face_diff(hipStream_t s, const DevParams &P, const Fields &F, const FaceArrays &ff);

@@ -767,6 +767,70 @@ __global__ __launch_bounds__(256) void k_vflux(DevParams P, Fields F)
     F.VF[11][id] = u * tau_xz + v * tau_yz + w * tau_zz + qz;
 }
 
+// The end of a divergence/stage pass at cell id: r[1..4] holds the viscous
+// divergence (r[0] = 0) and the RHS is r - a, a the inviscid accumulator
+// (range_kutta.cpp:11-99).  MODE 0 stores the RHS in A; MODE 1..3 is the
+// RK3 stage (range_kutta.cpp:101-161) from the cell's primitives w (rho, u,
+// v, w, p; P2C as field_structures.h:430-451) and, for stages 2 and 3, the
+// step's start state u0: the new primitives (C2P) and T are stored, and the
+// non-physical scan of the stages in P.check_mask reports the cell.
+template <int MODE>
+__device__ __forceinline__ void vdiv_finish64(const DevParams &P, const Fields &F, int id, double r[5],
+                                              const double a[5], const double w5[5], const double u0[5], double dt,
+                                              unsigned long long *flag)
+{
+#pragma unroll
+    for (int m = 0; m < 5; ++m) r[m] += -(a[m]);
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) F.A[m][id] = r[m];
+    } else {
+        const double gamma = P.gamma;
+        double rr = w5[0], u = w5[1], v = w5[2], w = w5[3], p = w5[4];
+        double U[5];
+        U[0] = rr;
+        U[1] = rr * u;
+        U[2] = rr * v;
+        U[3] = rr * w;
+        {
+            double kinetic = 0.5 * (u * u + v * v + w * w);
+            double e_internal = p / ((gamma - 1.0) * rr);
+            U[4] = rr * (e_internal + kinetic);
+        }
+        double Un[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if constexpr (MODE == 1) {
+                F.U0[m][id] = U[m];
+                Un[m] = U[m] + dt * r[m];
+            } else if constexpr (MODE == 2) {
+                Un[m] = 0.75 * u0[m] + 0.25 * (U[m] + dt * r[m]);
+            } else {
+                Un[m] = (1.0 / 3.0) * u0[m] + (2.0 / 3.0) * (U[m] + dt * r[m]);
+            }
+        }
+        double rn = Un[0];
+        if (rn <= 0.0) rn = 1e-12;
+        double un = Un[1] / rn;
+        double vn = Un[2] / rn;
+        double wn = Un[3] / rn;
+        double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
+        double e_internal = Un[4] / rn - kinetic;
+        double pn = (gamma - 1.0) * rn * e_internal;
+        double Tn = pn / (rn * P.Rgas);
+        F.W[0][id] = rn; F.W[1][id] = un; F.W[2][id] = vn; F.W[3][id] = wn; F.W[4][id] = pn;
+        F.W[5][id] = Tn;
+        // the scan is compiled into every stage and enabled by a runtime
+        // mask (the third stage's, range_kutta.cpp:150-161): written for
+        // the third stage only, the compiler scheduled the second
+        // stage's loads in small groups (1.68 ms against 1.18 ms at 256^3)
+        if ((P.check_mask >> MODE) & 1) {
+            if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
+                atomicMin(flag, nonphysical_key(flag, id));
+        }
+    }
+}
+
 // compute_vis_flux (ViscousFlux.cpp:341-386) + the RHS combine of
 // range_kutta.cpp:61-97, and for MODE 1..3 the RK3 stage update of
 // range_kutta.cpp:109-147 with the C2P of field_structures.h:405-427.
@@ -833,56 +897,143 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
             acc += diff_window_bc<VIS>(&win[m][R], P.dz, dzl, dzh);
             r[m + 1] = acc;
         }
+        double a[5], w5[5], u0[5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) r[m] += -(F.A[m][id]);
-        if constexpr (MODE == 0) {
-#pragma unroll
-            for (int m = 0; m < 5; ++m) F.A[m][id] = r[m];
-        } else {
-            const double gamma = P.gamma;
-            double rr = F.W[0][id], u = F.W[1][id], v = F.W[2][id], w = F.W[3][id], p = F.W[4][id];
-            double U[5];
-            U[0] = rr;
-            U[1] = rr * u;
-            U[2] = rr * v;
-            U[3] = rr * w;
-            {
-                double kinetic = 0.5 * (u * u + v * v + w * w);
-                double e_internal = p / ((gamma - 1.0) * rr);
-                U[4] = rr * (e_internal + kinetic);
-            }
-            double Un[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                if constexpr (MODE == 1) {
-                    F.U0[m][id] = U[m];
-                    Un[m] = U[m] + dt * r[m];
-                } else if constexpr (MODE == 2) {
-                    Un[m] = 0.75 * F.U0[m][id] + 0.25 * (U[m] + dt * r[m]);
-                } else {
-                    Un[m] = (1.0 / 3.0) * F.U0[m][id] + (2.0 / 3.0) * (U[m] + dt * r[m]);
-                }
-            }
-            double rn = Un[0];
-            if (rn <= 0.0) rn = 1e-12;
-            double un = Un[1] / rn;
-            double vn = Un[2] / rn;
-            double wn = Un[3] / rn;
-            double kinetic = 0.5 * (un * un + vn * vn + wn * wn);
-            double e_internal = Un[4] / rn - kinetic;
-            double pn = (gamma - 1.0) * rn * e_internal;
-            double Tn = pn / (rn * P.Rgas);
-            F.W[0][id] = rn; F.W[1][id] = un; F.W[2][id] = vn; F.W[3][id] = wn; F.W[4][id] = pn;
-            F.W[5][id] = Tn;
-            // the scan is compiled into every stage and enabled by a runtime
-            // mask (the third stage's, range_kutta.cpp:150-161): written for
-            // the third stage only, the compiler scheduled the second
-            // stage's loads in small groups (1.68 ms against 1.18 ms at 256^3)
-            if ((P.check_mask >> MODE) & 1) {
-                if (!(rn > 0.0) || !(pn > 0.0) || !(Tn > 0.0))
-                    atomicMin(flag, nonphysical_key(flag, id));
-            }
+        for (int m = 0; m < 5; ++m) {
+            a[m] = F.A[m][id];
+            if constexpr (MODE != 0) w5[m] = F.W[m][id];
+            if constexpr (MODE >= 2) u0[m] = F.U0[m][id];
         }
+        vdiv_finish64<MODE>(P, F, id, r, a, w5, u0, dt, flag);
+    }
+}
+
+// The divergence/stage pass on a periodic box with each plane's x / y
+// viscous fluxes staged in LDS: a 64 x 4 workgroup writes its plane's x
+// fluxes with an R-cell x halo and its y fluxes with an R-row y halo
+// (periodic images: the flux arrays hold no ghosts) into one of two
+// buffers, one barrier per plane, and takes both derivatives from there.
+// Everything the next plane needs (own fluxes, halos, the entering z-flux
+// plane, A, the primitives, U0) is loaded before this plane's arithmetic, so
+// a plane's loads overlap the previous plane's arithmetic.  The same
+// arithmetic in the same order as k_vdiv_z<VIS, MODE, 0>: equal bit for
+// bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before a
+// barrier).
+template <int VIS, int MODE>
+__global__ __launch_bounds__(256) void k_vdiv_lds(DevParams P, Fields F, double dt, int seg_len, int gx, int gy,
+                                                  unsigned long long *flag, int dev_dt)
+{
+    constexpr int R = VisReach<VIS>::R;
+    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
+    __shared__ double fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
+    __shared__ double fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
+    const int ng = P.ng;
+    const int t = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int i0 = ng + bx * 64, j0 = ng + by * 4;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = ng + bz * seg_len;
+    const int k1 = min(k0 + seg_len, ng + P.nz);
+    if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
+    const int sx = P.sx, sxy = P.sx * P.sy;
+    const int col = j * sx + i;
+    // the x-halo cell of this thread (tid < 32 R): flux tid / 8R, tile row
+    // (tid / 2R) % 4, halo column tid % 2R
+    const bool xh_on = tid < 32 * R;
+    const int xh_q = xh_on ? tid / (8 * R) : 0;
+    const int xh_r = (tid / (2 * R)) % 4;
+    const int xh_c = tid % (2 * R);
+    const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
+    const int xh_off = (j0 + xh_r) * sx + wrapi(i0 - R + xh_tc, ng, P.nx);
+    // the y-halo rows of this thread: h = ty, ty + 4, ... < 2R, tile row h
+    // (below the tile) or h + 4 (above it)
+    constexpr int NYH = (2 * R + 3) / 4;
+    int yh_row[NYH], yh_off[NYH];
+#pragma unroll
+    for (int n = 0; n < NYH; ++n) {
+        const int h = ty + 4 * n;
+        const int tr = h < R ? h : h + 4;
+        yh_row[n] = h < 2 * R ? tr : -1;
+        yh_off[n] = wrapi(j0 - R + tr, ng, P.ny) * sx + i;
+    }
+    double pfx[4], pfy[4], pxh = 0.0, pyh[NYH][4], pz[4], pa[5], pw[5], pu0[5];
+    const auto prefetch = [&](int kp) {
+        const int pl = kp * sxy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pfx[q] = F.VF[q][pl + col];
+            pfy[q] = F.VF[4 + q][pl + col];
+        }
+        if (xh_on) pxh = F.VF[xh_q][pl + xh_off];
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pyh[n][q] = F.VF[4 + q][pl + yh_off[n]];
+        const int zn = wrapi(kp + R, ng, P.nz) * sxy + col;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pz[m] = F.VF[8 + m][zn];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            pa[m] = F.A[m][pl + col];
+            if constexpr (MODE != 0) pw[m] = F.W[m][pl + col];
+            if constexpr (MODE >= 2) pu0[m] = F.U0[m][pl + col];
+        }
+    };
+    double win[4][2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int o = 1; o < 2 * R + 1; ++o) win[m][o] = F.VF[8 + m][wrapi(k0 - R - 1 + o, ng, P.nz) * sxy + col];
+    prefetch(k0);
+    for (int k = k0; k < k1; ++k) {
+        const int id = k * sxy + col;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
+            win[m][2 * R] = pz[m];
+        }
+        double (*FX)[4][TC] = fxt[k & 1];
+        double (*FY)[TYR][64] = fyt[k & 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            FX[q][ty][R + tx] = pfx[q];
+            FY[q][R + ty][tx] = pfy[q];
+        }
+        if (xh_on) FX[xh_q][xh_r][xh_tc] = pxh;
+#pragma unroll
+        for (int n = 0; n < NYH; ++n)
+            if (yh_row[n] >= 0)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) FY[q][yh_row[n]][tx] = pyh[n][q];
+        double a[5], w5[5], u0[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            a[m] = pa[m];
+            w5[m] = MODE != 0 ? pw[m] : 0.0;
+            u0[m] = MODE >= 2 ? pu0[m] : 0.0;
+        }
+        __syncthreads();
+        if (k + 1 < k1) prefetch(k + 1);
+        double r[5];
+        r[0] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            double wx[2 * R + 1], wy[2 * R + 1];
+#pragma unroll
+            for (int o = -R; o <= R; ++o) {
+                wx[o + R] = FX[m][ty][R + tx + o];
+                wy[o + R] = FY[m][R + ty + o][tx];
+            }
+            double acc = 0.0;
+            acc += diff_window<VIS>(wx + R, P.dx);
+            acc += diff_window<VIS>(wy + R, P.dy);
+            acc += diff_window<VIS>(&win[m][R], P.dz);
+            r[m + 1] = acc;
+        }
+        vdiv_finish64<MODE>(P, F, id, r, a, w5, u0, dt, flag);
     }
 }
 
@@ -1233,10 +1384,36 @@ static hipError_t vdivz_vis(hipStream_t s, const DevParams &P, const Fields &F, 
 }
 
 // the closure-aware variant only where an axis is not periodic
-hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
-                         unsigned long long *flag, int dev_dt)
+template <int VIS>
+static void vdiv_lds_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                          unsigned long long *flag, int dev_dt)
 {
-    if (P.periodic[0] && P.periodic[1] && P.periodic[2]) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, dev_dt);
+    const int gx = P.nx / 64, gy = P.ny / 4, gz = (P.nz + seg - 1) / seg;
+    const dim3 g(gx * gy * gz), b(64, 4);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_vdiv_lds<VIS, 0>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((k_vdiv_lds<VIS, 1>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((k_vdiv_lds<VIS, 2>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((k_vdiv_lds<VIS, 3>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    }
+}
+
+// the LDS-staged form where it applies (lds != 0; periodic box, nx % 64 == 0,
+// ny % 4 == 0), else k_vdiv_z
+hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                         unsigned long long *flag, int dev_dt, int lds)
+{
+    const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
+    if (lds && per && P.nx % 64 == 0 && P.ny % 4 == 0) {
+        switch (P.vis) {
+        case VIS_C8: vdiv_lds_mode<VIS_C8>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C6: vdiv_lds_mode<VIS_C6>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+        case VIS_C4: vdiv_lds_mode<VIS_C4>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    if (per) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, dev_dt);
     return vdivz_vis<1>(s, P, F, mode, dt, seg, flag, dev_dt);
 }
 

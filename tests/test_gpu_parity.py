@@ -478,35 +478,42 @@ def test_weno5_sweeps_rhs_bitwise(gpu, case, n, scale, cfg):
 
 
 @pytest.mark.parametrize("vis", ["C8th", "C6th", "C4th"])
-def test_fp64_gradflux_forms_bitwise(gpu, vis, monkeypatch):
-    """The FP64 gradient/flux pass on periodic boxes with nx % 64 == 0 and
-    ny % 4 == 0 stages each entering plane's x / y neighbourhood in LDS
-    (NS3D_GF64_LDS=1, the default) instead of loading the stencils through
-    the caches (0).  Both evaluate ViscousFlux.cpp:240-339 in the same order
-    on the same values, so the RHS, two fixed-dt RK3 steps and three CFL
-    steps must agree bit for bit.  A 64 x 32 x 24 box
-    (a different spacing per axis) in 2 z segments of unequal length."""
+def test_fp64_viscous_pass_forms_bitwise(gpu, vis, monkeypatch):
+    """The FP64 viscous passes on periodic boxes with nx % 64 == 0 and
+    ny % 4 == 0 stage each plane's x / y neighbourhood in LDS: the gradient/
+    flux pass (NS3D_GF64_LDS=1, the default; 0 loads the stencils through
+    the caches) and the divergence/stage pass (NS3D_VD64_LDS, likewise).
+    All four combinations evaluate ViscousFlux.cpp:240-386 and
+    range_kutta.cpp:11-161 in the same order on the same values, so the RHS,
+    the divergence of stored viscous fluxes, two fixed-dt RK3 steps and
+    three CFL steps must agree bit for bit.  A 64 x 32 x 24 box (a different
+    spacing per axis) in 2 z segments of unequal length."""
     nx, ny, nz = 64, 32, 24
     rng = np.random.default_rng(5)
     st, Ma = fields.hit(nz, k0=2, Mt=0.3, seed=13)
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    for form in ("0", "1"):
-        monkeypatch.setenv("NS3D_GF64_LDS", form)
+    for form in (("0", "0"), ("1", "0"), ("0", "1"), ("1", "1")):
+        monkeypatch.setenv("NS3D_GF64_LDS", form[0])
+        monkeypatch.setenv("NS3D_VD64_LDS", form[1])
         monkeypatch.setenv("NS3D_SEG_Z", "16")
         p, g = ns.make_params((nx, ny, nz), L=(2 * np.pi), Ma=Ma, Re=100.0, recon="WENO5",
                               vis_scheme=vis, Ly=np.pi, Lz=0.75 * np.pi)
         s = ns.Solver(p, g)
         s.upload_primitive(*x)
         r = [s.interior(a).copy() for a in s.compute_rhs()]
+        s.compute_viscous_flux()
+        dv = [s.interior(a).copy() for a in s.compute_vis_flux()]
         s.rk3_step(1e-3)
         s.rk3_step(1e-3)
         u2 = [s.interior(a).copy() for a in s.conserved()]
         s.advance_cfl(3)
         u5 = [s.interior(a).copy() for a in s.conserved()]
-        out[form] = (r, u2, u5)
+        out[form] = (r, dv, u2, u5)
         s.close()
-    for part in range(3):
-        for m, (a, b) in enumerate(zip(out["0"][part], out["1"][part])):
-            assert np.array_equal(a, b), (part, m, relmax(a, b))
+    ref = out[("0", "0")]
+    for form, got in out.items():
+        for part in range(4):
+            for m, (a, b) in enumerate(zip(got[part], ref[part])):
+                assert np.array_equal(a, b), (form, part, m, relmax(a, b))
