@@ -78,8 +78,8 @@ struct ns3d_ctx {
     int seg_len[3] = {64, 64, 64};
     int seg_z = 32;                          // z segments of the z-marching kernels
     int gradflux_minb = 1;                   // waves-per-EU floor of k_gradflux
-    int gf64_lds = 1;                        // FP64 gradient/flux pass LDS-staged where it applies
-    int vd64_lds = 1;                        // FP64 divergence/stage pass LDS-staged where it applies
+    int gf64_lds = 8;                        // FP64 gradient/flux pass LDS-staged (tile rows; 0: cached)
+    int vd64_lds = 8;                        // FP64 divergence/stage pass LDS-staged (tile rows; 0: cached)
     int sweep_minw = 0;                      // waves-per-EU floor of k_sweep (0: per scheme)
     int m_gf_block[2] = {64, 4};             // workgroup shape of km_gradflux
     int m_vd_block[2] = {64, 4};             // workgroup shape of km_vdiv_z
@@ -114,6 +114,7 @@ struct ns3d_ctx {
     double fw_dt = 0.0;
     bool fw_valid = false;          // a forward pass matches the current state
     bool fw_last_valid = false;     // stage inputs of the last segment are stored
+    bool fw_prev_valid = false;     // and of the one before it, in the second buffer
     // monitors, allocated on first use
     hipfftHandle fft = 0;
     bool fft_ready = false;
@@ -923,8 +924,14 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     d.adj_zmarch = 1;
     if (const char *env = std::getenv("NS3D_ADJ_ZMARCH")) d.adj_zmarch = std::atoi(env) != 0;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
-    if (const char *env = std::getenv("NS3D_GF64_LDS")) c->gf64_lds = std::atoi(env) != 0;
-    if (const char *env = std::getenv("NS3D_VD64_LDS")) c->vd64_lds = std::atoi(env) != 0;
+    // NS3D_GF64_LDS / NS3D_VD64_LDS = 0 selects the cached forms;
+    // NS3D_FP64_TILE_Y = 4 or 8 the LDS forms' tile height (8: 5-9 % faster
+    // at 256^3, half the y-halo rows per row)
+    int fp64_ty = 8;
+    if (const char *env = std::getenv("NS3D_FP64_TILE_Y")) fp64_ty = std::atoi(env) == 8 ? 8 : 4;
+    c->gf64_lds = c->vd64_lds = fp64_ty;
+    if (const char *env = std::getenv("NS3D_GF64_LDS")) c->gf64_lds = std::atoi(env) ? fp64_ty : 0;
+    if (const char *env = std::getenv("NS3D_VD64_LDS")) c->vd64_lds = std::atoi(env) ? fp64_ty : 0;
     if (const char *env = std::getenv("NS3D_SWEEP_MINW")) c->sweep_minw = std::atoi(env) == 3 ? 3 : 2;
     if (const char *env = std::getenv("NS3D_M_GRADFLUX_BLOCK")) {   // "BXxBY"
         int bx = 0, by = 0;
@@ -1588,10 +1595,13 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     if (st) return st;
     // forward: J terms into adj_sums[3*nsteps + n]
     double *jsum = c->adj_sums + 3LL * nsteps;
-    // the last segment's stage inputs are kept from this pass, so the
-    // reverse pass starts without recomputing it
+    // the last segment's stage inputs are kept from this pass in the first
+    // snapshot buffer, and with two buffers the segment before it in the
+    // second (where the overlapped reverse pass would recompute it), so the
+    // reverse pass starts without recomputing either
     const long long snap0 = ncp;
     const int last0 = (int)((ncp - 1) * ckpt_every);
+    const int prev0 = nbuf == 2 ? last0 - ckpt_every : last0;
     for (int n = 0; n < nsteps; ++n) {
         if (n % ckpt_every == 0) {
             st = save_state(c, ck_state(c, n / ckpt_every));
@@ -1602,9 +1612,11 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
         if (st) return st;
         TIMED(c, "dissipation", launch_dissip(c->stream, c->dp, F, c->adj_part));
         TIMED(c, "adj_sum", launch_sum_partials(c->stream, c->adj_part, ad_blocks(c->dp), jsum + n));
-        if (n >= last0) {
+        if (n >= prev0) {
+            const long long slot = n >= last0 ? snap0 + 3LL * (n - last0)
+                                              : snap0 + 3LL * ckpt_every + 3LL * (n - prev0);
             for (int stage = 1; stage <= 3; ++stage) {
-                st = snapshot_stage(c, ck_state(c, snap0 + 3LL * (n - last0) + stage - 1), stage);
+                st = snapshot_stage(c, ck_state(c, slot + stage - 1), stage);
                 if (!st) st = stage_enqueue(c, stage, dt);
                 if (st) return st;
             }
@@ -1630,6 +1642,7 @@ int ns3d_adj_forward(ns3d_ctx *c, int nsteps, double dt, int ckpt_every, double 
     c->fw_dt = dt;
     c->fw_valid = true;
     c->fw_last_valid = true;
+    c->fw_prev_valid = nbuf == 2;
     return NS3D_OK;
 }
 
@@ -1693,8 +1706,11 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
     // recompute of segment seg - 1 runs on the second stream, into the
     // buffer the reverse of seg + 1 has released, while the main stream
     // reverses seg.  The recompute touches only the forward arrays, the
-    // reversal only the snapshots it reads and the adjoint arrays.
+    // reversal only the snapshots it reads and the adjoint arrays.  The
+    // forward pass left segment last - 1 in buffer 1 already (fw_prev_valid).
     const bool overlap = c->ck_bufs == 2 && !c->prof && last > 0;
+    const bool prev_kept = overlap && c->fw_prev_valid;
+    c->fw_prev_valid = false;
     if (overlap) {
         st = ensure_stream2(c);
         if (st) return st;
@@ -1706,13 +1722,18 @@ int ns3d_adj_backward(ns3d_ctx *c, const double *const lamN[5], double *const la
             const int b = overlap ? (last - seg) & 1 : 0;
             if (overlap && seg > 0) {
                 const int bn = b ^ 1;
-                if (seg < last) HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_free[bn], 0));
-                const hipStream_t main_stream = c->stream;
-                c->stream = c->stream2;
-                const int r = recompute(seg - 1, bn);
-                c->stream = main_stream;
-                if (r) return r;
-                HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream2));
+                if (seg == last && prev_kept) {
+                    // segment last - 1 is in buffer 1 from the forward pass
+                    HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream));
+                } else {
+                    if (seg < last) HIPCALL(c, hipStreamWaitEvent(c->stream2, c->ev_free[bn], 0));
+                    const hipStream_t main_stream = c->stream;
+                    c->stream = c->stream2;
+                    const int r = recompute(seg - 1, bn);
+                    c->stream = main_stream;
+                    if (r) return r;
+                    HIPCALL(c, hipEventRecord(c->ev_ready[bn], c->stream2));
+                }
             }
             if (seg < last) {
                 if (overlap) HIPCALL(c, hipStreamWaitEvent(c->stream, c->ev_ready[b], 0));

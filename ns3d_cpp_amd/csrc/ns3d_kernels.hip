@@ -458,37 +458,37 @@ __global__ __launch_bounds__(256, MINB) void k_gradflux(DevParams P, Fields F, i
 // arithmetic.  The same arithmetic in the same order as k_gradflux<VIS, *,
 // 0>: equal bit for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread
 // leaves before a barrier).
-template <int VIS>
-__global__ __launch_bounds__(256) void k_gradflux_lds(DevParams P, Fields F, int seg_len, int gx, int gy)
+template <int VIS, int TY>
+__global__ __launch_bounds__(64 * TY) void k_gradflux_lds(DevParams P, Fields F, int seg_len, int gx, int gy)
 {
     constexpr int R = VisReach<VIS>::R;
-    constexpr int TR = 4 + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
+    constexpr int TR = TY + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
     __shared__ double tile[2][4][TR][TC];
     const int t = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int i0 = P.ng + bx * 64, j0 = P.ng + by * 4;
+    const int i0 = P.ng + bx * 64, j0 = P.ng + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = P.ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, P.ng + P.nz);
     const int sx = P.sx, sxy = P.sx * P.sy;
     const int col = j * sx + i;
     const double *const src[4] = {F.W[1], F.W[2], F.W[3], F.W[5]};
-    // halo cells this thread stages: y-halo rows h = ty, ty + 4, ... < 2R
-    // (tile row h, or h + 4 above the tile), and one x-halo cell (tid < 32 R:
-    // quantity tid / 8R, own row (tid / 2R) % 4, halo column tid % 2R)
-    constexpr int NYH = (2 * R + 3) / 4;
+    // halo cells this thread stages: y-halo rows h = ty, ty + TY, ... < 2R
+    // (tile row h, or h + TY above the tile), and one x-halo cell (tid < 8 R TY:
+    // quantity tid / (2R TY), own row (tid / 2R) % TY, halo column tid % 2R)
+    constexpr int NYH = (2 * R + TY - 1) / TY;
     int yh_row[NYH], yh_off[NYH];
 #pragma unroll
     for (int n = 0; n < NYH; ++n) {
-        const int h = ty + 4 * n;
-        const int tr = h < R ? h : h + 4;
+        const int h = ty + TY * n;
+        const int tr = h < R ? h : h + TY;
         yh_row[n] = h < 2 * R ? tr : -1;
         yh_off[n] = (j0 - R + tr) * sx + i;
     }
-    const bool xh_on = tid < 32 * R;
-    const int xh_q = xh_on ? tid / (8 * R) : 0;
-    const int xh_r = (tid / (2 * R)) % 4;
+    const bool xh_on = tid < 8 * R * TY;
+    const int xh_q = xh_on ? tid / (2 * R * TY) : 0;
+    const int xh_r = (tid / (2 * R)) % TY;
     const int xh_c = tid % (2 * R);
     const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
     const int xh_off = (j0 + xh_r) * sx + i0 - R + xh_tc;
@@ -919,41 +919,41 @@ __global__ __launch_bounds__(256) void k_vdiv_z(DevParams P, Fields F, double dt
 // arithmetic in the same order as k_vdiv_z<VIS, MODE, 0>: equal bit for
 // bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before a
 // barrier).
-template <int VIS, int MODE>
-__global__ __launch_bounds__(256) void k_vdiv_lds(DevParams P, Fields F, double dt, int seg_len, int gx, int gy,
+template <int VIS, int MODE, int TY>
+__global__ __launch_bounds__(64 * TY) void k_vdiv_lds(DevParams P, Fields F, double dt, int seg_len, int gx, int gy,
                                                   unsigned long long *flag, int dev_dt)
 {
     constexpr int R = VisReach<VIS>::R;
-    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
-    __shared__ double fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
+    constexpr int TC = 64 + 2 * R, TYR = TY + 2 * R;
+    __shared__ double fxt[2][4][TY][TC];    // x fluxes: tile rows, R-cell x halo
     __shared__ double fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
     const int ng = P.ng;
     const int t = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int i0 = ng + bx * 64, j0 = ng + by * 4;
+    const int i0 = ng + bx * 64, j0 = ng + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
     if (MODE != 0 && dev_dt) dt = cfl_dt(P, flag);
     const int sx = P.sx, sxy = P.sx * P.sy;
     const int col = j * sx + i;
-    // the x-halo cell of this thread (tid < 32 R): flux tid / 8R, tile row
-    // (tid / 2R) % 4, halo column tid % 2R
-    const bool xh_on = tid < 32 * R;
-    const int xh_q = xh_on ? tid / (8 * R) : 0;
-    const int xh_r = (tid / (2 * R)) % 4;
+    // the x-halo cell of this thread (tid < 8 R TY): flux tid / (2R TY), tile
+    // row (tid / 2R) % TY, halo column tid % 2R
+    const bool xh_on = tid < 8 * R * TY;
+    const int xh_q = xh_on ? tid / (2 * R * TY) : 0;
+    const int xh_r = (tid / (2 * R)) % TY;
     const int xh_c = tid % (2 * R);
     const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
     const int xh_off = (j0 + xh_r) * sx + wrapi(i0 - R + xh_tc, ng, P.nx);
-    // the y-halo rows of this thread: h = ty, ty + 4, ... < 2R, tile row h
-    // (below the tile) or h + 4 (above it)
-    constexpr int NYH = (2 * R + 3) / 4;
+    // the y-halo rows of this thread: h = ty, ty + TY, ... < 2R, tile row h
+    // (below the tile) or h + TY (above it)
+    constexpr int NYH = (2 * R + TY - 1) / TY;
     int yh_row[NYH], yh_off[NYH];
 #pragma unroll
     for (int n = 0; n < NYH; ++n) {
-        const int h = ty + 4 * n;
-        const int tr = h < R ? h : h + 4;
+        const int h = ty + TY * n;
+        const int tr = h < R ? h : h + TY;
         yh_row[n] = h < 2 * R ? tr : -1;
         yh_off[n] = wrapi(j0 - R + tr, ng, P.ny) * sx + i;
     }
@@ -995,7 +995,7 @@ __global__ __launch_bounds__(256) void k_vdiv_lds(DevParams P, Fields F, double 
             for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
             win[m][2 * R] = pz[m];
         }
-        double (*FX)[4][TC] = fxt[k & 1];
+        double (*FX)[TY][TC] = fxt[k & 1];
         double (*FY)[TYR][64] = fyt[k & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1337,21 +1337,29 @@ static hipError_t gradflux_vis(hipStream_t s, const DevParams &P, const Fields &
 }
 
 // the closure-aware variant only where an axis is not periodic
-// the LDS-staged form where it applies (lds != 0; periodic box, nx % 64 == 0,
-// ny % 4 == 0), else k_gradflux with its waves-per-EU floor minb
+template <int TY>
+static hipError_t gradflux_lds_vis(hipStream_t s, const DevParams &P, const Fields &F, int seg_len)
+{
+    const int gx = P.nx / 64, gy = P.ny / TY, gz = (P.nz + seg_len - 1) / seg_len;
+    const dim3 g(gx * gy * gz), b(64, TY);
+    switch (P.vis) {
+    case VIS_C8: hipLaunchKernelGGL((k_gradflux_lds<VIS_C8, TY>), g, b, 0, s, P, F, seg_len, gx, gy); break;
+    case VIS_C6: hipLaunchKernelGGL((k_gradflux_lds<VIS_C6, TY>), g, b, 0, s, P, F, seg_len, gx, gy); break;
+    case VIS_C4: hipLaunchKernelGGL((k_gradflux_lds<VIS_C4, TY>), g, b, 0, s, P, F, seg_len, gx, gy); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// the LDS-staged form where it applies (a periodic box with nx % 64 == 0):
+// tiles of lds = 8 rows where 8 divides ny, else of 4 where 4 does; lds = 0
+// or any other box: k_gradflux with its waves-per-EU floor minb
 hipError_t launch_gradflux(hipStream_t s, const DevParams &P, const Fields &F, int seg_len, int minb, int lds)
 {
     const bool bc = !(P.periodic[0] && P.periodic[1] && P.periodic[2]);
-    if (lds && !bc && P.nx % 64 == 0 && P.ny % 4 == 0) {
-        const int gx = P.nx / 64, gy = P.ny / 4, gz = (P.nz + seg_len - 1) / seg_len;
-        const dim3 b(64, 4);
-        switch (P.vis) {
-        case VIS_C8: hipLaunchKernelGGL((k_gradflux_lds<VIS_C8>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
-        case VIS_C6: hipLaunchKernelGGL((k_gradflux_lds<VIS_C6>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
-        case VIS_C4: hipLaunchKernelGGL((k_gradflux_lds<VIS_C4>), dim3(gx * gy * gz), b, 0, s, P, F, seg_len, gx, gy); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
+    if (lds && !bc && P.nx % 64 == 0) {
+        if (lds == 8 && P.ny % 8 == 0) return gradflux_lds_vis<8>(s, P, F, seg_len);
+        if (P.ny % 4 == 0) return gradflux_lds_vis<4>(s, P, F, seg_len);
     }
     if (minb == 2) return bc ? gradflux_vis<2, 1>(s, P, F, seg_len) : gradflux_vis<2, 0>(s, P, F, seg_len);
     return bc ? gradflux_vis<1, 1>(s, P, F, seg_len) : gradflux_vis<1, 0>(s, P, F, seg_len);
@@ -1384,34 +1392,43 @@ static hipError_t vdivz_vis(hipStream_t s, const DevParams &P, const Fields &F, 
 }
 
 // the closure-aware variant only where an axis is not periodic
-template <int VIS>
+template <int VIS, int TY>
 static void vdiv_lds_mode(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
                           unsigned long long *flag, int dev_dt)
 {
-    const int gx = P.nx / 64, gy = P.ny / 4, gz = (P.nz + seg - 1) / seg;
-    const dim3 g(gx * gy * gz), b(64, 4);
+    const int gx = P.nx / 64, gy = P.ny / TY, gz = (P.nz + seg - 1) / seg;
+    const dim3 g(gx * gy * gz), b(64, TY);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_vdiv_lds<VIS, 0>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
-    case 1: hipLaunchKernelGGL((k_vdiv_lds<VIS, 1>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
-    case 2: hipLaunchKernelGGL((k_vdiv_lds<VIS, 2>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
-    default: hipLaunchKernelGGL((k_vdiv_lds<VIS, 3>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    case 0: hipLaunchKernelGGL((k_vdiv_lds<VIS, 0, TY>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((k_vdiv_lds<VIS, 1, TY>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((k_vdiv_lds<VIS, 2, TY>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((k_vdiv_lds<VIS, 3, TY>), g, b, 0, s, P, F, dt, seg, gx, gy, flag, dev_dt); break;
     }
 }
 
-// the LDS-staged form where it applies (lds != 0; periodic box, nx % 64 == 0,
-// ny % 4 == 0), else k_vdiv_z
+template <int TY>
+static hipError_t vdiv_lds_vis(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
+                               unsigned long long *flag, int dev_dt)
+{
+    switch (P.vis) {
+    case VIS_C8: vdiv_lds_mode<VIS_C8, TY>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+    case VIS_C6: vdiv_lds_mode<VIS_C6, TY>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+    case VIS_C4: vdiv_lds_mode<VIS_C4, TY>(s, P, F, mode, dt, seg, flag, dev_dt); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// the LDS-staged form where it applies (a periodic box with nx % 64 == 0):
+// tiles of lds = 8 rows where 8 divides ny, else of 4 where 4 does; lds = 0
+// or any other box: k_vdiv_z
 hipError_t launch_vdiv_z(hipStream_t s, const DevParams &P, const Fields &F, int mode, double dt, int seg,
                          unsigned long long *flag, int dev_dt, int lds)
 {
     const bool per = P.periodic[0] && P.periodic[1] && P.periodic[2];
-    if (lds && per && P.nx % 64 == 0 && P.ny % 4 == 0) {
-        switch (P.vis) {
-        case VIS_C8: vdiv_lds_mode<VIS_C8>(s, P, F, mode, dt, seg, flag, dev_dt); break;
-        case VIS_C6: vdiv_lds_mode<VIS_C6>(s, P, F, mode, dt, seg, flag, dev_dt); break;
-        case VIS_C4: vdiv_lds_mode<VIS_C4>(s, P, F, mode, dt, seg, flag, dev_dt); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
+    if (lds && per && P.nx % 64 == 0) {
+        if (lds == 8 && P.ny % 8 == 0) return vdiv_lds_vis<8>(s, P, F, mode, dt, seg, flag, dev_dt);
+        if (P.ny % 4 == 0) return vdiv_lds_vis<4>(s, P, F, mode, dt, seg, flag, dev_dt);
     }
     if (per) return vdivz_vis<0>(s, P, F, mode, dt, seg, flag, dev_dt);
     return vdivz_vis<1>(s, P, F, mode, dt, seg, flag, dev_dt);

@@ -478,25 +478,30 @@ def test_weno5_sweeps_rhs_bitwise(gpu, case, n, scale, cfg):
 
 
 @pytest.mark.parametrize("vis", ["C8th", "C6th", "C4th"])
-def test_fp64_viscous_pass_forms_bitwise(gpu, vis, monkeypatch):
+@pytest.mark.parametrize("shape", [(64, 32, 24), (64, 20, 16)], ids=["ny32", "ny20"])
+def test_fp64_viscous_pass_forms_bitwise(gpu, vis, shape, monkeypatch):
     """The FP64 viscous passes on periodic boxes with nx % 64 == 0 and
     ny % 4 == 0 stage each plane's x / y neighbourhood in LDS: the gradient/
     flux pass (NS3D_GF64_LDS=1, the default; 0 loads the stencils through
-    the caches) and the divergence/stage pass (NS3D_VD64_LDS, likewise).
-    All four combinations evaluate ViscousFlux.cpp:240-386 and
+    the caches) and the divergence/stage pass (NS3D_VD64_LDS, likewise), in
+    tiles of 64 x 4 or 64 x 8 cells (NS3D_FP64_TILE_Y).  All the
+    combinations evaluate ViscousFlux.cpp:240-386 and
     range_kutta.cpp:11-161 in the same order on the same values, so the RHS,
     the divergence of stored viscous fluxes, two fixed-dt RK3 steps and
     three CFL steps must agree bit for bit.  A 64 x 32 x 24 box (a different
-    spacing per axis) in 2 z segments of unequal length."""
-    nx, ny, nz = 64, 32, 24
+    spacing per axis) in 2 z segments of unequal length, and a 64 x 20 x 16
+    box, whose 20 rows the 8-row tiles do not divide (those forms then run
+    in 4-row tiles)."""
+    nx, ny, nz = shape
     rng = np.random.default_rng(5)
     st, Ma = fields.hit(nz, k0=2, Mt=0.3, seed=13)
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    for form in (("0", "0"), ("1", "0"), ("0", "1"), ("1", "1")):
+    for form in (("0", "0", "4"), ("1", "0", "4"), ("0", "1", "4"), ("1", "1", "4"), ("1", "1", "8")):
         monkeypatch.setenv("NS3D_GF64_LDS", form[0])
         monkeypatch.setenv("NS3D_VD64_LDS", form[1])
+        monkeypatch.setenv("NS3D_FP64_TILE_Y", form[2])
         monkeypatch.setenv("NS3D_SEG_Z", "16")
         p, g = ns.make_params((nx, ny, nz), L=(2 * np.pi), Ma=Ma, Re=100.0, recon="WENO5",
                               vis_scheme=vis, Ly=np.pi, Lz=0.75 * np.pi)
@@ -512,7 +517,7 @@ def test_fp64_viscous_pass_forms_bitwise(gpu, vis, monkeypatch):
         u5 = [s.interior(a).copy() for a in s.conserved()]
         out[form] = (r, dv, u2, u5)
         s.close()
-    ref = out[("0", "0")]
+    ref = out[("0", "0", "4")]
     for form, got in out.items():
         for part in range(4):
             for m, (a, b) in enumerate(zip(got[part], ref[part])):
