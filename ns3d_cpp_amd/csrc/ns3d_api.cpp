@@ -923,6 +923,10 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     if (const char *env = std::getenv("NS3D_ADJ_MARCH")) d.adj_lines = std::atoi(env) ? 0 : 1;
     d.adj_zmarch = 1;
     if (const char *env = std::getenv("NS3D_ADJ_ZMARCH")) d.adj_zmarch = std::atoi(env) != 0;
+    // NS3D_M_TILE_Y = 4 or 8: thread rows of the mixed LDS-staged gradient
+    // and stage passes (8 where it divides the rows the pass tiles)
+    d.m_ty = 8;
+    if (const char *env = std::getenv("NS3D_M_TILE_Y")) d.m_ty = std::atoi(env) == 4 ? 4 : 8;
     if (const char *env = std::getenv("NS3D_GRADFLUX_MINB")) c->gradflux_minb = std::atoi(env) == 2 ? 2 : 1;
     // NS3D_GF64_LDS / NS3D_VD64_LDS = 0 selects the cached forms;
     // NS3D_FP64_TILE_Y = 4 or 8 the LDS forms' tile height (8: 5-9 % faster

@@ -29,6 +29,7 @@ struct DevParams {
     double rho_ref, p_ref;   // reference state of the mixed-precision shadow
     int adj_lines;           // 1: adjoint sweeps may hold whole lines in LDS
     int adj_zmarch;          // 1: the viscous adjoint kernels march z segments (periodic boxes)
+    int m_ty;                // 4 or 8: thread rows of the mixed LDS-staged viscous passes
     int lds_max;             // LDS bytes one workgroup may use on this device
     double cfl;              // CFL number of the device time step
     int check_mask;          // bit s: stage s scans for non-physical cells (the third only)

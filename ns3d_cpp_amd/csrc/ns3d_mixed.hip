@@ -942,45 +942,46 @@ __device__ __forceinline__ void gradflux_tile_rows2(const DevParams &P, const MF
 }
 
 // The two-row pass with each entering plane's x / y neighbourhood staged in
-// LDS: a 64 x 4 workgroup (64 x 8 cells) writes its plane tile with an R-cell
+// LDS: a 64 x TY workgroup (64 x 2TY cells) writes its plane tile with an R-cell
 // halo in x (own rows) and y (64 columns) into one of two LDS buffers, one
 // barrier per plane, and takes the x / y derivatives from there.  The own
 // values are the z window's entering plane; the halo costs 2R + 1 loads per
 // thread and plane instead of the 2 (2R + 1) + 2R + 2 stencil loads per
 // quantity of the cached form.  The next plane's values are loaded before the
 // derivative and flux arithmetic of this one, which hides their latency.
-// Needs nx % 64 == 0 and ny % 8 == 0 (no thread leaves before a barrier).
-template <int VIS>
-__global__ __launch_bounds__(256) void km_gradflux_lds(DevParams P, MFields M, int seg_len, int gx, int gy)
+// TY = 4 or 8 thread rows (taller tiles fetch fewer halo rows per row).
+// Needs nx % 64 == 0 and ny % 2TY == 0 (no thread leaves before a barrier).
+template <int VIS, int TY>
+__global__ __launch_bounds__(64 * TY) void km_gradflux_lds(DevParams P, MFields M, int seg_len, int gx, int gy)
 {
     constexpr int R = VisReach<VIS>::R;
-    constexpr int TR = 8 + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
+    constexpr int TR = 2 * TY + 2 * R, TC = 64 + 2 * R;   // tile rows, columns
     __shared__ float tile[2][4][TR][TC];
     const int t = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int i0 = P.ng + bx * 64, j0 = P.ng + by * 8;
+    const int i0 = P.ng + bx * 64, j0 = P.ng + by * 2 * TY;
     const int i = i0 + tx, j = j0 + 2 * ty;
     const int k0 = P.ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, P.ng + P.nz);
     const int sx = P.sx, sxy = P.sx * P.sy;
     const int col = j * sx + i;
     const float *const src[4] = {M.S[1], M.S[2], M.S[3], M.S[5]};
-    // halo cells this thread stages: y-halo rows h = ty, ty + 4, ... < 2R
-    // (tile row h or h + 8), and one x-halo cell (tid < 64 R: quantity
-    // tid / (16 R), own row (tid / 2R) % 8, halo column tid % 2R)
-    constexpr int NYH = (2 * R + 3) / 4;
+    // halo cells this thread stages: y-halo rows h = ty, ty + TY, ... < 2R
+    // (tile row h or h + 2 TY), and one x-halo cell (tid < 16 R TY: quantity
+    // tid / (4 R TY), own row (tid / 2R) % 2 TY, halo column tid % 2R)
+    constexpr int NYH = (2 * R + TY - 1) / TY;
     int yh_row[NYH], yh_off[NYH];
 #pragma unroll
     for (int n = 0; n < NYH; ++n) {
-        const int h = ty + 4 * n;
-        const int tr = h < R ? h : h + 8;
+        const int h = ty + TY * n;
+        const int tr = h < R ? h : h + 2 * TY;
         yh_row[n] = h < 2 * R ? tr : -1;
         yh_off[n] = (j0 - R + tr) * sx + i;
     }
-    const bool xh_on = tid < 64 * R;
-    const int xh_q = xh_on ? tid / (16 * R) : 0;
-    const int xh_r = (tid / (2 * R)) % 8;
+    const bool xh_on = tid < 16 * R * TY;
+    const int xh_q = xh_on ? tid / (4 * R * TY) : 0;
+    const int xh_r = (tid / (2 * R)) % (2 * TY);
     const int xh_c = tid % (2 * R);
     const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
     const int xh_off = (j0 + xh_r) * sx + i0 - R + xh_tc;
@@ -1259,8 +1260,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// The LDS-staged stage pass on a periodic box, one z segment of a 64 x 4
-// tile per workgroup (km_vdiv_lds; with ZS, km_vdiv_zs).
+// The LDS-staged stage pass on a periodic box, one z segment of a 64 x TY
+// tile per workgroup (TY = 4 or 8) (km_vdiv_lds; with ZS, km_vdiv_zs).
 //
 // km_vdiv_z's arithmetic with each plane's x / y stencil fluxes staged in
 // LDS: the workgroup writes its plane's x fluxes with an R-cell x halo and
@@ -1271,13 +1272,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // before this plane's arithmetic, so the loads of a plane overlap the
 // arithmetic of the previous one: about 35 loads per cell instead of 91.
 // The same arithmetic in the same order as km_vdiv_z<VIS, MODE, 0>: equal
-// bit for bit.  Needs nx % 64 == 0 and ny % 4 == 0 (no thread leaves before
+// bit for bit.  Needs nx % 64 == 0 and ny % TY == 0 (no thread leaves before
 // a barrier).
 // IMG (MODE 1..3): the stage also writes the new shadow's periodic ghost
 // images (the face layers of the fill pass), so that pass can be skipped.
 // ZS (MODE 1..3, with IMG): the same stage's z sweep folded in.  Each thread
 // also marches its column's z sweep (the state and arithmetic of
-// km_sweep<2>, its minus-side ring 256 lanes wide), which at plane k yields
+// km_sweep<2>, its minus-side ring 64 TY lanes wide), which at plane k yields
 // that cell's z flux difference, added to the accumulator of the x and y
 // sweeps as km_sweep<2> adds it.  The accumulator's z update is then
 // neither stored nor read back (40 B per cell), and the sweep's arithmetic
@@ -1285,13 +1286,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // shadow M.S across the segment's ends, which other workgroups' stages
 // overwrite: the new shadow goes to So (a second buffer, the next stage's
 // input).  Equal bit for bit to km_sweep<2> followed by the stage.
-template <int VIS, int MODE, bool IMG, bool ZS, int RECON, int FVS>
+template <int VIS, int MODE, bool IMG, bool ZS, int RECON, int FVS, int TY>
 __device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields &M, const Shadow &So, double dt,
                                               int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
     constexpr int R = VisReach<VIS>::R;
-    constexpr int TC = 64 + 2 * R, TYR = 4 + 2 * R;
-    __shared__ float fxt[2][4][4][TC];     // x fluxes: tile rows, R-cell x halo
+    constexpr int TC = 64 + 2 * R, TYR = TY + 2 * R;
+    __shared__ float fxt[2][4][TY][TC];    // x fluxes: tile rows, R-cell x halo
     __shared__ float fyt[2][4][TYR][64];   // y fluxes: tile columns, R-row y halo
     // the z sweep (km_sweep<2>): per-lane ring of the minus side's window
     constexpr int S = Stencil<RECON>::S;
@@ -1300,14 +1301,14 @@ __device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields 
     constexpr int L = S - T0;
     constexpr int R0 = 5;
     constexpr bool CARRY = ZS && RECON == R_WENO5 && FVS != FVS_SWE;
-    constexpr int NL = ZS ? 256 : 1;
+    constexpr int NL = ZS ? 64 * TY : 1;
     __shared__ float ring[ZS ? L : 1][5][NL];
     __shared__ float uring[CARRY ? 2 : 1][5][NL];
     const int ng = P.ng;
     const int t = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int i0 = ng + bx * 64, j0 = ng + by * 4;
+    const int i0 = ng + bx * 64, j0 = ng + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = ng + bz * seg_len;
     const int k1 = min(k0 + seg_len, ng + P.nz);
@@ -1439,22 +1440,22 @@ __device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields 
     }
 
     // ---- the stage pass
-    // the x-halo cell of this thread (tid < 32 R): flux tid / 8R, tile row
-    // (tid / 2R) % 4, halo column tid % 2R
-    const bool xh_on = tid < 32 * R;
-    const int xh_q = xh_on ? tid / (8 * R) : 0;
-    const int xh_r = (tid / (2 * R)) % 4;
+    // the x-halo cell of this thread (tid < 8 R TY): flux tid / (2R TY), tile
+    // row (tid / 2R) % TY, halo column tid % 2R
+    const bool xh_on = tid < 8 * R * TY;
+    const int xh_q = xh_on ? tid / (2 * R * TY) : 0;
+    const int xh_r = (tid / (2 * R)) % TY;
     const int xh_c = tid % (2 * R);
     const int xh_tc = xh_c < R ? xh_c : 64 + xh_c;
     const int xh_off = (j0 + xh_r) * sx + wrapi_m(i0 - R + xh_tc, ng, P.nx);
-    // the y-halo rows of this thread: h = ty, ty + 4, ... < 2R, tile row h
-    // (below the tile) or h + 4 (above it)
-    constexpr int NYH = (2 * R + 3) / 4;
+    // the y-halo rows of this thread: h = ty, ty + TY, ... < 2R, tile row h
+    // (below the tile) or h + TY (above it)
+    constexpr int NYH = (2 * R + TY - 1) / TY;
     int yh_row[NYH], yh_off[NYH];
 #pragma unroll
     for (int n = 0; n < NYH; ++n) {
-        const int h = ty + 4 * n;
-        const int tr = h < R ? h : h + 4;
+        const int h = ty + TY * n;
+        const int tr = h < R ? h : h + TY;
         yh_row[n] = h < 2 * R ? tr : -1;
         yh_off[n] = wrapi_m(j0 - R + tr, ng, P.ny) * sx + i;
     }
@@ -1502,7 +1503,7 @@ __device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields 
             for (int o = 0; o < 2 * R; ++o) win[m][o] = win[m][o + 1];
             win[m][2 * R] = pz[m];
         }
-        float (*FX)[4][TC] = fxt[k & 1];
+        float (*FX)[TY][TC] = fxt[k & 1];
         float (*FY)[TYR][64] = fyt[k & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) FX[q][ty][R + tx] = pfx[q];
@@ -1571,19 +1572,19 @@ __device__ __forceinline__ void vdiv_lds_pass(const DevParams &P, const MFields 
 
 // Occupancy 2 for both: at the default target of 4 waves the compiler waits
 // for the ~35 loads of a plane in small groups
-template <int VIS, int MODE, bool IMG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
+template <int VIS, int MODE, bool IMG, int TY>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_lds(
     DevParams P, MFields M, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
-    vdiv_lds_pass<VIS, MODE, IMG, false, R_WENO5, FVS_SW>(P, M, Shadow{}, dt, seg_len, gx, gy, flag, dev_dt);
+    vdiv_lds_pass<VIS, MODE, IMG, false, R_WENO5, FVS_SW, TY>(P, M, Shadow{}, dt, seg_len, gx, gy, flag, dev_dt);
 }
 
-template <int VIS, int MODE, int RECON, int FVS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_zs(
+template <int VIS, int MODE, int RECON, int FVS, int TY>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(2, 2))) void km_vdiv_zs(
     DevParams P, MFields M, Shadow So, double dt, int seg_len, int gx, int gy, unsigned long long *flag, int dev_dt)
 {
     static_assert(MODE != 0, "the folded z sweep belongs to a stage");
-    vdiv_lds_pass<VIS, MODE, true, true, RECON, FVS>(P, M, So, dt, seg_len, gx, gy, flag, dev_dt);
+    vdiv_lds_pass<VIS, MODE, true, true, RECON, FVS, TY>(P, M, So, dt, seg_len, gx, gy, flag, dev_dt);
 }
 
 // ------------------------------------------------------ state conversion
@@ -1936,12 +1937,23 @@ hipError_t launch_m_gradflux(hipStream_t s, const DevParams &P, const MFields &M
     dim3 b(bx, by, 1);
     const int gx = (P.nx + bx - 1) / bx, gz = (P.nz + seg_len - 1) / seg_len;
     if (rows == 3 && per && bx == 64 && by == 4 && P.nx % 64 == 0 && P.ny % 8 == 0) {
+        // 64 x 8 threads (16 rows) where that tiles y and P.m_ty asks for it
+        if (P.m_ty == 8 && P.ny % 16 == 0) {
+            const int gy = P.ny / 16, n = gx * gy * gz;
+            const dim3 b8(64, 8, 1);
+            switch (P.vis) {
+            case VIS_C8: hipLaunchKernelGGL((km_gradflux_lds<VIS_C8, 8>), dim3(n), b8, 0, s, P, M, seg_len, gx, gy); break;
+            case VIS_C6: hipLaunchKernelGGL((km_gradflux_lds<VIS_C6, 8>), dim3(n), b8, 0, s, P, M, seg_len, gx, gy); break;
+            default: hipLaunchKernelGGL((km_gradflux_lds<VIS_C4, 8>), dim3(n), b8, 0, s, P, M, seg_len, gx, gy); break;
+            }
+            return hipGetLastError();
+        }
         const int gy = P.ny / 8;
         const int n = gx * gy * gz;
         switch (P.vis) {
-        case VIS_C8: hipLaunchKernelGGL((km_gradflux_lds<VIS_C8>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
-        case VIS_C6: hipLaunchKernelGGL((km_gradflux_lds<VIS_C6>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
-        default: hipLaunchKernelGGL((km_gradflux_lds<VIS_C4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        case VIS_C8: hipLaunchKernelGGL((km_gradflux_lds<VIS_C8, 4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        case VIS_C6: hipLaunchKernelGGL((km_gradflux_lds<VIS_C6, 4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
+        default: hipLaunchKernelGGL((km_gradflux_lds<VIS_C4, 4>), dim3(n), b, 0, s, P, M, seg_len, gx, gy); break;
         }
         return hipGetLastError();
     }
@@ -1993,18 +2005,30 @@ bool m_vdiv_cfl_next(const DevParams &P, int bx, int by)
     return (bx * by) % 64 == 0 && P.nx % bx == 0 && P.ny % by == 0;
 }
 
+// thread rows of the LDS-staged stage pass: 8 where P.m_ty asks for it and
+// 8 divides ny, else 4
+static int m_vdiv_ty(const DevParams &P) { return P.m_ty == 8 && P.ny % 8 == 0 ? 8 : 4; }
+
+template <int VIS, bool IMG, int TY>
+static void m_vdiv_lds_ty(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
+                          unsigned long long *flag, int dev_dt)
+{
+    const int gx = P.nx / 64, gy = P.ny / TY, n = gx * gy * ((P.nz + seg - 1) / seg);
+    dim3 b(64, TY, 1);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((km_vdiv_lds<VIS, 0, false, TY>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 1: hipLaunchKernelGGL((km_vdiv_lds<VIS, 1, IMG, TY>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    case 2: hipLaunchKernelGGL((km_vdiv_lds<VIS, 2, IMG, TY>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    default: hipLaunchKernelGGL((km_vdiv_lds<VIS, 3, IMG, TY>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
+    }
+}
+
 template <int VIS, bool IMG>
 static void m_vdiv_lds(hipStream_t s, const DevParams &P, const MFields &M, int mode, double dt, int seg,
                        unsigned long long *flag, int dev_dt)
 {
-    const int gx = P.nx / 64, gy = P.ny / 4, n = gx * gy * ((P.nz + seg - 1) / seg);
-    dim3 b(64, 4, 1);
-    switch (mode) {
-    case 0: hipLaunchKernelGGL((km_vdiv_lds<VIS, 0, false>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    case 1: hipLaunchKernelGGL((km_vdiv_lds<VIS, 1, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    case 2: hipLaunchKernelGGL((km_vdiv_lds<VIS, 2, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    default: hipLaunchKernelGGL((km_vdiv_lds<VIS, 3, IMG>), dim3(n), b, 0, s, P, M, dt, seg, gx, gy, flag, dev_dt); break;
-    }
+    if (m_vdiv_ty(P) == 8) m_vdiv_lds_ty<VIS, IMG, 8>(s, P, M, mode, dt, seg, flag, dev_dt);
+    else m_vdiv_lds_ty<VIS, IMG, 4>(s, P, M, mode, dt, seg, flag, dev_dt);
 }
 
 // whether launch_m_vdiv takes the LDS-staged form for these arguments
@@ -2023,15 +2047,23 @@ bool m_vdiv_zs_form(const DevParams &P, int bx, int by, int lds, int img)
            (P.vis == VIS_C8 || P.vis == VIS_C6 || P.vis == VIS_C4);
 }
 
+template <int VIS, int RECON, int FVS, int TY>
+static void m_vdiv_zs_ty(hipStream_t s, const DevParams &P, const MFields &M, const Shadow &So, int mode, double dt,
+                         int seg, unsigned long long *flag, int dev_dt)
+{
+    const int gx = P.nx / 64, gy = P.ny / TY, n = gx * gy * ((P.nz + seg - 1) / seg);
+    const dim3 b(64, TY, 1);
+    if (mode == 1) hipLaunchKernelGGL((km_vdiv_zs<VIS, 1, RECON, FVS, TY>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
+    else if (mode == 2) hipLaunchKernelGGL((km_vdiv_zs<VIS, 2, RECON, FVS, TY>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
+    else hipLaunchKernelGGL((km_vdiv_zs<VIS, 3, RECON, FVS, TY>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
+}
+
 template <int VIS, int RECON, int FVS>
 static void m_vdiv_zs(hipStream_t s, const DevParams &P, const MFields &M, const Shadow &So, int mode, double dt,
                       int seg, unsigned long long *flag, int dev_dt)
 {
-    const int gx = P.nx / 64, gy = P.ny / 4, n = gx * gy * ((P.nz + seg - 1) / seg);
-    const dim3 b(64, 4, 1);
-    if (mode == 1) hipLaunchKernelGGL((km_vdiv_zs<VIS, 1, RECON, FVS>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
-    else if (mode == 2) hipLaunchKernelGGL((km_vdiv_zs<VIS, 2, RECON, FVS>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
-    else hipLaunchKernelGGL((km_vdiv_zs<VIS, 3, RECON, FVS>), dim3(n), b, 0, s, P, M, So, dt, seg, gx, gy, flag, dev_dt);
+    if (m_vdiv_ty(P) == 8) m_vdiv_zs_ty<VIS, RECON, FVS, 8>(s, P, M, So, mode, dt, seg, flag, dev_dt);
+    else m_vdiv_zs_ty<VIS, RECON, FVS, 4>(s, P, M, So, mode, dt, seg, flag, dev_dt);
 }
 
 template <int VIS, int RECON>

@@ -170,7 +170,8 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
     writing the new shadow's periodic ghost images itself (no fill pass) or
     not.  The stage pass can also fold in the z sweep, writing the new shadow
     into a second buffer (the two alternate, each with its own step graph).
-    All of them evaluate the same arithmetic on the same values, so the RHS,
+    The LDS-staged passes run in workgroups of 64 x 8 threads or 64 x 4
+    (NS3D_M_TILE_Y).  All of them evaluate the same arithmetic on the same values, so the RHS,
     two RK3 steps at a fixed dt, three graph-replayed steps, four CFL steps
     with the device time step (the third stage reducing the next CFL
     minimum) and the downloaded primitives, ghost layers included, must
@@ -178,8 +179,10 @@ def test_viscous_pass_variants_bitwise(gpu, vis, monkeypatch):
     axis) in 2 z segments of unequal length: tile shapes and segment ends
     that the 256^3 workload does not exercise."""
     out = _mixed_forms(vis, CFG, monkeypatch, [("1", "0", "0", "0"), ("2", "0", "0", "0"), ("3", "0", "0", "0"),
+                                               ("3", "0", "0", "0", "4"),
                                                ("1", "1", "0", "0"), ("3", "1", "0", "0"), ("3", "1", "1", "0"),
-                                               ("3", "1", "1", "1")])
+                                               ("3", "1", "1", "1"), ("3", "1", "0", "0", "4"),
+                                               ("3", "1", "1", "1", "4")])
     _assert_forms_equal(out)
 
 
@@ -190,11 +193,13 @@ def _mixed_forms(vis, cfg, monkeypatch, forms):
     shape = (nz, ny, nx)
     x = [np.resize(a, shape) * (1.0 + 0.01 * rng.standard_normal(shape)) for a in st]
     out = {}
-    for rows, lds, img, zf in forms:
+    for form in forms:
+        rows, lds, img, zf = form[:4]
         monkeypatch.setenv("NS3D_GF_ROWS", rows)
         monkeypatch.setenv("NS3D_VD_LDS", lds)
         monkeypatch.setenv("NS3D_VD_IMG", img)
         monkeypatch.setenv("NS3D_ZFUSE", zf)
+        monkeypatch.setenv("NS3D_M_TILE_Y", form[4] if len(form) > 4 else "8")
         monkeypatch.setenv("NS3D_SEG_Z", "16")
         p, g = ns.make_params((nx, ny, nz), precision="mixed", Ma=Ma, **dict(cfg, vis_scheme=vis))
         s = ns.Solver(p, g)
@@ -209,7 +214,7 @@ def _mixed_forms(vis, cfg, monkeypatch, forms):
         u2 = [s.interior(a).copy() for a in s.conserved()]
         w = list(s.primitive().values())
         s.close()
-        out[(rows, lds, img, zf)] = (r + u + u1 + u2 + w, t)
+        out[tuple(form)] = (r + u + u1 + u2 + w, t)
     return out
 
 
@@ -230,7 +235,8 @@ def test_z_sweep_folded_into_stage_bitwise(gpu, recon, fvs, eps, monkeypatch):
     sweep and stage kernels, bit for bit, for the splits and
     reconstructions it is built for."""
     cfg = dict(CFG, recon=recon, fvs_type=fvs, sw_eps=eps)
-    out = _mixed_forms("C8th", cfg, monkeypatch, [("3", "1", "1", "0"), ("3", "1", "1", "1")])
+    out = _mixed_forms("C8th", cfg, monkeypatch, [("3", "1", "1", "0"), ("3", "1", "1", "1"),
+                                                  ("3", "1", "1", "1", "4")])
     _assert_forms_equal(out)
 
 
