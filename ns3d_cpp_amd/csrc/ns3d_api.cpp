@@ -908,8 +908,10 @@ int ns3d_create(const ns3d_params *pin, const ns3d_grid *gin, ns3d_ctx **out)
     const int nd[3] = {g.nx, g.ny, g.nz};
     for (int dd = 0; dd < 3; ++dd) c->seg_len[dd] = pick_seg(nd[dd], ncol[dd]);
     {
-        // z segments for the (i, j)-column kernels: ~8 waves per SIMD
-        const long long want = 256LL * 4 * 8 * 64;
+        // z segments for the (i, j)-column kernels: ~4 waves per SIMD of
+        // columns (at 256^3 64-plane segments: the mixed step 4.83 ms with
+        // 32, 4.71 with 64, 5.13 with 128; FP64 12.1-12.3 / 12.07 / 12.07)
+        const long long want = 256LL * 4 * 4 * 64;
         const long long cols = (long long)g.nx * g.ny;
         long long nseg = (want + cols - 1) / cols;
         long long maxseg = g.nz / 16 > 0 ? g.nz / 16 : 1;
